@@ -1,0 +1,120 @@
+"""ctypes binding of ``include/acfdtd_hip.h``: the descriptor mirror and the loaded library.
+
+The reference binds its C loop the same way (``src/ac_fdtd/c_backend.py:83-157``): a ctypes
+``Structure`` mirrored field for field and checked against the library's own ``sizeof``.
+
+torch is imported before the library is opened on purpose: torch ships its own HIP runtime
+(soname ``libamdhip64.so.7``), and a library opened afterwards binds to that same copy, so
+device pointers and streams from torch tensors are valid in the library's calls.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch  # noqa: F401  (must precede the dlopen below; see module docstring)
+
+from .build import LIBRARY
+
+__all__ = ["ABI_VERSION", "COMPENSATED", "Desc", "MAX_RELAXATION", "check", "library"]
+
+ABI_VERSION = 1
+MAX_RELAXATION = 4
+WALL_CODES = 64
+COMPENSATED = 1
+
+_c = ctypes
+_ptr = _c.c_void_p
+
+
+class Desc(_c.Structure):
+    """Mirror of ``acfdtd_hip_desc``."""
+
+    _fields_ = [
+        ("abi_version", _c.c_uint32),
+        ("struct_size", _c.c_uint32),
+        ("nx", _c.c_int32),
+        ("ny", _c.c_int32),
+        ("nz", _c.c_int32),
+        ("real_bytes", _c.c_int32),
+        ("n_relax", _c.c_int32),
+        ("has_layer", _c.c_int32),
+        ("n_sources", _c.c_int32),
+        ("n_receivers", _c.c_int32),
+        ("flags", _c.c_uint32),
+        ("tile_j", _c.c_int32),
+        ("chunk_x", _c.c_int32),
+        ("reserved", _c.c_int32),
+        ("vel_coef", _c.c_double),
+        ("pres_coef", _c.c_double),
+        ("src_coef", _c.c_double),
+        ("relax_scale", _c.c_double),
+        ("relax_decay", _c.c_double * MAX_RELAXATION),
+        ("relax_mean", _c.c_double * MAX_RELAXATION),
+        ("relax_gain", _c.c_double * MAX_RELAXATION),
+        ("wall_mask", _c.c_uint64),
+        ("wall_table", _ptr),
+        ("p", _ptr),
+        ("vx", _ptr),
+        ("vy", _ptr),
+        ("vz", _ptr),
+        ("psi", _ptr),
+        ("cell", _ptr * 3),
+        ("face", _ptr * 3),
+        ("source_cell", _ptr),
+        ("source_length", _ptr),
+        ("source_signal", _ptr),
+        ("source_stride", _c.c_int64),
+        ("receiver_cell", _ptr),
+        ("recording", _ptr),
+        ("recording_stride", _c.c_int64),
+    ]
+
+
+_library: ctypes.CDLL | None = None
+
+
+def library() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it was never built or does not match this binding."""
+    global _library
+    if _library is not None:
+        return _library
+    if not LIBRARY.exists():
+        raise RuntimeError(
+            f"{LIBRARY} is not built; run `python -m ac_fdtd_amd.build` "
+            "(or __graft_entry__.build()) first"
+        )
+    lib = ctypes.CDLL(str(LIBRARY))
+    lib.acfdtd_hip_struct_size.restype = _c.c_size_t
+    lib.acfdtd_hip_abi_version.restype = _c.c_int
+    lib.acfdtd_hip_supports_real.restype = _c.c_int
+    lib.acfdtd_hip_supports_real.argtypes = [_c.c_int]
+    lib.acfdtd_hip_last_error.restype = _c.c_char_p
+    lib.acfdtd_hip_run.restype = _c.c_int
+    lib.acfdtd_hip_run.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _c.c_int64, _ptr]
+    lib.acfdtd_hip_velocity_sweep.restype = _c.c_int
+    lib.acfdtd_hip_velocity_sweep.argtypes = [_c.POINTER(Desc), _ptr]
+    lib.acfdtd_hip_pressure_sweep.restype = _c.c_int
+    lib.acfdtd_hip_pressure_sweep.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr]
+    lib.acfdtd_hip_energy_workspace_bytes.restype = _c.c_size_t
+    lib.acfdtd_hip_energy_workspace_bytes.argtypes = [_c.POINTER(Desc)]
+    lib.acfdtd_hip_energy.restype = _c.c_int
+    lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
+    if lib.acfdtd_hip_abi_version() != ABI_VERSION:
+        raise RuntimeError(
+            f"library ABI {lib.acfdtd_hip_abi_version()} does not match binding {ABI_VERSION}"
+        )
+    if lib.acfdtd_hip_struct_size() != ctypes.sizeof(Desc):
+        raise RuntimeError(
+            "the ctypes mirror of acfdtd_hip_desc no longer matches the library: "
+            f"{ctypes.sizeof(Desc)} bytes here, {lib.acfdtd_hip_struct_size()} there"
+        )
+    _library = lib
+    return lib
+
+
+def check(status: int, what: str) -> None:
+    """Raise ``RuntimeError`` with the library's message for a non-zero status."""
+    if status != 0:
+        message = library().acfdtd_hip_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({status}): {message}")

@@ -1,0 +1,71 @@
+"""Closed forms for the rigid box: modal frequencies and an exact single-mode initial state.
+
+Follows the reference's ``src/ac_fdtd/analytic.py:42-125``. The cosine mode shapes are exact
+eigenvectors of the discrete rigid-wall operator, so a run started from
+:func:`mode_initial_state` stays ``cos(2 pi f_d t) * Phi`` at :func:`discrete_mode_frequency`
+to round-off — the strongest whole-scheme check available.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .grid import Grid
+from .medium import Medium
+
+__all__ = [
+    "discrete_mode_frequency",
+    "mode_frequency",
+    "mode_initial_state",
+    "mode_pressure_field",
+    "wavenumbers",
+]
+
+
+def mode_frequency(lengths, mode, sound_speed: float) -> float:
+    """``(c/2) sqrt(sum (n_a / L_a)^2)`` (analytic.py:42-57)."""
+    return 0.5 * sound_speed * math.sqrt(sum((n / length) ** 2 for n, length in zip(mode, lengths)))
+
+
+def wavenumbers(lengths, mode) -> tuple[float, float, float]:
+    return tuple(n * math.pi / length for n, length in zip(mode, lengths))
+
+
+def discrete_mode_frequency(grid: Grid, mode, medium: Medium, dt: float) -> float:
+    """``asin((c dt / dx) sqrt(sum sin^2(k dx / 2))) / (pi dt)`` (analytic.py:66-87)."""
+    dx = grid.dx
+    spatial = math.sqrt(sum(math.sin(k * dx / 2.0) ** 2 for k in wavenumbers(grid.lengths, mode)))
+    argument = medium.sound_speed * dt / dx * spatial
+    if argument > 1.0:
+        raise ValueError(
+            f"mode {mode} is not representable: dispersion argument {argument:.6f} > 1, "
+            "which means this time step is unstable for this grid"
+        )
+    return math.asin(argument) / (math.pi * dt)
+
+
+def mode_pressure_field(grid: Grid, mode) -> np.ndarray:
+    """Unit-amplitude product of cosines at cell centres."""
+    field = np.ones(grid.shape)
+    for axis, (n, length) in enumerate(zip(mode, grid.lengths)):
+        shape = [1, 1, 1]
+        shape[axis] = -1
+        field = field * np.cos(n * math.pi * grid.cell_centres(axis) / length).reshape(shape)
+    return field
+
+
+def mode_initial_state(grid: Grid, mode, medium: Medium, dt: float, amplitude: float = 1.0):
+    """``(p^0, v^{-1/2})`` with ``v^{-1/2} = (dt / (2 rho)) grad p^0`` (analytic.py:99-125)."""
+    pressure = amplitude * mode_pressure_field(grid, mode)
+    scale = dt / (2.0 * medium.density * grid.dx)
+    velocity = []
+    for axis in range(3):
+        component = np.zeros(grid.velocity_shape(axis))
+        inner = tuple(slice(1, -1) if a == axis else slice(None) for a in range(3))
+        hi = tuple(slice(1, None) if a == axis else slice(None) for a in range(3))
+        lo = tuple(slice(None, -1) if a == axis else slice(None) for a in range(3))
+        component[inner] = scale * (pressure[hi] - pressure[lo])
+        velocity.append(component)
+    return pressure, velocity

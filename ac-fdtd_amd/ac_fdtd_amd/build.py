@@ -1,0 +1,82 @@
+"""Compile the gfx950 HIP library in-tree: ``python -m ac_fdtd_amd.build``.
+
+The reference builds its C loop on first use (``src/ac_fdtd/c_backend.py:55-79``). Here the
+build is explicit and ahead of time, because the library is cross-compiled for gfx950 on a
+machine that may have no GPU and is then shipped with the tree; the loader never compiles.
+
+Flags:
+
+* ``-ffp-contract=off`` keeps every ``a * b + c`` as two roundings, which is what lets the fp64
+  kernels reproduce the NumPy oracle bit for bit (the reference uses the same flag for the
+  same reason, ``c_backend.py:20-24``). Fused multiply-adds the fp32 compensated path wants
+  are written explicitly.
+* ``-target-feature -packed-fp32-ops`` keeps the compiler from forming packed fp32
+  instructions on its own.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+SOURCE_DIR = PACKAGE_DIR.parent / "csrc"
+INCLUDE_DIR = PACKAGE_DIR.parent.parent / "include"
+LIBRARY = PACKAGE_DIR / "lib" / "libacfdtd_hip.so"
+SOURCES = ("sweeps.hip",)
+ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
+
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    found = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(found).exists():
+        raise RuntimeError("hipcc not found: the ROCm toolchain is needed to build the library")
+    return found
+
+
+def command(output: Path) -> list[str]:
+    return [
+        hipcc(),
+        *FLAGS,
+        f"-I{INCLUDE_DIR}",
+        f"-I{SOURCE_DIR}",
+        *(str(SOURCE_DIR / name) for name in SOURCES),
+        "-o",
+        str(output),
+    ]
+
+
+def build_library(verbose: bool = False) -> Path:
+    """Compile every source into :data:`LIBRARY`, replacing it atomically."""
+    LIBRARY.parent.mkdir(parents=True, exist_ok=True)
+    staging = LIBRARY.with_name(LIBRARY.name + f".tmp{os.getpid()}")
+    cmd = command(staging)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode != 0:
+        staging.unlink(missing_ok=True)
+        raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
+    os.replace(staging, LIBRARY)
+    return LIBRARY
+
+
+if __name__ == "__main__":
+    print(build_library(verbose="-v" in sys.argv))

@@ -1,0 +1,353 @@
+"""``AcousticFDTD`` on MI355X: the reference solver API over the HIP time loop.
+
+Drop-in for the reference's duck-typed solver classes (``src/ac_fdtd/scheme.py:82-389``,
+``c_backend.py:160-382``, ``torch_backend.py:74-339``): same constructor arguments, the same
+``(p^n, v^{n-1/2})`` state on public tensors, ``add_volume_source`` / ``add_pressure_receiver``
+/ ``run`` / ``step`` / ``recorded_pressure`` / ``energy``. The state lives in device memory as
+torch tensors; :meth:`run` hands them to the library, which advances ``n_steps`` with two fused
+kernels per step on the current HIP stream. There is no CPU path: without a GPU or without the
+built library the constructor raises.
+
+Precision: ``float64`` runs the plain kernels, which reproduce the NumPy reference bit for bit;
+``float32`` runs the same kernels in single precision, and ``compensated=True`` switches the
+fp32 pressure increment to error-free summation (see ``csrc/sweeps.hip``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from .air import AirAbsorption
+from .boundaries import AbsorbingLayer, WallAdmittances
+from .coefficients import build_coefficients
+from .grid import Grid
+from .medium import AIR, Medium
+
+__all__ = ["AcousticFDTD", "synchronize"]
+
+_TORCH_DTYPES = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
+
+
+def _numpy_dtype(dtype) -> np.dtype:
+    if isinstance(dtype, torch.dtype):
+        dtype = {torch.float32: np.float32, torch.float64: np.float64}.get(dtype, dtype)
+    return np.dtype(dtype)
+
+
+def synchronize() -> None:
+    torch.cuda.synchronize()
+
+
+def _require_gpu(device) -> torch.device:
+    device = torch.device(device)
+    if device.type != "cuda":
+        raise ValueError(f"the HIP backend runs on a GPU device ('cuda' on ROCm), got {device}")
+    if not torch.cuda.is_available():
+        raise RuntimeError("no ROCm GPU is visible: the HIP backend has no CPU fallback")
+    if device.index is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    return device
+
+
+class AcousticFDTD:
+    """The staggered pressure-velocity leapfrog in a rectangular room, on one GPU.
+
+    Args:
+        grid, medium, courant, walls, absorbing_layer, air_absorption: as the reference.
+        dtype: ``float64`` (default, the reference's precision) or ``float32``; numpy or torch.
+        device: a ROCm GPU, ``"cuda"`` / ``"cuda:N"``.
+        compensated: fp32 only — error-free summation of the pressure increment.
+        tile_j, chunk_x: kernel tiling (rows per workgroup, planes per march); 0 = default.
+    """
+
+    def __init__(
+        self,
+        grid: Grid,
+        medium: Medium = AIR,
+        courant: float = 1.0,
+        dtype=np.float64,
+        walls: WallAdmittances | None = None,
+        absorbing_layer: AbsorbingLayer | None = None,
+        air_absorption: AirAbsorption | None = None,
+        device="cuda",
+        compensated: bool = False,
+        tile_j: int = 0,
+        chunk_x: int = 0,
+    ) -> None:
+        self.device = _require_gpu(device)
+        self._lib = _native.library()
+        self.grid = grid
+        self.medium = medium
+        self.courant = courant
+        self.walls = walls or WallAdmittances()
+        self.absorbing_layer = absorbing_layer
+        self.air_absorption = air_absorption
+        self.np_dtype = _numpy_dtype(dtype)
+        self.coefficients = build_coefficients(
+            grid, medium, courant, self.np_dtype, self.walls, absorbing_layer, air_absorption
+        )
+        if compensated and self.np_dtype != np.float32:
+            raise ValueError("compensated summation is an fp32 mode")
+        self.compensated = compensated
+        self.tile_j, self.chunk_x = tile_j, chunk_x
+        self.dtype = _TORCH_DTYPES[self.np_dtype]
+        self.dt = self.coefficients.dt
+        self.step_index = 0
+
+        opts = {"device": self.device, "dtype": self.dtype}
+        self.p = torch.zeros(grid.shape, **opts)
+        self.velocity = [torch.zeros(grid.velocity_shape(a), **opts) for a in range(3)]
+        k = self.coefficients
+        self.relaxation_states = torch.zeros((k.n_relax, *grid.shape), **opts)
+
+        self._wall_table = None
+        self._wall_mask = 0
+        if k.has_walls:
+            table = np.concatenate([k.wall_from_updated, k.wall_from_previous])
+            self._wall_table = torch.as_tensor(table, **opts)
+            self._wall_mask = int(sum(1 << c for c in range(64) if k.wall_is_wall[c]))
+        self._cell = self._face = None
+        if k.has_layer:
+            self._cell = [torch.as_tensor(x, **opts) for x in k.cell_profiles]
+            self._face = [torch.as_tensor(x, **opts) for x in k.face_profiles]
+
+        self._sources: list[tuple[int, int, np.ndarray]] = []
+        self._source_tensors = None
+        self._receivers: list[tuple[int, int]] = []
+        self._receiver_first_run: list[int] = []  # index into _runs of each receiver's first run
+        self._receiver_tensor = None
+        self._runs: list[torch.Tensor] = []  # per run(): (receivers at that time, n_steps)
+        self._energy_workspace = None
+
+    # -- reference API ----------------------------------------------------------------------
+    @property
+    def vx(self) -> torch.Tensor:
+        return self.velocity[0]
+
+    @property
+    def vy(self) -> torch.Tensor:
+        return self.velocity[1]
+
+    @property
+    def vz(self) -> torch.Tensor:
+        return self.velocity[2]
+
+    @property
+    def time(self) -> float:
+        return self.step_index * self.dt
+
+    @property
+    def sample_rate(self) -> float:
+        return 1.0 / self.dt
+
+    def _cell_of(self, point) -> tuple[int, int]:
+        i, j, k = self.grid.cell_index(point)
+        return i, j * self.grid.nz + k
+
+    def add_volume_source(self, point, signal) -> None:
+        """Soft volume source; sample ``n`` acts on the step from ``p^n`` (scheme.py:231-244)."""
+        i, jk = self._cell_of(point)
+        samples = np.asarray(signal, dtype=self.np_dtype).ravel()
+        self._sources.append((i, jk, samples))
+        self._source_tensors = None
+
+    def add_pressure_receiver(self, point) -> int:
+        """Record ``p`` at ``point`` after every later step; returns the channel (:246-255)."""
+        self._receivers.append(self._cell_of(point))
+        self._receiver_first_run.append(len(self._runs))
+        self._receiver_tensor = None
+        return len(self._receivers) - 1
+
+    @property
+    def recorded_pressure(self) -> np.ndarray:
+        """``(channel, sample)`` on the host. Like the reference, receivers added part-way
+        through have shorter rows, and a ragged set cannot form one array (ValueError)."""
+        if not self._receivers:
+            return np.zeros((0, 0))
+        rows = []
+        for channel, first in enumerate(self._receiver_first_run):
+            parts = [run[channel] for run in self._runs[first:]]
+            rows.append(torch.cat(parts) if parts else torch.zeros(0, **self._opts))
+        lengths = {row.numel() for row in rows}
+        if len(lengths) != 1:
+            raise ValueError(
+                "receivers were added part-way through the run and have different lengths "
+                f"{sorted(lengths)}; read them one at a time with recorded_channel()"
+            )
+        return torch.stack(rows).cpu().numpy()
+
+    def recorded_channel(self, channel: int) -> np.ndarray:
+        first = self._receiver_first_run[channel]
+        parts = [run[channel] for run in self._runs[first:]]
+        return torch.cat(parts).cpu().numpy() if parts else np.zeros(0, dtype=self.np_dtype)
+
+    def probe_pressure(self, point) -> float:
+        return float(self.p[self.grid.cell_index(point)])
+
+    def probe_velocity(self, point, axis: int) -> float:
+        """Lower face of the cell containing ``point``, at ``time - dt/2`` (scheme.py:269-283)."""
+        return float(self.velocity[axis][self.grid.cell_index(point)])
+
+    def step(self) -> None:
+        self.run(1)
+
+    def run(self, n_steps: int) -> None:
+        """Advance ``n_steps`` on the current stream (asynchronous, like any GPU launch)."""
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            return
+        recording = None
+        if self._receivers:
+            recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
+        desc = self._descriptor(recording)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        with torch.cuda.device(self.device):
+            status = self._lib.acfdtd_hip_run(
+                ctypes.byref(desc), self.step_index, n_steps, 0, ctypes.c_void_p(stream)
+            )
+        _native.check(status, "acfdtd_hip_run")
+        self.step_index += n_steps
+        if recording is not None:
+            self._runs.append(recording)
+
+    def timed_steps(self, n_steps: int) -> tuple[float, float]:
+        """Advance ``n_steps`` like :meth:`run`, one kernel per library call, with HIP events
+        around every sweep on the launch stream; returns the mean velocity-sweep and
+        pressure-sweep durations in ms. The state advances exactly as :meth:`run` would."""
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            return 0.0, 0.0
+        recording = None
+        if self._receivers:
+            recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
+        desc = self._descriptor(recording)
+        stream = torch.cuda.current_stream(self.device)
+        handle = ctypes.c_void_p(stream.cuda_stream)
+        events = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_steps + 1)]
+        with torch.cuda.device(self.device):
+            events[0].record(stream)
+            for n in range(n_steps):
+                _native.check(self._lib.acfdtd_hip_velocity_sweep(ctypes.byref(desc), handle),
+                              "acfdtd_hip_velocity_sweep")
+                events[2 * n + 1].record(stream)
+                _native.check(
+                    self._lib.acfdtd_hip_pressure_sweep(
+                        ctypes.byref(desc), self.step_index + n, n, handle
+                    ),
+                    "acfdtd_hip_pressure_sweep",
+                )
+                events[2 * n + 2].record(stream)
+        events[-1].synchronize()
+        self.step_index += n_steps
+        if recording is not None:
+            self._runs.append(recording)
+        velocity = sum(events[2 * n].elapsed_time(events[2 * n + 1]) for n in range(n_steps))
+        pressure = sum(events[2 * n + 1].elapsed_time(events[2 * n + 2]) for n in range(n_steps))
+        return velocity / n_steps, pressure / n_steps
+
+    def synchronize(self) -> None:
+        torch.cuda.synchronize(self.device)
+
+    def energy(self) -> float:
+        """Acoustic energy of ``(p^n, v^{n-1/2})`` in J, reduced in fp64 on the device.
+
+        ``dx^3 (sum p^2 / (2 rho c^2) + rho/2 sum v^2 - dt/2 sum v . grad p)``
+        (scheme.py:365-389).
+        """
+        desc = self._descriptor(None)
+        size = self._lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(desc))
+        if self._energy_workspace is None or self._energy_workspace.numel() < size // 8 + 3:
+            self._energy_workspace = torch.empty(size // 8 + 3, dtype=torch.float64,
+                                                 device=self.device)
+        out = self._energy_workspace[-3:]
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        with torch.cuda.device(self.device):
+            status = self._lib.acfdtd_hip_energy(
+                ctypes.byref(desc),
+                ctypes.c_void_p(self._energy_workspace.data_ptr()),
+                ctypes.c_void_p(out.data_ptr()),
+                ctypes.c_void_p(stream),
+            )
+        _native.check(status, "acfdtd_hip_energy")
+        pp, vv, cp = out.cpu().tolist()
+        m, dx = self.medium, self.grid.dx
+        potential = pp / (2.0 * m.density * m.sound_speed**2)
+        kinetic = 0.5 * m.density * vv
+        return dx**3 * (potential + kinetic - 0.5 * self.dt * (cp / dx))
+
+    # -- plumbing ---------------------------------------------------------------------------
+    @property
+    def _opts(self):
+        return {"device": self.device, "dtype": self.dtype}
+
+    def _checked(self, tensor: torch.Tensor, shape, name: str) -> int:
+        if tuple(tensor.shape) != tuple(shape) or tensor.dtype != self.dtype:
+            raise ValueError(f"{name} must be a {self.dtype} tensor of shape {tuple(shape)}")
+        if tensor.device != self.device or not tensor.is_contiguous():
+            raise ValueError(f"{name} must be contiguous on {self.device}")
+        return tensor.data_ptr()
+
+    def _source_arrays(self):
+        if self._source_tensors is None and self._sources:
+            length = max(s.size for _, _, s in self._sources)
+            signal = np.zeros((len(self._sources), max(length, 1)), dtype=self.np_dtype)
+            for n, (_, _, s) in enumerate(self._sources):
+                signal[n, : s.size] = s
+            cells = np.array([(i, jk) for i, jk, _ in self._sources], dtype=np.int32).ravel()
+            lengths = np.array([s.size for _, _, s in self._sources], dtype=np.int64)
+            self._source_tensors = (
+                torch.as_tensor(cells, device=self.device),
+                torch.as_tensor(lengths, device=self.device),
+                torch.as_tensor(signal, device=self.device),
+            )
+        return self._source_tensors
+
+    def _descriptor(self, recording: torch.Tensor | None) -> _native.Desc:
+        g, k = self.grid, self.coefficients
+        d = _native.Desc()
+        d.abi_version = _native.ABI_VERSION
+        d.struct_size = ctypes.sizeof(_native.Desc)
+        d.nx, d.ny, d.nz = g.nx, g.ny, g.nz
+        d.real_bytes = self.np_dtype.itemsize
+        d.n_relax = k.n_relax
+        d.has_layer = int(k.has_layer)
+        d.flags = _native.COMPENSATED if self.compensated else 0
+        d.tile_j, d.chunk_x = self.tile_j, self.chunk_x
+        d.vel_coef, d.pres_coef, d.src_coef = k.velocity, k.pressure, k.source
+        d.relax_scale = k.relax_scale
+        for r in range(k.n_relax):
+            d.relax_decay[r] = k.relax_decay[r]
+            d.relax_mean[r] = k.relax_mean[r]
+            d.relax_gain[r] = k.relax_gain[r]
+        d.wall_mask = self._wall_mask
+        d.wall_table = self._wall_table.data_ptr() if self._wall_table is not None else None
+        d.p = self._checked(self.p, g.shape, "p")
+        for a, name in enumerate(("vx", "vy", "vz")):
+            setattr(d, name, self._checked(self.velocity[a], g.velocity_shape(a), name))
+        if k.n_relax:
+            d.psi = self._checked(self.relaxation_states, (k.n_relax, *g.shape), "psi")
+        if k.has_layer:
+            for a in range(3):
+                d.cell[a] = self._cell[a].data_ptr()
+                d.face[a] = self._face[a].data_ptr()
+        sources = self._source_arrays()
+        if sources is not None:
+            cells, lengths, signal = sources
+            d.n_sources = len(self._sources)
+            d.source_cell = cells.data_ptr()
+            d.source_length = lengths.data_ptr()
+            d.source_signal = signal.data_ptr()
+            d.source_stride = signal.shape[1]
+        if recording is not None:
+            if self._receiver_tensor is None:
+                cells = np.array(self._receivers, dtype=np.int32).ravel()
+                self._receiver_tensor = torch.as_tensor(cells, device=self.device)
+            d.n_receivers = len(self._receivers)
+            d.receiver_cell = self._receiver_tensor.data_ptr()
+            d.recording = recording.data_ptr()
+            d.recording_stride = recording.shape[1]
+        return d

@@ -1,0 +1,431 @@
+// The two fused sweeps of one time step of the staggered pressure-velocity scheme, for gfx950.
+//
+// One step is (reference src/ac_fdtd/scheme.py:285-296):
+//   velocity sweep : v_a -= vel * (p[hi] - p[lo]) on interior faces, then the sponge
+//                    v_a = ((v_a * f_x) * f_y) * f_z                      (scheme.py:307-311, :298)
+//   pressure sweep : div = ((dvx + dvy) + dvz); relaxation states; p -= pres * div; wall closed
+//                    form; sponge; sources; receiver samples               (scheme.py:313-363)
+// The sweeps keep the NumPy reference's association term by term, and the library is built
+// with -ffp-contract=off, so the fp64 kernels reproduce the CPU oracle bit for bit. The fp32
+// kernels run the same code in float; with ACFDTD_HIP_COMPENSATED they instead form the
+// pressure increment with error-free transformations and update velocity with a single fma.
+//
+// Work decomposition (both sweeps): a workgroup is 64 lanes along z (one wavefront per row,
+// unit-stride, coalesced) times tile_j rows along y, and marches a chunk of chunk_x planes
+// along x. The x-neighbour each cell needs (p[i-1] for the velocity sweep, vx[i+1] for the
+// pressure sweep) is carried in a register from the previous plane, so it is read from memory
+// once; the y and z neighbours are re-read from L1/L2. Workgroup ids are remapped so that each
+// of the 8 XCDs, which receive consecutive ids round-robin, owns a contiguous range of tiles
+// and the halo rows/planes shared between neighbouring tiles stay in one L2.
+//
+// The scheme is a bandwidth-bound stencil (about 0.4 flop per byte in fp32): no MFMA.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "acfdtd_hip.h"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &message) {
+    g_last_error = message;
+    return code;
+}
+
+int check_hip(hipError_t status, const char *what) {
+    if (status == hipSuccess) return ACFDTD_HIP_OK;
+    return fail(ACFDTD_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(status));
+}
+
+// ------------------------------------------------------------------------------- kernels --
+
+template <typename real, bool LAYER, bool FUSED_FMA>
+__global__ __launch_bounds__(512) void velocity_sweep(const SweepArgs<real> a) {
+    int kb, jb, xc;
+    if (!decode_tile(a.tiles, kb, jb, xc)) return;
+    const int k = kb * 64 + (int)threadIdx.x;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    if (k >= a.nz || j >= a.ny) return;
+    const int i0 = xc * a.tiles.chunk;
+    const int i1 = min(i0 + a.tiles.chunk, a.nx);
+
+    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const real vel = a.vel;
+    real fyc = 1, fzc = 1, fyf = 1, fzf = 1;
+    if (LAYER) {
+        fyc = a.cell[1][j];
+        fzc = a.cell[2][k];
+        fyf = a.face[1][j];
+        fzf = a.face[2][k];
+    }
+
+    real p_prev = i0 > 0 ? a.p[L.cell(i0 - 1)] : real(0);
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real pc = a.p[c];
+        real fxc = 1, fxf = 1;
+        if (LAYER) {
+            fxc = a.cell[0][i];
+            fxf = a.face[0][i];
+        }
+        // vx on face i (lower face of cell i); face 0 is a wall and is only ever damped.
+        if (LAYER || i > 0) {
+            real v = a.vx[c];
+            if (i > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p_prev);
+            if (LAYER) v = ((v * fxf) * fyc) * fzc;
+            a.vx[c] = v;
+        }
+        const int64_t cy = L.face_y(i);
+        if (LAYER || j > 0) {
+            real v = a.vy[cy];
+            if (j > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - a.p[c - a.nz]);
+            if (LAYER) v = ((v * fxc) * fyf) * fzc;
+            a.vy[cy] = v;
+        }
+        const int64_t cz = L.face_z(i);
+        if (LAYER || k > 0) {
+            real v = a.vz[cz];
+            if (k > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - a.p[c - 1]);
+            if (LAYER) v = ((v * fxc) * fyc) * fzf;
+            a.vz[cz] = v;
+        }
+        if (LAYER) {
+            // The far wall faces belong to the last cell on each axis; like face 0 they are
+            // never updated, only damped (the reference damps whole slabs, walls included).
+            if (i == a.nx - 1) {
+                const int64_t t = c + L.plane;
+                a.vx[t] = ((a.vx[t] * a.face[0][a.nx]) * fyc) * fzc;
+            }
+            if (j == a.ny - 1) {
+                const int64_t t = cy + a.nz;
+                a.vy[t] = ((a.vy[t] * fxc) * a.face[1][a.ny]) * fzc;
+            }
+            if (k == a.nz - 1) {
+                const int64_t t = cz + 1;
+                a.vz[t] = ((a.vz[t] * fxc) * fyc) * a.face[2][a.nz];
+            }
+        }
+        p_prev = pc;
+    }
+}
+
+template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__global__ __launch_bounds__(512) void pressure_sweep(const SweepArgs<real> a, const int64_t step,
+                                                      const int64_t column) {
+    int kb, jb, xc;
+    if (!decode_tile(a.tiles, kb, jb, xc)) return;
+    const int k = kb * 64 + (int)threadIdx.x;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    if (k >= a.nz || j >= a.ny) return;
+    const int i0 = xc * a.tiles.chunk;
+    const int i1 = min(i0 + a.tiles.chunk, a.nx);
+
+    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const int64_t n_cells = L.plane * a.nx;
+    const int jk = j * a.nz + k;
+    const int code_yz = WALLS ? (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4) : 0;
+    real fyc = 1, fzc = 1;
+    if (LAYER) {
+        fyc = a.cell[1][j];
+        fzc = a.cell[2][k];
+    }
+
+    real vx_lo_next = a.vx[L.cell(i0)];
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real vx_lo = vx_lo_next;
+        const real vx_hi = a.vx[c + L.plane];
+        vx_lo_next = vx_hi;
+        const int64_t cy = L.face_y(i);
+        const int64_t cz = L.face_z(i);
+        const real div =
+            ((vx_hi - vx_lo) + (a.vy[cy + a.nz] - a.vy[cy])) + (a.vz[cz + 1] - a.vz[cz]);
+
+        const real p_old = a.p[c];
+        real pn;
+        if (COMPENSATED) {
+            // p_old + sum_r off_r mean_r + div relax_scale - div pres, summed with TwoSum so the
+            // increment costs one rounding instead of NR + 2, and the dominant product exact.
+            real s = p_old, e = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = a.psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                two_sum_acc(s, offset * a.mean[r], e);
+            }
+            if (NR > 0) two_sum_acc(s, div * a.relax_scale, e);
+            const real prod = div * a.pres;
+            const real prod_err = fma_real(div, a.pres, -prod);
+            two_sum_acc(s, -prod, e);
+            pn = s + (e - prod_err);
+        } else {
+            pn = p_old;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = a.psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                pn = pn + offset * a.mean[r];
+            }
+            if (NR > 0) pn = pn + div * a.relax_scale;
+            pn = pn - div * a.pres;
+        }
+
+        if (WALLS) {
+            const int code = code_yz | wall_class(i, a.nx);
+            if ((a.wall_mask >> code) & 1ull)
+                pn = pn * a.wall_table[code] - p_old * a.wall_table[ACFDTD_HIP_WALL_CODES + code];
+        }
+        if (LAYER) pn = ((pn * a.cell[0][i]) * fyc) * fzc;
+
+        // Sources, in insertion order; the plane test is wave-uniform (scalar).
+        for (int s = 0; s < a.n_sources; ++s) {
+            if (a.source_cell[2 * s] == i && a.source_cell[2 * s + 1] == jk &&
+                step < a.source_length[s])
+                pn = pn + a.src * a.source_signal[s * a.source_stride + step];
+        }
+        a.p[c] = pn;
+        for (int r = 0; r < a.n_receivers; ++r) {
+            if (a.receiver_cell[2 * r] == i && a.receiver_cell[2 * r + 1] == jk)
+                a.recording[r * a.recording_stride + column] = pn;
+        }
+    }
+}
+
+// Energy terms: per-workgroup fp64 partials of (sum p^2, sum v^2, sum v (p_hi - p_lo)).
+template <typename real>
+__global__ __launch_bounds__(512) void energy_partials(const SweepArgs<real> a, double *partials) {
+    double pp = 0, vv = 0, cp = 0;
+    int kb, jb, xc;
+    const bool active = decode_tile(a.tiles, kb, jb, xc);
+    if (active) {
+        const int k = kb * 64 + (int)threadIdx.x;
+        const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+        if (k < a.nz && j < a.ny) {
+            const int i0 = xc * a.tiles.chunk;
+            const int i1 = min(i0 + a.tiles.chunk, a.nx);
+            const Layout L(a.nx, a.ny, a.nz, j, k);
+            double p_prev = i0 > 0 ? (double)a.p[L.cell(i0 - 1)] : 0.0;
+            for (int i = i0; i < i1; ++i) {
+                const int64_t c = L.cell(i);
+                const double pc = a.p[c];
+                pp += pc * pc;
+                const double vx = a.vx[c];
+                vv += vx * vx;
+                if (i > 0) cp += vx * (pc - p_prev);
+                const int64_t cy = L.face_y(i);
+                const double vy = a.vy[cy];
+                vv += vy * vy;
+                if (j > 0) cp += vy * (pc - (double)a.p[c - a.nz]);
+                const int64_t cz = L.face_z(i);
+                const double vz = a.vz[cz];
+                vv += vz * vz;
+                if (k > 0) cp += vz * (pc - (double)a.p[c - 1]);
+                if (i == a.nx - 1) {
+                    const double t = a.vx[c + L.plane];
+                    vv += t * t;
+                }
+                if (j == a.ny - 1) {
+                    const double t = a.vy[cy + a.nz];
+                    vv += t * t;
+                }
+                if (k == a.nz - 1) {
+                    const double t = a.vz[cz + 1];
+                    vv += t * t;
+                }
+                p_prev = pc;
+            }
+        }
+    }
+    __shared__ double lds[3][8];
+    block_sum3(pp, vv, cp, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        partials[3 * blockIdx.x + 0] = pp;
+        partials[3 * blockIdx.x + 1] = vv;
+        partials[3 * blockIdx.x + 2] = cp;
+    }
+}
+
+__global__ __launch_bounds__(512) void energy_finish(const double *partials, int n, double *out) {
+    double s[3] = {0, 0, 0};
+    const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int nt = (int)(blockDim.x * blockDim.y);
+    for (int b = t; b < n; b += nt)
+        for (int q = 0; q < 3; ++q) s[q] += partials[3 * b + q];
+    __shared__ double lds[3][8];
+    block_sum3(s[0], s[1], s[2], lds);
+    if (t == 0) {
+        out[0] = s[0];
+        out[1] = s[1];
+        out[2] = s[2];
+    }
+}
+
+// ------------------------------------------------------------------------------- dispatch --
+
+template <typename real>
+using PressureKernel = void (*)(const SweepArgs<real>, const int64_t, const int64_t);
+template <typename real>
+using VelocityKernel = void (*)(const SweepArgs<real>);
+
+template <typename real, int NR>
+PressureKernel<real> pick_pressure_nr(bool walls, bool layer, bool comp) {
+    const int key = (walls ? 1 : 0) | (layer ? 2 : 0) | (comp ? 4 : 0);
+    switch (key) {
+        case 0: return pressure_sweep<real, NR, false, false, false>;
+        case 1: return pressure_sweep<real, NR, true, false, false>;
+        case 2: return pressure_sweep<real, NR, false, true, false>;
+        case 3: return pressure_sweep<real, NR, true, true, false>;
+        case 4: return pressure_sweep<real, NR, false, false, true>;
+        case 5: return pressure_sweep<real, NR, true, false, true>;
+        case 6: return pressure_sweep<real, NR, false, true, true>;
+        default: return pressure_sweep<real, NR, true, true, true>;
+    }
+}
+
+template <typename real>
+PressureKernel<real> pick_pressure(int nr, bool walls, bool layer, bool comp) {
+    switch (nr) {
+        case 0: return pick_pressure_nr<real, 0>(walls, layer, comp);
+        case 1: return pick_pressure_nr<real, 1>(walls, layer, comp);
+        case 2: return pick_pressure_nr<real, 2>(walls, layer, comp);
+        case 3: return pick_pressure_nr<real, 3>(walls, layer, comp);
+        default: return pick_pressure_nr<real, 4>(walls, layer, comp);
+    }
+}
+
+template <typename real>
+VelocityKernel<real> pick_velocity(bool layer, bool comp) {
+    if (layer) return comp ? velocity_sweep<real, true, true> : velocity_sweep<real, true, false>;
+    return comp ? velocity_sweep<real, false, true> : velocity_sweep<real, false, false>;
+}
+
+template <typename real>
+int run_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
+              hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const VelocityKernel<real> vk = pick_velocity<real>(d.has_layer != 0, comp);
+    const PressureKernel<real> pk = pick_pressure<real>(d.n_relax, d.wall_mask != 0,
+                                                        d.has_layer != 0, comp);
+    const dim3 block(64, a.tiles.tile_j);
+    const dim3 grid(a.tiles.launched);
+    for (int64_t n = 0; n < n_steps; ++n) {
+        hipLaunchKernelGGL(vk, grid, block, 0, stream, a);
+        hipLaunchKernelGGL(pk, grid, block, 0, stream, a, step_offset + n, column + n);
+    }
+    return check_hip(hipGetLastError(), "kernel launch");
+}
+
+template <typename real>
+int single_sweep(const acfdtd_hip_desc &d, bool pressure, int64_t step, int64_t column,
+                 hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const dim3 block(64, a.tiles.tile_j);
+    const dim3 grid(a.tiles.launched);
+    if (pressure) {
+        const PressureKernel<real> pk = pick_pressure<real>(d.n_relax, d.wall_mask != 0,
+                                                            d.has_layer != 0, comp);
+        hipLaunchKernelGGL(pk, grid, block, 0, stream, a, step, column);
+    } else {
+        hipLaunchKernelGGL(pick_velocity<real>(d.has_layer != 0, comp), grid, block, 0, stream, a);
+    }
+    return check_hip(hipGetLastError(), "sweep launch");
+}
+
+template <typename real>
+int energy_terms(const acfdtd_hip_desc &d, void *workspace, double *out, hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    double *partials = static_cast<double *>(workspace);
+    hipLaunchKernelGGL(energy_partials<real>, dim3(a.tiles.launched), dim3(64, a.tiles.tile_j), 0,
+                       stream, a, partials);
+    hipLaunchKernelGGL(energy_finish, dim3(1), dim3(64, 8), 0, stream, partials,
+                       (int)a.tiles.launched, out);
+    return check_hip(hipGetLastError(), "energy launch");
+}
+
+}  // namespace acfdtd
+
+// ---------------------------------------------------------------------------------- C-ABI --
+
+using namespace acfdtd;
+
+extern "C" size_t acfdtd_hip_struct_size(void) { return sizeof(acfdtd_hip_desc); }
+
+extern "C" int acfdtd_hip_abi_version(void) { return ACFDTD_HIP_ABI_VERSION; }
+
+extern "C" int acfdtd_hip_supports_real(int real_bytes) {
+    return real_bytes == 4 || real_bytes == 8;
+}
+
+extern "C" const char *acfdtd_hip_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int acfdtd_hip_run(const acfdtd_hip_desc *desc, int64_t step_offset, int64_t n_steps,
+                              int64_t recording_column, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (n_steps < 0 || step_offset < 0 || recording_column < 0)
+        return fail(ACFDTD_HIP_EINVAL, "n_steps, step_offset and recording_column must be >= 0");
+    if (desc->n_receivers > 0 && recording_column + n_steps > desc->recording_stride)
+        return fail(ACFDTD_HIP_EINVAL, "recording is too short for this many steps");
+    if (n_steps == 0) return ACFDTD_HIP_OK;
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8)
+        return run_steps<double>(*desc, step_offset, n_steps, recording_column, s);
+    return run_steps<float>(*desc, step_offset, n_steps, recording_column, s);
+}
+
+extern "C" int acfdtd_hip_velocity_sweep(const acfdtd_hip_desc *desc, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return single_sweep<double>(*desc, false, 0, 0, s);
+    return single_sweep<float>(*desc, false, 0, 0, s);
+}
+
+extern "C" int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t step,
+                                         int64_t recording_column, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (step < 0 || recording_column < 0)
+        return fail(ACFDTD_HIP_EINVAL, "step and recording_column must be >= 0");
+    if (desc->n_receivers > 0 && recording_column >= desc->recording_stride)
+        return fail(ACFDTD_HIP_EINVAL, "recording column out of range");
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return single_sweep<double>(*desc, true, step, recording_column, s);
+    return single_sweep<float>(*desc, true, step, recording_column, s);
+}
+
+extern "C" size_t acfdtd_hip_energy_workspace_bytes(const acfdtd_hip_desc *desc) {
+    if (validate(desc) != ACFDTD_HIP_OK) return 0;
+    return (size_t)tiling_for(*desc).launched * 3 * sizeof(double);
+}
+
+extern "C" int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out,
+                                 void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (workspace == nullptr || out == nullptr)
+        return fail(ACFDTD_HIP_EINVAL, "energy needs a workspace and an output buffer");
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return energy_terms<double>(*desc, workspace, out, s);
+    return energy_terms<float>(*desc, workspace, out, s);
+}

@@ -1,0 +1,237 @@
+// Shared pieces of the sweep kernels: tiling, field layout, kernel arguments, small numerics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "acfdtd_hip.h"
+
+namespace acfdtd {
+
+int fail(int code, const std::string &message);
+
+// Workgroup tiling of a room: 64 lanes along z, tile_j rows along y, chunk planes along x.
+// `launched` rounds the tile count up to a multiple of the 8 XCDs so that the remap in
+// decode_tile is a bijection; the surplus workgroups exit at once.
+struct Tiling {
+    int tile_j;
+    int chunk;
+    int nblk_k;
+    int nblk_j;
+    int nchunk;
+    int tiles;
+    int launched;
+};
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+inline Tiling make_tiling(int nx, int ny, int nz, int tile_j, int chunk) {
+    Tiling t{};
+    t.tile_j = tile_j > 0 ? tile_j : 4;
+    t.nblk_k = ceil_div(nz, 64);
+    t.nblk_j = ceil_div(ny, t.tile_j);
+    if (chunk <= 0) {
+        // Enough workgroups to fill 256 CUs several times over, but never fewer than 4 planes
+        // per march, so the one re-read halo plane stays a small fraction of the traffic.
+        const int columns = t.nblk_k * t.nblk_j;
+        const int wanted = ceil_div(4096, columns);
+        chunk = ceil_div(nx, wanted > 0 ? wanted : 1);
+        if (chunk < 4) chunk = 4;
+    }
+    t.chunk = chunk;
+    t.nchunk = ceil_div(nx, chunk);
+    t.tiles = t.nblk_k * t.nblk_j * t.nchunk;
+    t.launched = ceil_div(t.tiles, 8) * 8;
+    return t;
+}
+
+inline Tiling tiling_for(const acfdtd_hip_desc &d) {
+    return make_tiling(d.nx, d.ny, d.nz, d.tile_j, d.chunk_x);
+}
+
+// Consecutive workgroup ids go to the 8 XCDs in turn; give XCD x the contiguous tile range
+// [x * launched / 8, (x + 1) * launched / 8). Tiles are ordered z-block, then y-block, then
+// x-chunk.
+__device__ inline bool decode_tile(const Tiling &t, int &kb, int &jb, int &xc) {
+    const unsigned n = (unsigned)t.launched;
+    unsigned b = blockIdx.x;
+    b = (b & 7u) * (n >> 3) + (b >> 3);
+    if (b >= (unsigned)t.tiles) return false;
+    kb = (int)(b % (unsigned)t.nblk_k);
+    b /= (unsigned)t.nblk_k;
+    jb = (int)(b % (unsigned)t.nblk_j);
+    xc = (int)(b / (unsigned)t.nblk_j);
+    return true;
+}
+
+// Offsets of cell (i, j, k) and of the lower y / z faces of that cell in the staggered arrays.
+struct Layout {
+    int64_t plane, plane_y, plane_z, jk, jk_z;
+    __device__ Layout(int nx, int ny, int nz, int j, int k)
+        : plane((int64_t)ny * nz),
+          plane_y((int64_t)(ny + 1) * nz),
+          plane_z((int64_t)ny * (nz + 1)),
+          jk((int64_t)j * nz + k),
+          jk_z((int64_t)j * (nz + 1) + k) {
+        (void)nx;
+    }
+    __device__ int64_t cell(int i) const { return (int64_t)i * plane + jk; }
+    __device__ int64_t face_y(int i) const { return (int64_t)i * plane_y + jk; }
+    __device__ int64_t face_z(int i) const { return (int64_t)i * plane_z + jk_z; }
+};
+
+// Wall class of index i on an axis of n cells: bit 0 low face, bit 1 high face.
+__device__ inline int wall_class(int i, int n) { return (i == 0 ? 1 : 0) | (i == n - 1 ? 2 : 0); }
+
+template <typename real>
+struct SweepArgs {
+    Tiling tiles;
+    int nx, ny, nz;
+    int n_sources, n_receivers;
+    real vel, pres, src, relax_scale;
+    real decay[ACFDTD_HIP_MAX_RELAXATION];
+    real mean[ACFDTD_HIP_MAX_RELAXATION];
+    real gain[ACFDTD_HIP_MAX_RELAXATION];
+    uint64_t wall_mask;
+    const real *wall_table;
+    real *p, *vx, *vy, *vz, *psi;
+    const real *cell[3];
+    const real *face[3];
+    const int32_t *source_cell;
+    const int64_t *source_length;
+    const real *source_signal;
+    int64_t source_stride;
+    const int32_t *receiver_cell;
+    real *recording;
+    int64_t recording_stride;
+};
+
+template <typename real>
+void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
+    a.tiles = tiling_for(d);
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.nz = d.nz;
+    a.n_sources = d.n_sources;
+    a.n_receivers = d.n_receivers;
+    a.vel = (real)d.vel_coef;
+    a.pres = (real)d.pres_coef;
+    a.src = (real)d.src_coef;
+    a.relax_scale = (real)d.relax_scale;
+    for (int r = 0; r < ACFDTD_HIP_MAX_RELAXATION; ++r) {
+        a.decay[r] = (real)d.relax_decay[r];
+        a.mean[r] = (real)d.relax_mean[r];
+        a.gain[r] = (real)d.relax_gain[r];
+    }
+    a.wall_mask = d.wall_mask;
+    a.wall_table = static_cast<const real *>(d.wall_table);
+    a.p = static_cast<real *>(d.p);
+    a.vx = static_cast<real *>(d.vx);
+    a.vy = static_cast<real *>(d.vy);
+    a.vz = static_cast<real *>(d.vz);
+    a.psi = static_cast<real *>(d.psi);
+    for (int q = 0; q < 3; ++q) {
+        a.cell[q] = static_cast<const real *>(d.cell[q]);
+        a.face[q] = static_cast<const real *>(d.face[q]);
+    }
+    a.source_cell = d.source_cell;
+    a.source_length = d.source_length;
+    a.source_signal = static_cast<const real *>(d.source_signal);
+    a.source_stride = d.source_stride;
+    a.receiver_cell = d.receiver_cell;
+    a.recording = static_cast<real *>(d.recording);
+    a.recording_stride = d.recording_stride;
+}
+
+inline int validate(const acfdtd_hip_desc *d) {
+    if (d == nullptr) return fail(ACFDTD_HIP_EINVAL, "descriptor is NULL");
+    if (d->abi_version != ACFDTD_HIP_ABI_VERSION)
+        return fail(ACFDTD_HIP_EABI, "descriptor ABI version " + std::to_string(d->abi_version) +
+                                         " does not match library version " +
+                                         std::to_string(ACFDTD_HIP_ABI_VERSION));
+    if (d->struct_size != sizeof(acfdtd_hip_desc))
+        return fail(ACFDTD_HIP_EABI, "descriptor is " + std::to_string(d->struct_size) +
+                                         " bytes, library expects " +
+                                         std::to_string(sizeof(acfdtd_hip_desc)));
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(ACFDTD_HIP_EINVAL, "cell counts must be >= 1");
+    if ((int64_t)d->ny * d->nz > INT32_MAX)
+        return fail(ACFDTD_HIP_EINVAL, "ny * nz must fit in 31 bits");
+    if (d->real_bytes != 4 && d->real_bytes != 8)
+        return fail(ACFDTD_HIP_EINVAL, "real_bytes must be 4 or 8");
+    if (d->n_relax < 0 || d->n_relax > ACFDTD_HIP_MAX_RELAXATION)
+        return fail(ACFDTD_HIP_EINVAL, "n_relax out of range");
+    if ((d->flags & ~ACFDTD_HIP_COMPENSATED) != 0) return fail(ACFDTD_HIP_EINVAL, "unknown flags");
+    if ((d->flags & ACFDTD_HIP_COMPENSATED) && d->real_bytes != 4)
+        return fail(ACFDTD_HIP_EINVAL, "compensated summation is an fp32 mode");
+    if (!(d->tile_j == 0 || d->tile_j == 1 || d->tile_j == 2 || d->tile_j == 4 || d->tile_j == 8))
+        return fail(ACFDTD_HIP_EINVAL, "tile_j must be 0, 1, 2, 4 or 8");
+    if (d->chunk_x < 0) return fail(ACFDTD_HIP_EINVAL, "chunk_x must be >= 0");
+    if (!d->p || !d->vx || !d->vy || !d->vz) return fail(ACFDTD_HIP_EINVAL, "field pointer is NULL");
+    if (d->n_relax > 0 && !d->psi) return fail(ACFDTD_HIP_EINVAL, "relaxation states are NULL");
+    if (d->wall_mask != 0 && !d->wall_table) return fail(ACFDTD_HIP_EINVAL, "wall table is NULL");
+    if (d->has_layer)
+        for (int q = 0; q < 3; ++q)
+            if (!d->cell[q] || !d->face[q]) return fail(ACFDTD_HIP_EINVAL, "sponge profile is NULL");
+    if (d->n_sources < 0 || d->n_receivers < 0)
+        return fail(ACFDTD_HIP_EINVAL, "negative source or receiver count");
+    if (d->n_sources > 0 &&
+        (!d->source_cell || !d->source_length || !d->source_signal || d->source_stride < 0))
+        return fail(ACFDTD_HIP_EINVAL, "source arrays are NULL");
+    if (d->n_receivers > 0 && (!d->receiver_cell || !d->recording))
+        return fail(ACFDTD_HIP_EINVAL, "receiver arrays are NULL");
+    const Tiling t = tiling_for(*d);
+    if ((int64_t)t.nblk_k * t.nblk_j * t.nchunk > (int64_t)INT32_MAX - 8)
+        return fail(ACFDTD_HIP_EINVAL, "grid too large for one launch");
+    return ACFDTD_HIP_OK;
+}
+
+// ------------------------------------------------------------------------------ numerics --
+
+__device__ inline float fma_real(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_real(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// v - vel * d, either as the reference's two roundings or as one fma.
+template <typename real, bool FUSED>
+__device__ inline real face_update(real v, real vel, real d) {
+    if (FUSED) return fma_real(-vel, d, v);
+    return v - vel * d;
+}
+
+// s += t with the rounding error of the addition accumulated into e (Knuth's TwoSum).
+template <typename real>
+__device__ inline void two_sum_acc(real &s, real t, real &e) {
+    const real x = s + t;
+    const real tb = x - s;
+    const real sa = x - tb;
+    e += (s - sa) + (t - tb);
+    s = x;
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
+    return v;
+}
+
+// Sum three values over a (64, blockDim.y) workgroup, in a fixed order; result in thread 0.
+__device__ inline void block_sum3(double &a, double &b, double &c, double (*lds)[8]) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if (threadIdx.x == 0) {
+        lds[0][threadIdx.y] = a;
+        lds[1][threadIdx.y] = b;
+        lds[2][threadIdx.y] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        for (unsigned w = 1; w < blockDim.y; ++w) {
+            a += lds[0][w];
+            b += lds[1][w];
+            c += lds[2][w];
+        }
+    }
+}
+
+}  // namespace acfdtd

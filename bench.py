@@ -1,0 +1,268 @@
+"""Throughput of the fused HIP time loop, in Gcell-updates/s, on one MI355X.
+
+Workload (the throughput configuration): a cubic grid of ``--size``^3 cells at dx = 1 cm,
+walls of normal-incidence absorption 0.15 on all six faces, a 16-cell sponge on all faces, air
+absorption at 20 C / 50 % RH (three relaxation states), a Gaussian-derivative pulse of seeded
+width 4-8 cells at a seeded interior cell, 8 receivers at seeded interior cells, ``--steps``
+steps. All state is resident in HBM before timing; the timed region is one call of the library
+advancing ``--steps`` steps, bracketed by HIP events and synchronisations.
+
+Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3; the fp64 plain
+kernel on the same workload, a rigid-box fp64 energy drift and a 384^3 fp32 run are under
+``extra``. ``roofline`` is for the dominant kernel (the pressure sweep), from HIP events around
+each of its launches; ``cpu_baseline`` is the reference's own compiled C loop (``oracle/_ref``,
+all host cores this process may use) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _path in (ROOT / "ac-fdtd_amd", ROOT):
+    if str(_path) not in sys.path:
+        sys.path.insert(0, str(_path))
+
+import numpy as np  # noqa: E402
+
+METRIC = "Gcell-updates/s at 256³ (fp64 plain vs fp32/batched, %HBM); grad/fwd time"
+HBM_PEAK_GBS = 8000.0
+DX = 0.01
+SPONGE = 16
+MARGIN = 24  # source and receivers at least this many cells from every face
+
+
+def log(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def workload(size: int, steps: int, seed: int = 2024):
+    """Seeded source/receiver cells and pulse of the throughput workload."""
+    from ac_fdtd_amd import AIR, AbsorbingLayer, AirAbsorption, Grid, WallAdmittances
+
+    grid = Grid(dx=DX, nx=size, ny=size, nz=size)
+    rng = np.random.default_rng(seed)
+    width_cells = rng.uniform(4.0, 8.0)
+
+    def point():
+        cell = rng.integers(MARGIN, size - MARGIN, size=3)
+        return tuple((cell + 0.5) * DX)
+
+    source = point()
+    receivers = [point() for _ in range(8)]
+    config = {
+        "walls": WallAdmittances.from_absorption(0.15),
+        "absorbing_layer": AbsorbingLayer(thickness=SPONGE),
+        "air_absorption": AirAbsorption(temperature=20.0, relative_humidity=50.0),
+    }
+    width = width_cells * DX / AIR.sound_speed
+    return grid, config, source, receivers, width
+
+
+def make_solver(cls, size, steps, dtype, **kwargs):
+    from ac_fdtd_amd.sources import gaussian_derivative_pulse
+
+    grid, config, source, receivers, width = workload(size, steps)
+    solver = cls(grid, dtype=dtype, **config, **kwargs)
+    solver.add_volume_source(source, gaussian_derivative_pulse(steps + 16, solver.dt, width))
+    for r in receivers:
+        solver.add_pressure_receiver(r)
+    return solver
+
+
+def bytes_per_cell(n_relax: int, itemsize: int) -> tuple[int, int]:
+    """Algorithmic bytes per cell-update: (whole step, pressure sweep).
+
+    Velocity sweep reads p and reads/writes vx, vy, vz (7 elements); pressure sweep reads
+    vx, vy, vz, reads/writes p and each relaxation state (5 + 2 n_relax elements).
+    """
+    return (12 + 2 * n_relax) * itemsize, (5 + 2 * n_relax) * itemsize
+
+
+def gpu_throughput(size, steps, warmup, dtype, compensated=False):
+    import torch
+
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    solver = make_solver(AcousticFDTD, size, steps + warmup + 40, dtype, compensated=compensated)
+    solver.run(warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall = time.perf_counter()
+    start.record()
+    solver.run(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall
+    ms = start.elapsed_time(stop)
+    cells = solver.grid.n_cells
+    ms_velocity, ms_pressure = solver.timed_steps(20)
+    recorded = solver.recorded_pressure
+    if not np.all(np.isfinite(recorded)):
+        raise RuntimeError("non-finite receiver samples")
+    return {
+        "gcells": cells * steps / (ms * 1e-3) / 1e9,
+        "ms_per_step": ms / steps,
+        "wall_s": wall,
+        "ms_velocity": ms_velocity,
+        "ms_pressure": ms_pressure,
+        "n_relax": solver.coefficients.n_relax,
+        "cells": cells,
+    }
+
+
+def rigid_energy_drift(size, steps):
+    """fp64 rigid box from a seeded random pressure: max |E/E0 - 1| over the run."""
+    import torch
+
+    from ac_fdtd_amd import Grid
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    solver = AcousticFDTD(Grid(dx=DX, nx=size, ny=size, nz=size), dtype=np.float64)
+    solver.p.copy_(torch.as_tensor(np.random.default_rng(0).standard_normal(solver.grid.shape)))
+    initial = solver.energy()
+    drift = 0.0
+    chunk = max(1, steps // 8)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        solver.run(n)
+        done += n
+        drift = max(drift, abs(solver.energy() / initial - 1.0))
+    return drift
+
+
+def cpu_baseline(size, seconds):
+    """The reference C loop (oracle/_ref) on the host, fp32, same workload, bounded sample."""
+    from oracle import ref_c
+
+    if not ref_c.is_available(np.float32):
+        return None
+    threads = ref_c.host_thread_count()
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    solver = make_solver(ref_c.RefCFDTD, size, 4000, np.float32, threads=threads)
+    solver.run(2)
+    started = time.perf_counter()
+    solver.run(2)
+    per_step = (time.perf_counter() - started) / 2
+    steps = max(3, min(4000, int(seconds / per_step)))
+    started = time.perf_counter()
+    solver.run(steps)
+    elapsed = time.perf_counter() - started
+    return {
+        "value": round(solver.grid.n_cells * steps / elapsed / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "cores": solver.threads,
+        "kind": "reference",
+        "sample": f"reference C loop (acfdtd.c, fp32, -O3 -ffp-contract=off, pthreads) on the "
+        f"same {size}^3 walls+sponge+air workload, {steps} steps after 4 warm-up steps, "
+        f"{elapsed:.1f} s",
+    }
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=4000)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--size", type=int, default=256)
+    parser.add_argument("--cpu-seconds", type=float, default=15.0)
+    parser.add_argument("--skip-cpu", action="store_true")
+    parser.add_argument("--skip-extra", action="store_true")
+    args = parser.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(
+            f"bench.py measures one GPU; --gpus {args.gpus} / WORLD_SIZE {world} is not "
+            "supported (rooms shard by slab with one-plane halo exchanges, which this build "
+            "does not implement)",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 2
+
+    log(f"fp32 {args.size}^3, {args.steps} steps")
+    f32 = gpu_throughput(args.size, args.steps, args.warmup, np.float32)
+    log(f"fp32: {f32['gcells']:.2f} Gcell/s")
+    step_bytes, sweep_bytes = bytes_per_cell(f32["n_relax"], 4)
+    achieved = sweep_bytes * f32["cells"] / (f32["ms_pressure"] * 1e-3) / 1e9
+
+    extra = {
+        "bytes_per_cell_update": step_bytes,
+        "step_hbm_fraction": round(step_bytes * f32["gcells"] / HBM_PEAK_GBS, 4),
+        "ms_velocity_sweep": round(f32["ms_velocity"], 4),
+        "ms_pressure_sweep": round(f32["ms_pressure"], 4),
+    }
+    if not args.skip_extra:
+        log("fp32 compensated")
+        comp = gpu_throughput(args.size, args.steps, args.warmup, np.float32, compensated=True)
+        extra["fp32_compensated_gcells"] = round(comp["gcells"], 3)
+        log("fp64 plain")
+        f64 = gpu_throughput(args.size, args.steps, args.warmup, np.float64)
+        extra["fp64_plain_gcells"] = round(f64["gcells"], 3)
+        extra["fp64_step_hbm_fraction"] = round(
+            bytes_per_cell(f64["n_relax"], 8)[0] * f64["gcells"] / HBM_PEAK_GBS, 4
+        )
+        log("fp32 384^3")
+        big = gpu_throughput(384, args.steps, args.warmup, np.float32)
+        extra["fp32_384_gcells"] = round(big["gcells"], 3)
+        log("fp64 rigid energy drift")
+        extra["fp64_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.steps)
+        torch.cuda.empty_cache()
+
+    baseline = None
+    if not args.skip_cpu:
+        log("cpu baseline")
+        baseline = cpu_baseline(args.size, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(f32["gcells"], 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"throughput {args.size}^3, dx 1 cm, walls alpha 0.15, 16-cell sponge on "
+            "all faces, air 20 C / 50 % RH, seeded Gaussian-derivative pulse, 8 receivers",
+            "grid": [args.size] * 3,
+            "relaxation_states": f32["n_relax"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "pressure_sweep",
+            "algorithmic_bytes_per_launch": sweep_bytes * f32["cells"],
+        },
+        "cpu_baseline": baseline,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

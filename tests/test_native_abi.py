@@ -1,0 +1,132 @@
+"""The C-ABI library on the CPU: it loads, exports what ``include/*.h`` declares, checks its
+descriptor and reports errors — all without touching a GPU (validation precedes any HIP call).
+Also the Python layer's argument errors (reference ``scheme.py:117-118``,
+``c_backend.py:195-200``)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from ac_fdtd_amd import _native
+from ac_fdtd_amd.build import LIBRARY
+
+INCLUDE = Path(__file__).resolve().parents[1] / "include"
+EINVAL, EABI = -1, -2
+
+
+def declared_functions() -> list[str]:
+    names = []
+    for header in INCLUDE.glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names += re.findall(r"\b(acfdtd_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not LIBRARY.exists():
+        pytest.skip("HIP library not built")
+    return _native.library()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert "acfdtd_hip_run" in names and len(names) >= 9
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_struct_size_and_abi_version(lib):
+    assert lib.acfdtd_hip_struct_size() == ctypes.sizeof(_native.Desc)
+    assert lib.acfdtd_hip_abi_version() == _native.ABI_VERSION
+    assert lib.acfdtd_hip_supports_real(4) and lib.acfdtd_hip_supports_real(8)
+    assert not lib.acfdtd_hip_supports_real(2)
+
+
+def _desc(**overrides):
+    d = _native.Desc()
+    d.abi_version = _native.ABI_VERSION
+    d.struct_size = ctypes.sizeof(_native.Desc)
+    d.nx = d.ny = d.nz = 4
+    d.real_bytes = 8
+    fake = 4096  # never dereferenced: every case fails validation first
+    d.p = d.vx = d.vy = d.vz = fake
+    for key, value in overrides.items():
+        setattr(d, key, value)
+    return d
+
+
+@pytest.mark.parametrize(
+    ("overrides", "code", "message"),
+    [
+        ({"abi_version": 99}, EABI, "ABI version"),
+        ({"struct_size": 8}, EABI, "bytes"),
+        ({"nz": 0}, EINVAL, "cell counts"),
+        ({"real_bytes": 2}, EINVAL, "real_bytes"),
+        ({"n_relax": 5}, EINVAL, "n_relax"),
+        ({"n_relax": 2}, EINVAL, "relaxation states"),
+        ({"flags": 1}, EINVAL, "fp32 mode"),
+        ({"flags": 8}, EINVAL, "unknown flags"),
+        ({"tile_j": 3}, EINVAL, "tile_j"),
+        ({"p": None}, EINVAL, "field pointer"),
+        ({"wall_mask": 5}, EINVAL, "wall table"),
+        ({"has_layer": 1}, EINVAL, "sponge profile"),
+        ({"n_sources": 1}, EINVAL, "source arrays"),
+        ({"n_receivers": 2}, EINVAL, "receiver arrays"),
+    ],
+)
+def test_malformed_descriptors_are_refused(lib, overrides, code, message):
+    desc = _desc(**overrides)
+    assert lib.acfdtd_hip_run(ctypes.byref(desc), 0, 1, 0, None) == code
+    assert message in lib.acfdtd_hip_last_error().decode()
+    assert lib.acfdtd_hip_energy(ctypes.byref(desc), None, None, None) == code
+
+
+def test_null_descriptor_and_bad_run_arguments(lib):
+    assert lib.acfdtd_hip_run(None, 0, 1, 0, None) == EINVAL
+    desc = _desc()
+    assert lib.acfdtd_hip_run(ctypes.byref(desc), 0, -1, 0, None) == EINVAL
+    assert lib.acfdtd_hip_run(ctypes.byref(desc), -3, 1, 0, None) == EINVAL
+    recording = _desc(n_receivers=1, receiver_cell=4096, recording=4096, recording_stride=10)
+    assert lib.acfdtd_hip_run(ctypes.byref(recording), 0, 11, 0, None) == EINVAL
+    assert "too short" in lib.acfdtd_hip_last_error().decode()
+    assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(recording), 0, 10, None) == EINVAL
+    assert lib.acfdtd_hip_energy(ctypes.byref(desc), None, None, None) == EINVAL
+    # Zero steps is a valid no-op and enqueues nothing.
+    assert lib.acfdtd_hip_run(ctypes.byref(desc), 0, 0, 0, None) == 0
+
+
+def test_energy_workspace_size_follows_the_tiling(lib):
+    small = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc()))
+    large = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc(nx=200, ny=130, nz=70)))
+    assert small > 0 and small % 24 == 0 and large > small
+    assert lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc(nx=0))) == 0
+
+
+def test_the_python_solver_refuses_to_run_without_a_gpu():
+    import torch
+
+    from ac_fdtd_amd import Grid
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    grid = Grid(dx=0.1, nx=3, ny=3, nz=3)
+    with pytest.raises(ValueError, match="GPU device"):
+        AcousticFDTD(grid, device="cpu")
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            AcousticFDTD(grid)
+
+
+def test_host_side_argument_errors():
+    from ac_fdtd_amd import Grid
+    from ac_fdtd_amd.coefficients import build_coefficients
+    from ac_fdtd_amd.medium import AIR
+
+    grid = Grid(dx=0.1, nx=3, ny=3, nz=3)
+    with pytest.raises(ValueError, match="dtype"):
+        build_coefficients(grid, AIR, 1.0, np.float16, None, None, None)
