@@ -16,7 +16,19 @@ import torch  # noqa: F401  (must precede the dlopen below; see module docstring
 
 from .build import LIBRARY
 
-__all__ = ["ABI_VERSION", "COMPENSATED", "Desc", "MAX_RELAXATION", "check", "library"]
+__all__ = [
+    "ABI_VERSION",
+    "AdjointDesc",
+    "COMPENSATED",
+    "Desc",
+    "GRAD_AIR",
+    "GRAD_SLOTS",
+    "GRAD_SPONGE",
+    "GRAD_WALLS",
+    "MAX_RELAXATION",
+    "check",
+    "library",
+]
 
 ABI_VERSION = 1
 MAX_RELAXATION = 4
@@ -68,6 +80,49 @@ class Desc(_c.Structure):
         ("receiver_cell", _ptr),
         ("recording", _ptr),
         ("recording_stride", _c.c_int64),
+        ("p_swap", _ptr),
+        ("vx_swap", _ptr),
+        ("vy_swap", _ptr),
+        ("vz_swap", _ptr),
+        ("wall_tape", _ptr),
+        ("wall_tape_stride", _c.c_int64),
+        ("wall_tape_step0", _c.c_int64),
+        ("wall_tape_rows", _c.c_int64),
+    ]
+
+
+GRAD_WALLS = 1
+GRAD_SPONGE = 2
+GRAD_AIR = 4
+GRAD_SLOTS = 12  # per-workgroup gradient accumulators: 6 walls, sponge, scale, 4 gains
+
+
+class AdjointDesc(_c.Structure):
+    """Mirror of ``acfdtd_hip_adjoint_desc``."""
+
+    _fields_ = [
+        ("abi_version", _c.c_uint32),
+        ("struct_size", _c.c_uint32),
+        ("grads", _c.c_uint32),
+        ("reserved", _c.c_int32),
+        ("lp", _ptr),
+        ("lvx", _ptr),
+        ("lvy", _ptr),
+        ("lvz", _ptr),
+        ("lpsi", _ptr),
+        ("ldiv", _ptr),
+        ("grad_recording", _ptr),
+        ("grad_recording_stride", _c.c_int64),
+        ("grad_recording_step0", _c.c_int64),
+        ("grad_signal", _ptr),
+        ("fwd_p", _ptr),
+        ("fwd_vx", _ptr),
+        ("fwd_vy", _ptr),
+        ("fwd_vz", _ptr),
+        ("weight_cell", _ptr * 3),
+        ("weight_face", _ptr * 3),
+        ("dt", _c.c_double),
+        ("partials", _ptr),
     ]
 
 
@@ -96,6 +151,18 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_velocity_sweep.argtypes = [_c.POINTER(Desc), _ptr]
     lib.acfdtd_hip_pressure_sweep.restype = _c.c_int
     lib.acfdtd_hip_pressure_sweep.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr]
+    lib.acfdtd_hip_fused_step.restype = _c.c_int
+    lib.acfdtd_hip_fused_step.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _c.c_int, _ptr]
+    lib.acfdtd_hip_wall_tape_stride.restype = _c.c_int64
+    lib.acfdtd_hip_wall_tape_stride.argtypes = [_c.c_int32, _c.c_int32, _c.c_int32]
+    lib.acfdtd_hip_adjoint_partials_count.restype = _c.c_size_t
+    lib.acfdtd_hip_adjoint_partials_count.argtypes = [_c.POINTER(Desc)]
+    lib.acfdtd_hip_adjoint_face_sweep.restype = _c.c_int
+    lib.acfdtd_hip_adjoint_face_sweep.argtypes = [_c.POINTER(Desc), _c.POINTER(AdjointDesc), _ptr]
+    lib.acfdtd_hip_adjoint_cell_sweep.restype = _c.c_int
+    lib.acfdtd_hip_adjoint_cell_sweep.argtypes = [
+        _c.POINTER(Desc), _c.POINTER(AdjointDesc), _c.c_int64, _c.c_int, _c.c_int, _ptr
+    ]
     lib.acfdtd_hip_energy_workspace_bytes.restype = _c.c_size_t
     lib.acfdtd_hip_energy_workspace_bytes.argtypes = [_c.POINTER(Desc)]
     lib.acfdtd_hip_energy.restype = _c.c_int

@@ -1,0 +1,347 @@
+"""Gradients of receiver recordings with respect to the room's parameters, on the GPU.
+
+:class:`RoomResponse` maps parameters to the ``(receiver, step)`` pressure recording of one
+room through the HIP time loop, as a differentiable torch operation. Differentiable inputs:
+
+* ``admittance`` — the six normalised wall admittances ``xi`` (FACES order);
+* ``signal`` — the source signals, ``(n_sources, n_samples)``;
+* ``sponge_strength`` — ``sigma_max`` of the absorbing layer, 1/s;
+* ``air_strength`` — the relaxation strengths ``Delta`` of the air model, one per process.
+
+The backward pass is the hand-written adjoint of ``csrc/adjoint.hip``: the transposed,
+time-reversed leapfrog, two kernels per step like the forward. The adjoint recurrence needs no
+forward state (the step is linear in it); the parameter gradients do:
+
+* walls need ``p_tmp + p_old`` on the boundary only, which the forward kernels write to a tape
+  of ``2 (ny nz + nx nz + nx ny)`` values per step, so wall and source gradients cost one
+  forward pass plus one adjoint pass;
+* the sponge and air strengths need the forward fields at every step. The forward pass then
+  keeps a checkpoint of the full state every ``checkpoint_every`` steps, and the backward pass
+  recomputes each segment from its checkpoint, keeping that segment's fields, before running
+  the adjoint through it.
+
+There is no reference counterpart: the reference's PyTorch backend uses ``out=`` operations and
+host-float coefficients and cannot be differentiated (``src/ac_fdtd/torch_backend.py:149-195``,
+``:278-309``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native
+from .air import AirAbsorption
+from .boundaries import AbsorbingLayer, WallAdmittances, layer_weights
+from .coefficients import build_coefficients
+from .grid import Grid
+from .medium import AIR, Medium
+from .scheme import AcousticFDTD
+
+__all__ = ["RoomResponse", "GradientReport"]
+
+
+@dataclass
+class GradientReport:
+    """What the last backward pass did: adjoint steps, recomputed steps, state checkpoints."""
+
+    adjoint_steps: int = 0
+    recomputed_steps: int = 0
+    checkpoints: int = 0
+    checkpoint_bytes: int = 0
+    tape_bytes: int = 0
+
+
+class RoomResponse:
+    """A differentiable room: parameters in, receiver recording out.
+
+    Args:
+        grid, medium, courant, walls, absorbing_layer, air_absorption: as
+            :class:`~ac_fdtd_amd.scheme.AcousticFDTD`; these fix everything that is not passed
+            to :meth:`__call__`.
+        n_steps: steps per evaluation.
+        sources: ``[(point, signal), ...]``; the signals are the defaults for ``signal``.
+        receivers: receiver points.
+        checkpoint_every: state checkpoint interval K, used when sponge or air gradients are
+            requested (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field sets).
+    """
+
+    def __init__(
+        self,
+        grid: Grid,
+        n_steps: int,
+        sources,
+        receivers,
+        medium: Medium = AIR,
+        courant: float = 1.0,
+        dtype=np.float32,
+        walls: WallAdmittances | None = None,
+        absorbing_layer: AbsorbingLayer | None = None,
+        air_absorption: AirAbsorption | None = None,
+        device="cuda",
+        checkpoint_every: int = 100,
+        fused: bool = True,
+        compensated: bool = False,
+    ) -> None:
+        if n_steps < 1:
+            raise ValueError("n_steps must be >= 1")
+        if checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.grid, self.medium, self.courant = grid, medium, courant
+        self.n_steps = int(n_steps)
+        self.dtype = np.dtype(dtype) if not isinstance(dtype, torch.dtype) else (
+            np.dtype(np.float32) if dtype == torch.float32 else np.dtype(np.float64)
+        )
+        self.walls = walls or WallAdmittances()
+        self.absorbing_layer = absorbing_layer
+        self.air_absorption = air_absorption
+        self.device = device
+        self.checkpoint_every = int(checkpoint_every)
+        self.fused, self.compensated = fused, compensated
+        self.source_points = [point for point, _ in sources]
+        self.signals = [np.asarray(signal, dtype=np.float64) for _, signal in sources]
+        self.receivers = list(receivers)
+        base = build_coefficients(grid, medium, courant, np.float64, self.walls,
+                                  absorbing_layer, air_absorption)
+        self.dt = base.dt
+        self.base_sponge_strength = base.sponge_strength
+        self.base_air_strengths = [q.strength for q in base.relaxation]
+        self.report = GradientReport()
+
+    # -- defaults for the parameters, as tensors ---------------------------------------------
+    def default_admittance(self, **options) -> torch.Tensor:
+        return torch.tensor(self.walls.values, dtype=torch.float64, **options)
+
+    def default_signal(self, **options) -> torch.Tensor:
+        length = max((s.size for s in self.signals), default=0)
+        out = torch.zeros((len(self.signals), length), dtype=torch.float64, **options)
+        for n, s in enumerate(self.signals):
+            out[n, : s.size] = torch.as_tensor(s)
+        return out
+
+    def default_sponge_strength(self, **options) -> torch.Tensor:
+        if self.base_sponge_strength is None:
+            raise ValueError("the room has no absorbing layer")
+        return torch.tensor(self.base_sponge_strength, dtype=torch.float64, **options)
+
+    def default_air_strength(self, **options) -> torch.Tensor:
+        if not self.base_air_strengths:
+            raise ValueError("the room has no air absorption")
+        return torch.tensor(self.base_air_strengths, dtype=torch.float64, **options)
+
+    def __call__(self, admittance=None, signal=None, sponge_strength=None, air_strength=None):
+        """The ``(n_receivers, n_steps)`` recording, differentiable in every tensor given."""
+        return _RoomResponseFunction.apply(self, admittance, signal, sponge_strength, air_strength)
+
+    # -- building blocks ------------------------------------------------------------------------
+    def solver(self, admittance=None, signal=None, sponge_strength=None, air_strength=None,
+               wall_table_always: bool = False) -> AcousticFDTD:
+        """A forward solver with these parameter values (plain floats / arrays or tensors)."""
+        xi = self.walls.values if admittance is None else tuple(
+            float(x) for x in torch.as_tensor(admittance).detach().cpu().double().tolist()
+        )
+        sigma = None if sponge_strength is None else float(torch.as_tensor(sponge_strength))
+        delta = None if air_strength is None else [
+            float(x) for x in torch.as_tensor(air_strength).detach().cpu().double().tolist()
+        ]
+        solver = AcousticFDTD(
+            self.grid,
+            medium=self.medium,
+            courant=self.courant,
+            dtype=self.dtype,
+            walls=WallAdmittances(*xi),
+            absorbing_layer=self.absorbing_layer,
+            air_absorption=self.air_absorption,
+            device=self.device,
+            compensated=self.compensated,
+            fused=self.fused,
+            sponge_strength=sigma,
+            relaxation_strengths=delta,
+            wall_table_always=wall_table_always,
+        )
+        signals = self.signals if signal is None else list(
+            torch.as_tensor(signal).detach().cpu().double().numpy()
+        )
+        if len(signals) != len(self.source_points):
+            raise ValueError(f"expected {len(self.source_points)} source signals")
+        for point, s in zip(self.source_points, signals):
+            solver.add_volume_source(point, s)
+        for point in self.receivers:
+            solver.add_pressure_receiver(point)
+        return solver
+
+
+class _RoomResponseFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, room: RoomResponse, admittance, signal, sponge_strength, air_strength):
+        need = ctx.needs_input_grad
+        want_walls = admittance is not None and need[1]
+        want_signal = need[2]
+        want_sponge = sponge_strength is not None and need[3]
+        want_air = air_strength is not None and need[4]
+        if want_sponge and room.absorbing_layer is None:
+            raise ValueError("sponge_strength given but the room has no absorbing layer")
+        if want_air and room.air_absorption is None:
+            raise ValueError("air_strength given but the room has no air absorption")
+        solver = room.solver(admittance, signal, sponge_strength, air_strength,
+                             wall_table_always=want_walls)
+        report = GradientReport()
+        if want_walls:
+            tape = torch.empty((room.n_steps, solver.wall_tape_stride()), dtype=solver.dtype,
+                               device=solver.device)
+            solver.attach_wall_tape(tape, 0)
+            report.tape_bytes = tape.numel() * tape.element_size()
+        checkpoints = []
+        if want_sponge or want_air:
+            K = room.checkpoint_every
+            for start in range(0, room.n_steps, K):
+                checkpoints.append(solver.save_state())
+                solver.run(min(K, room.n_steps - start))
+            report.checkpoints = len(checkpoints)
+            report.checkpoint_bytes = sum(t.numel() * t.element_size() for c in checkpoints for t in c)
+        else:
+            solver.run(room.n_steps)
+        recording = solver.recording_tensor()
+        ctx.room, ctx.solver, ctx.checkpoints, ctx.report = room, solver, checkpoints, report
+        ctx.want = (want_walls, want_signal, want_sponge, want_air)
+        ctx.inputs = (admittance, signal, sponge_strength, air_strength)
+        return recording
+
+    @staticmethod
+    def backward(ctx, grad_recording):
+        room, solver = ctx.room, ctx.solver
+        want_walls, want_signal, want_sponge, want_air = ctx.want
+        grads = _run_adjoint(room, solver, grad_recording, ctx.checkpoints, ctx.want, ctx.report)
+        room.report = ctx.report
+        admittance, signal, sponge_strength, air_strength = ctx.inputs
+
+        def like(value, reference):
+            if value is None or reference is None:
+                return None
+            return torch.as_tensor(value).to(dtype=reference.dtype, device=reference.device)
+
+        out_signal = None
+        if want_signal and signal is not None:
+            g = grads["signal"][:, : signal.shape[1]]
+            out_signal = like(g, signal)
+        ctx.solver = ctx.checkpoints = None
+        return (
+            None,
+            like(grads.get("admittance"), admittance) if want_walls else None,
+            out_signal,
+            like(grads.get("sponge_strength"), sponge_strength) if want_sponge else None,
+            like(grads.get("air_strength"), air_strength) if want_air else None,
+        )
+
+
+def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, report):
+    """Transpose every step of the forward run, newest first; return the parameter gradients."""
+    want_walls, want_signal, want_sponge, want_air = want
+    lib = solver._lib
+    g, k = solver.grid, solver.coefficients
+    opts = {"device": solver.device, "dtype": solver.dtype}
+    n_steps = room.n_steps
+    lam = [torch.zeros(g.shape, **opts)] + [torch.zeros(g.velocity_shape(a), **opts) for a in range(3)]
+    lpsi = torch.zeros((k.n_relax, *g.shape), **opts)
+    ldiv = torch.zeros(g.shape, **opts)
+    G = grad_recording.to(**opts).contiguous()
+    recording = solver.recording_tensor()
+    fwd = solver._descriptor(recording)
+    n_partials = lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd))
+    partials = torch.zeros(n_partials, dtype=torch.float64, device=solver.device)
+    sources = solver._source_arrays()
+    grad_signal = None
+    if want_signal and sources is not None:
+        grad_signal = torch.zeros(sources[2].shape, **opts)
+
+    adj = _native.AdjointDesc()
+    adj.abi_version = _native.ABI_VERSION
+    adj.struct_size = ctypes.sizeof(_native.AdjointDesc)
+    adj.grads = (
+        (_native.GRAD_WALLS if want_walls else 0)
+        | (_native.GRAD_SPONGE if want_sponge else 0)
+        | (_native.GRAD_AIR if want_air else 0)
+    )
+    adj.lp, adj.lvx, adj.lvy, adj.lvz = (t.data_ptr() for t in lam)
+    adj.lpsi = lpsi.data_ptr() if k.n_relax else None
+    adj.ldiv = ldiv.data_ptr()
+    adj.grad_recording = G.data_ptr() if G.numel() else None
+    adj.grad_recording_stride = G.shape[1] if G.dim() == 2 else 0
+    adj.grad_recording_step0 = 0
+    adj.grad_signal = grad_signal.data_ptr() if grad_signal is not None else None
+    adj.dt = solver.dt
+    adj.partials = partials.data_ptr()
+    weights = None
+    if want_sponge:
+        cells, faces = layer_weights(g, room.absorbing_layer)
+        weights = [torch.as_tensor(w, **opts) for w in (*cells, *faces)]
+        for a in range(3):
+            adj.weight_cell[a] = weights[a].data_ptr()
+            adj.weight_face[a] = weights[3 + a].data_ptr()
+
+    stream = torch.cuda.current_stream(solver.device).cuda_stream
+    handle = ctypes.c_void_p(stream)
+
+    def set_forward(state):
+        if state is None:
+            adj.fwd_p = adj.fwd_vx = adj.fwd_vy = adj.fwd_vz = None
+        else:
+            adj.fwd_p, adj.fwd_vx, adj.fwd_vy, adj.fwd_vz = (t.data_ptr() for t in state)
+
+    def cell(step, stencil, local, state):
+        set_forward(state)
+        _native.check(lib.acfdtd_hip_adjoint_cell_sweep(ctypes.byref(fwd), ctypes.byref(adj),
+                                                        step, stencil, local, handle),
+                      "acfdtd_hip_adjoint_cell_sweep")
+
+    def face(state):
+        set_forward(state)
+        _native.check(lib.acfdtd_hip_adjoint_face_sweep(ctypes.byref(fwd), ctypes.byref(adj),
+                                                        handle), "acfdtd_hip_adjoint_face_sweep")
+
+    def transpose(n, state_before, state_after, last_state):
+        """Adjoint step n; state_before = x^n, state_after = x^{n+1}."""
+        if n == n_steps - 1:
+            cell(n_steps - 1, 0, 1, last_state)
+        face(state_after)
+        cell(n - 1, 1, 1 if n >= 1 else 0, state_before)
+
+    with torch.cuda.device(solver.device):
+        if checkpoints:
+            K = room.checkpoint_every
+            for seg in range(len(checkpoints) - 1, -1, -1):
+                start, stop = seg * K, min(seg * K + K, n_steps)
+                solver.load_state(checkpoints[seg], start)
+                states = [solver.save_state(with_relaxation=False)]
+                for _ in range(start, stop):
+                    solver.run(1, record=False)
+                    states.append(solver.save_state(with_relaxation=False))
+                report.recomputed_steps += stop - start
+                for n in range(stop - 1, start - 1, -1):
+                    transpose(n, states[n - start], states[n + 1 - start], states[-1])
+                del states
+        else:
+            for n in range(n_steps - 1, -1, -1):
+                transpose(n, None, None, None)
+    report.adjoint_steps = n_steps
+
+    sums = partials.view(-1, _native.GRAD_SLOTS).sum(dim=0).cpu().numpy()
+    out = {"adjoint_state": lam}
+    c, dx, dt, rho = solver.medium.sound_speed, g.dx, solver.dt, solver.medium.density
+    if want_walls:
+        out["admittance"] = sums[0:6] * (c * dt / (2.0 * dx))
+    if want_sponge:
+        out["sponge_strength"] = np.float64(sums[6])
+    if want_air:
+        delta = np.array([q.strength for q in k.relaxation])
+        total = 1.0 + delta.sum()
+        # gain_r = rho c^2 Delta_r / ((1 + sum Delta) dx);  relax_scale = dt sum_r gain_r
+        jac = (rho * c**2 / dx) * (np.eye(delta.size) / total - delta[:, None] / total**2)
+        d_gain = sums[8 : 8 + delta.size] + sums[7] * dt
+        out["air_strength"] = jac.T @ d_gain
+    if grad_signal is not None:
+        out["signal"] = grad_signal
+    return out

@@ -40,6 +40,7 @@ __all__ = [
     "axis_wall_codes",
     "edge_slabs",
     "layer_factors",
+    "layer_weights",
     "random_incidence_absorption",
     "reflection_coefficient",
     "wall_coefficient_table",
@@ -220,18 +221,16 @@ class AbsorbingLayer:
         return -(self.order + 1.0) * sound_speed * math.log(self.target_reflection) / (2.0 * depth)
 
 
-def layer_factors(
-    grid: Grid, medium: Medium, dt: float, layer: AbsorbingLayer
-) -> tuple[list[np.ndarray], list[list[np.ndarray]]]:
-    """Per-axis 1D damping factors: ``(pressure[axis], velocity[component][axis])``.
+def layer_weights(grid: Grid, layer: AbsorbingLayer) -> tuple[list[np.ndarray], list[np.ndarray]]:
+    """Grading ``inside^order`` per axis at cell centres and on faces (0 outside the layer).
 
-    ``velocity[c][a]`` is sampled on faces when ``a == c`` and at cell centres otherwise, so it
-    equals ``pressure[a]`` for ``a != c``. All factors are fp64 (boundaries.py:244-283).
+    The sponge factor at a position is ``exp(-sigma_max * weight * dt)``; the weights are the
+    part that does not depend on the damping strength, so gradients with respect to
+    ``sigma_max`` are ``-weight * dt`` times the factor.
     """
-    sigma_max = layer.max_damping(grid.dx, medium.sound_speed)
     depth = layer.thickness * grid.dx
 
-    def profile(positions: np.ndarray, axis: int) -> np.ndarray:
+    def weight(positions: np.ndarray, axis: int) -> np.ndarray:
         extent = grid.lengths[axis]
         distance = np.full(positions.shape, np.inf)
         if (axis, 0) in layer.faces:
@@ -239,18 +238,38 @@ def layer_factors(
         if (axis, 1) in layer.faces:
             distance = np.minimum(distance, extent - positions)
         inside = np.clip(1.0 - distance / depth, 0.0, 1.0)
-        return np.exp(-sigma_max * inside**layer.order * dt)
+        return inside**layer.order
 
-    pressure = [profile(grid.cell_centres(axis), axis) for axis in range(3)]
+    cells = [weight(grid.cell_centres(axis), axis) for axis in range(3)]
+    faces = [weight(np.arange(grid.shape[axis] + 1) * grid.dx, axis) for axis in range(3)]
+    return cells, faces
+
+
+def layer_factors(
+    grid: Grid,
+    medium: Medium,
+    dt: float,
+    layer: AbsorbingLayer,
+    sigma_max: float | None = None,
+) -> tuple[list[np.ndarray], list[list[np.ndarray]]]:
+    """Per-axis 1D damping factors: ``(pressure[axis], velocity[component][axis])``.
+
+    ``velocity[c][a]`` is sampled on faces when ``a == c`` and at cell centres otherwise, so it
+    equals ``pressure[a]`` for ``a != c``. All factors are fp64 (boundaries.py:244-283).
+    ``sigma_max`` overrides the layer's own damping strength.
+    """
+    if sigma_max is None:
+        sigma_max = layer.max_damping(grid.dx, medium.sound_speed)
+    cells, faces = layer_weights(grid, layer)
+    pressure = [np.exp(-sigma_max * w * dt) for w in cells]
     velocity = []
     for component in range(3):
-        per_axis = []
-        for axis in range(3):
-            if axis == component:
-                per_axis.append(profile(np.arange(grid.shape[axis] + 1) * grid.dx, axis))
-            else:
-                per_axis.append(profile(grid.cell_centres(axis), axis))
-        velocity.append(per_axis)
+        velocity.append(
+            [
+                np.exp(-sigma_max * faces[axis] * dt) if axis == component else pressure[axis]
+                for axis in range(3)
+            ]
+        )
     return pressure, velocity
 
 

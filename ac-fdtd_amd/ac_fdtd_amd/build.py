@@ -25,8 +25,10 @@ from pathlib import Path
 PACKAGE_DIR = Path(__file__).resolve().parent
 SOURCE_DIR = PACKAGE_DIR.parent / "csrc"
 INCLUDE_DIR = PACKAGE_DIR.parent.parent / "include"
-LIBRARY = PACKAGE_DIR / "lib" / "libacfdtd_hip.so"
-SOURCES = ("sweeps.hip",)
+#: The library the loader opens; ACFDTD_HIP_LIBRARY points it at another build of the same
+#: sources (tile experiments), the default is the in-tree build.
+LIBRARY = Path(os.environ.get("ACFDTD_HIP_LIBRARY", PACKAGE_DIR / "lib" / "libacfdtd_hip.so"))
+SOURCES = ("sweeps.hip", "adjoint.hip")
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 
 FLAGS = [

@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .air import AirAbsorption
+from .air import AirAbsorption, RelaxationProcess
 from .boundaries import AbsorbingLayer, WallAdmittances, layer_factors, wall_coefficient_table
 from .grid import Grid, max_stable_time_step
 from .medium import Medium
@@ -43,6 +43,8 @@ class SchemeCoefficients:
     wall_from_previous: np.ndarray | None = None  # (64,) working precision
     cell_profiles: list[np.ndarray] | None = None  # 3 x (n_a,), sponge at cell centres
     face_profiles: list[np.ndarray] | None = None  # 3 x (n_a + 1,), sponge on faces
+    sponge_strength: float | None = None  # sigma_max of the layer, 1/s
+    relaxation: tuple[RelaxationProcess, ...] = ()  # the processes the gains came from
 
     @property
     def n_relax(self) -> int:
@@ -69,7 +71,15 @@ def build_coefficients(
     walls: WallAdmittances | None,
     absorbing_layer: AbsorbingLayer | None,
     air_absorption: AirAbsorption | None,
+    sponge_strength: float | None = None,
+    relaxation_strengths=None,
+    wall_table_always: bool = False,
 ) -> SchemeCoefficients:
+    """Coefficients of the time step. ``sponge_strength`` (sigma_max) and
+    ``relaxation_strengths`` (one Delta per process) override the values the layer and the air
+    model would give, for differentiating with respect to them; ``wall_table_always`` builds
+    the wall table even for rigid walls (needed by the adjoint, which differentiates every
+    face)."""
     if not 0.0 < courant <= 1.0:
         raise ValueError(f"courant must be in (0, 1], got {courant}")
     dtype = np.dtype(dtype)
@@ -89,6 +99,15 @@ def build_coefficients(
 
     if air_absorption is not None:
         processes = air_absorption.processes(c, dt)
+        if relaxation_strengths is not None:
+            strengths = [float(x) for x in relaxation_strengths]
+            if len(strengths) != len(processes):
+                raise ValueError(f"expected {len(processes)} relaxation strengths")
+            processes = tuple(
+                RelaxationProcess(strength=x, relaxation_time=q.relaxation_time)
+                for x, q in zip(strengths, processes)
+            )
+        out.relaxation = tuple(processes)
         if len(processes) > MAX_RELAXATION:
             raise ValueError(f"at most {MAX_RELAXATION} relaxation processes are supported")
         relaxed_c2 = c**2 / (1.0 + sum(process.strength for process in processes))
@@ -104,14 +123,17 @@ def build_coefficients(
         # Summed over the rounded gains, in the working precision (see module docstring).
         out.relax_scale = _round(dtype, dt * sum(rounded_gains))
 
-    if not walls.is_rigid:
+    if not walls.is_rigid or wall_table_always:
         is_wall, updated, previous = wall_coefficient_table(grid, medium, dt, walls)
         out.wall_is_wall = is_wall
         out.wall_from_updated = updated.astype(dtype)
         out.wall_from_previous = previous.astype(dtype)
 
     if absorbing_layer is not None:
-        pressure, velocity = layer_factors(grid, medium, dt, absorbing_layer)
+        if sponge_strength is None:
+            sponge_strength = absorbing_layer.max_damping(dx, c)
+        out.sponge_strength = float(sponge_strength)
+        pressure, velocity = layer_factors(grid, medium, dt, absorbing_layer, sponge_strength)
         out.cell_profiles = [pressure[a].astype(dtype) for a in range(3)]
         out.face_profiles = [velocity[a][a].astype(dtype) for a in range(3)]
     return out

@@ -11,6 +11,12 @@ built library the constructor raises.
 Precision: ``float64`` runs the plain kernels, which reproduce the NumPy reference bit for bit;
 ``float32`` runs the same kernels in single precision, and ``compensated=True`` switches the
 fp32 pressure increment to error-free summation (see ``csrc/sweeps.hip``).
+
+Kernels: with ``fused=True`` (the default) the solver keeps a second set of ``p`` and velocity
+buffers and the library advances steps in pairs with one fused sweep per step (A -> B -> A),
+moving 8 + 2 n_relax instead of 12 + 2 n_relax field elements per cell-update; ``fused=False``
+uses the two in-place sweeps. Both give the same bits, and the state always ends in the
+public tensors.
 """
 
 from __future__ import annotations
@@ -61,6 +67,7 @@ class AcousticFDTD:
         dtype: ``float64`` (default, the reference's precision) or ``float32``; numpy or torch.
         device: a ROCm GPU, ``"cuda"`` / ``"cuda:N"``.
         compensated: fp32 only — error-free summation of the pressure increment.
+        fused: one sweep per step through a second set of buffers (default), or two in place.
         tile_j, chunk_x: kernel tiling (rows per workgroup, planes per march); 0 = default.
     """
 
@@ -75,8 +82,12 @@ class AcousticFDTD:
         air_absorption: AirAbsorption | None = None,
         device="cuda",
         compensated: bool = False,
+        fused: bool = True,
         tile_j: int = 0,
         chunk_x: int = 0,
+        sponge_strength: float | None = None,
+        relaxation_strengths=None,
+        wall_table_always: bool = False,
     ) -> None:
         self.device = _require_gpu(device)
         self._lib = _native.library()
@@ -88,7 +99,16 @@ class AcousticFDTD:
         self.air_absorption = air_absorption
         self.np_dtype = _numpy_dtype(dtype)
         self.coefficients = build_coefficients(
-            grid, medium, courant, self.np_dtype, self.walls, absorbing_layer, air_absorption
+            grid,
+            medium,
+            courant,
+            self.np_dtype,
+            self.walls,
+            absorbing_layer,
+            air_absorption,
+            sponge_strength=sponge_strength,
+            relaxation_strengths=relaxation_strengths,
+            wall_table_always=wall_table_always,
         )
         if compensated and self.np_dtype != np.float32:
             raise ValueError("compensated summation is an fp32 mode")
@@ -103,13 +123,20 @@ class AcousticFDTD:
         self.velocity = [torch.zeros(grid.velocity_shape(a), **opts) for a in range(3)]
         k = self.coefficients
         self.relaxation_states = torch.zeros((k.n_relax, *grid.shape), **opts)
+        self.fused = fused
+        self._swap = None
+        if fused:
+            self._swap = [torch.zeros(grid.shape, **opts)] + [
+                torch.zeros(grid.velocity_shape(a), **opts) for a in range(3)
+            ]
 
         self._wall_table = None
         self._wall_mask = 0
-        if k.has_walls:
+        if k.wall_is_wall is not None:
             table = np.concatenate([k.wall_from_updated, k.wall_from_previous])
             self._wall_table = torch.as_tensor(table, **opts)
             self._wall_mask = int(sum(1 << c for c in range(64) if k.wall_is_wall[c]))
+        self._wall_tape: tuple[torch.Tensor, int] | None = None
         self._cell = self._face = None
         if k.has_layer:
             self._cell = [torch.as_tensor(x, **opts) for x in k.cell_profiles]
@@ -195,15 +222,18 @@ class AcousticFDTD:
     def step(self) -> None:
         self.run(1)
 
-    def run(self, n_steps: int) -> None:
-        """Advance ``n_steps`` on the current stream (asynchronous, like any GPU launch)."""
+    def run(self, n_steps: int, record: bool = True) -> None:
+        """Advance ``n_steps`` on the current stream (asynchronous, like any GPU launch).
+
+        ``record=False`` skips the receivers (and an attached boundary tape) for these steps,
+        as a recomputation from a checkpoint does."""
         n_steps = int(n_steps)
         if n_steps <= 0:
             return
         recording = None
-        if self._receivers:
+        if self._receivers and record:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
-        desc = self._descriptor(recording)
+        desc = self._descriptor(recording, tape=record)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         with torch.cuda.device(self.device):
             status = self._lib.acfdtd_hip_run(
@@ -214,40 +244,50 @@ class AcousticFDTD:
         if recording is not None:
             self._runs.append(recording)
 
-    def timed_steps(self, n_steps: int) -> tuple[float, float]:
+    def timed_steps(self, n_steps: int) -> dict[str, float]:
         """Advance ``n_steps`` like :meth:`run`, one kernel per library call, with HIP events
-        around every sweep on the launch stream; returns the mean velocity-sweep and
-        pressure-sweep durations in ms. The state advances exactly as :meth:`run` would."""
+        around every kernel on the launch stream; returns the mean duration in ms of each
+        kernel kind used (``"fused"`` for the fused step, or ``"velocity"`` and ``"pressure"``
+        for the two sweeps). The state advances exactly as :meth:`run` would."""
         n_steps = int(n_steps)
         if n_steps <= 0:
-            return 0.0, 0.0
+            return {}
         recording = None
         if self._receivers:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
         desc = self._descriptor(recording)
         stream = torch.cuda.current_stream(self.device)
         handle = ctypes.c_void_p(stream.cuda_stream)
-        events = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_steps + 1)]
+        ref = ctypes.byref(desc)
+        launches = []  # (kind, call)
+        n = 0
+        if self.fused:
+            for n in range(0, n_steps - 1, 2):
+                for half in (0, 1):
+                    launches.append(("fused", lambda s=n + half, r=half: _native.check(
+                        self._lib.acfdtd_hip_fused_step(ref, self.step_index + s, s, r, handle),
+                        "acfdtd_hip_fused_step")))
+            n = n_steps - n_steps % 2
+        for s in range(n, n_steps):
+            launches.append(("velocity", lambda: _native.check(
+                self._lib.acfdtd_hip_velocity_sweep(ref, handle), "acfdtd_hip_velocity_sweep")))
+            launches.append(("pressure", lambda s=s: _native.check(
+                self._lib.acfdtd_hip_pressure_sweep(ref, self.step_index + s, s, handle),
+                "acfdtd_hip_pressure_sweep")))
+        events = [torch.cuda.Event(enable_timing=True) for _ in range(len(launches) + 1)]
         with torch.cuda.device(self.device):
             events[0].record(stream)
-            for n in range(n_steps):
-                _native.check(self._lib.acfdtd_hip_velocity_sweep(ctypes.byref(desc), handle),
-                              "acfdtd_hip_velocity_sweep")
-                events[2 * n + 1].record(stream)
-                _native.check(
-                    self._lib.acfdtd_hip_pressure_sweep(
-                        ctypes.byref(desc), self.step_index + n, n, handle
-                    ),
-                    "acfdtd_hip_pressure_sweep",
-                )
-                events[2 * n + 2].record(stream)
+            for e, (_, call) in enumerate(launches):
+                call()
+                events[e + 1].record(stream)
         events[-1].synchronize()
         self.step_index += n_steps
         if recording is not None:
             self._runs.append(recording)
-        velocity = sum(events[2 * n].elapsed_time(events[2 * n + 1]) for n in range(n_steps))
-        pressure = sum(events[2 * n + 1].elapsed_time(events[2 * n + 2]) for n in range(n_steps))
-        return velocity / n_steps, pressure / n_steps
+        totals: dict[str, list[float]] = {}
+        for e, (kind, _) in enumerate(launches):
+            totals.setdefault(kind, []).append(events[e].elapsed_time(events[e + 1]))
+        return {kind: sum(v) / len(v) for kind, v in totals.items()}
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)
@@ -306,7 +346,43 @@ class AcousticFDTD:
             )
         return self._source_tensors
 
-    def _descriptor(self, recording: torch.Tensor | None) -> _native.Desc:
+    # -- state, tapes and recordings for the adjoint ------------------------------------------
+    def save_state(self, with_relaxation: bool = True) -> list[torch.Tensor]:
+        """Copies of p, vx, vy, vz (and the relaxation states), e.g. a checkpoint."""
+        fields = [self.p, *self.velocity]
+        if with_relaxation:
+            fields.append(self.relaxation_states)
+        return [f.clone() for f in fields]
+
+    def load_state(self, state: list[torch.Tensor], step_index: int) -> None:
+        for target, source in zip([self.p, *self.velocity, self.relaxation_states], state):
+            target.copy_(source)
+        self.step_index = int(step_index)
+
+    def attach_wall_tape(self, tape: torch.Tensor | None, step0: int = 0) -> None:
+        """Record ``p_tmp + p_old`` on the boundary into ``tape`` (rows of
+        :meth:`wall_tape_stride` elements, one per step from ``step0``); None detaches."""
+        if tape is not None:
+            if tape.dim() != 2 or tape.shape[1] < self.wall_tape_stride() or not tape.is_contiguous():
+                raise ValueError("wall tape must be a contiguous (rows, wall_tape_stride) tensor")
+            if tape.dtype != self.dtype or tape.device != self.device:
+                raise ValueError(f"wall tape must be {self.dtype} on {self.device}")
+        self._wall_tape = None if tape is None else (tape, int(step0))
+
+    def wall_tape_stride(self) -> int:
+        return int(self._lib.acfdtd_hip_wall_tape_stride(*self.grid.shape))
+
+    def recording_tensor(self) -> torch.Tensor:
+        """All samples so far as one ``(channel, sample)`` device tensor (equal lengths)."""
+        if not self._receivers:
+            return torch.zeros((0, 0), **self._opts)
+        if len(set(self._receiver_first_run)) > 1:
+            raise ValueError("receivers were added part-way through the run")
+        return torch.cat(self._runs, dim=1) if self._runs else torch.zeros(
+            (len(self._receivers), 0), **self._opts
+        )
+
+    def _descriptor(self, recording: torch.Tensor | None, tape: bool = True) -> _native.Desc:
         g, k = self.grid, self.coefficients
         d = _native.Desc()
         d.abi_version = _native.ABI_VERSION
@@ -350,4 +426,14 @@ class AcousticFDTD:
             d.receiver_cell = self._receiver_tensor.data_ptr()
             d.recording = recording.data_ptr()
             d.recording_stride = recording.shape[1]
+        if tape and self._wall_tape is not None:
+            buffer, step0 = self._wall_tape
+            d.wall_tape = buffer.data_ptr()
+            d.wall_tape_stride = buffer.shape[1]
+            d.wall_tape_step0 = step0
+            d.wall_tape_rows = buffer.shape[0]
+        if self._swap is not None:
+            d.p_swap = self._checked(self._swap[0], g.shape, "p swap")
+            for a, name in enumerate(("vx_swap", "vy_swap", "vz_swap")):
+                setattr(d, name, self._checked(self._swap[a + 1], g.velocity_shape(a), name))
         return d

@@ -1,0 +1,424 @@
+// The adjoint (transposed, time-reversed) leapfrog and the parameter gradients it yields.
+//
+// The step of sweeps.hip is linear in the state (p, v, psi) and in the source samples, so its
+// transpose needs no forward state; only the gradients with respect to the coefficients do.
+// With lam_X the adjoint of X, one forward step n transposes to (derivation in DESIGN.md):
+//
+//   cells, local part of step n  (reverse order of the forward pressure update)
+//     lam_p'' = lam_p + G[:, n] at receivers           recording r_n = p^{n+1}[receiver]
+//     grad_signal[s, n] = src_coef * lam_p''           p^{n+1} = p'' + src_coef * signal[n]
+//     lam_hat = P * lam_p''                            p'' = P * p_hat      (sponge, cells)
+//     lam_til = fu * lam_hat, lam_p = -fp * lam_hat    p_hat = fu * p_til - fp * p^n (walls)
+//     lam_p += lam_til                                 p_til = p^n + ...
+//     lam_psi_r <- mean_r lam_til + decay_r lam_psi_r
+//     lam_div = lam_til (relax_scale - pres - sum mean_r gain_r)
+//             + sum_r gain_r (1 - decay_r) lam_psi_r(old)
+//   faces   lam_v <- D * (lam_v + lam_div[lower cell] - lam_div[upper cell])     (v' = D * v~)
+//   cells, stencil part of step n
+//     lam_p += vel * sum_a (lam_v[upper interior face] - lam_v[lower interior face])
+//
+// The stencil part of step n and the local part of step n - 1 touch only the cell itself, so
+// they run in one sweep: two kernels per adjoint step, the same shape and traffic as the
+// forward two-sweep step. Parameter gradients, accumulated in fp64:
+//   walls   dL/da_c  = -lam_hat fu^2 (p_til + p^n)      (p_til + p^n from the boundary tape)
+//   sponge  dL/dsigma += -dt W lam_p'' p''  (cells),  -dt W lam_tot v'  (faces),
+//           W the sum of the grading weights inside^order of the factors in P or D
+//   air     dL/dgain_r += div (-mean_r lam_til + (1 - decay_r) lam_psi_r(old)),
+//           dL/drelax_scale += div lam_til              (div of the forward v^{n+1/2})
+// Each workgroup adds its sums to its own slot of `partials` once per launch, so the totals
+// are deterministic; the host reduces the slots and applies the chain rule to the physical
+// parameters.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "acfdtd_hip.h"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+enum : int { kGradWall = 0, kGradSponge = 6, kGradScale = 7, kGradGain = 8, kGradSlots = 12 };
+
+template <typename real>
+struct AdjointArgs {
+    real *lp, *lvx, *lvy, *lvz, *lpsi, *ldiv;
+    const real *grad_recording;
+    int64_t grad_recording_stride;
+    int64_t grad_recording_step0;
+    real *grad_signal;
+    const real *fwd_p, *fwd_vx, *fwd_vy, *fwd_vz;
+    const real *w_cell[3];
+    const real *w_face[3];
+    double *partials;
+    uint32_t grads;
+    real dt;
+};
+
+template <typename real>
+void fill_adjoint(const acfdtd_hip_adjoint_desc &g, AdjointArgs<real> &a) {
+    a.lp = static_cast<real *>(g.lp);
+    a.lvx = static_cast<real *>(g.lvx);
+    a.lvy = static_cast<real *>(g.lvy);
+    a.lvz = static_cast<real *>(g.lvz);
+    a.lpsi = static_cast<real *>(g.lpsi);
+    a.ldiv = static_cast<real *>(g.ldiv);
+    a.grad_recording = static_cast<const real *>(g.grad_recording);
+    a.grad_recording_stride = g.grad_recording_stride;
+    a.grad_recording_step0 = g.grad_recording_step0;
+    a.grad_signal = static_cast<real *>(g.grad_signal);
+    a.fwd_p = static_cast<const real *>(g.fwd_p);
+    a.fwd_vx = static_cast<const real *>(g.fwd_vx);
+    a.fwd_vy = static_cast<const real *>(g.fwd_vy);
+    a.fwd_vz = static_cast<const real *>(g.fwd_vz);
+    for (int q = 0; q < 3; ++q) {
+        a.w_cell[q] = static_cast<const real *>(g.weight_cell[q]);
+        a.w_face[q] = static_cast<const real *>(g.weight_face[q]);
+    }
+    a.partials = g.partials;
+    a.grads = g.grads;
+    a.dt = (real)g.dt;
+}
+
+// Like decode_tile, also returning the tile's own index (its gradient slot).
+__device__ inline bool decode_tile_slot(const Tiling &t, int &kb, int &jb, int &xc, int &slot) {
+    const unsigned n = (unsigned)t.launched;
+    unsigned b = blockIdx.x;
+    b = (b & 7u) * (n >> 3) + (b >> 3);
+    if (b >= (unsigned)t.tiles) return false;
+    slot = (int)b;
+    kb = (int)(b % (unsigned)t.nblk_k);
+    b /= (unsigned)t.nblk_k;
+    jb = (int)(b % (unsigned)t.nblk_j);
+    xc = (int)(b / (unsigned)t.nblk_j);
+    return true;
+}
+
+// Add each thread's kGradSlots accumulators into partials[slot * kGradSlots + q], in a fixed
+// order. Every thread of the workgroup must call it.
+__device__ inline void flush_partials(double (&acc)[kGradSlots], bool active, int slot,
+                                      double *partials) {
+    __shared__ double lds[kGradSlots][8];
+    for (int q = 0; q < kGradSlots; ++q) {
+        double v = wave_sum(acc[q]);
+        if (threadIdx.x == 0) lds[q][threadIdx.y] = v;
+    }
+    __syncthreads();
+    if (active && threadIdx.x == 0 && threadIdx.y == 0) {
+        for (int q = 0; q < kGradSlots; ++q) {
+            double v = 0;
+            for (unsigned w = 0; w < blockDim.y; ++w) v += lds[q][w];
+            if (v != 0) partials[(int64_t)slot * kGradSlots + q] += v;
+        }
+    }
+}
+
+template <typename real, bool LAYER>
+__global__ __launch_bounds__(512) void adjoint_face_sweep(const SweepArgs<real> a,
+                                                          const AdjointArgs<real> g) {
+    double acc[kGradSlots] = {};
+    int kb = 0, jb = 0, xc = 0, slot = 0;
+    const bool tile = decode_tile_slot(a.tiles, kb, jb, xc, slot);
+    const int k = kb * 64 + (int)threadIdx.x;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    const bool active = tile && k < a.nz && j < a.ny;
+    const bool sponge = LAYER && (g.grads & ACFDTD_HIP_GRAD_SPONGE) && g.fwd_vx;
+    if (active) {
+        const int i0 = xc * a.tiles.chunk;
+        const int i1 = min(i0 + a.tiles.chunk, a.nx);
+        const Layout L(a.nx, a.ny, a.nz, j, k);
+        real fyc = 1, fzc = 1, fyf = 1, fzf = 1, wyc = 0, wzc = 0, wyf = 0, wzf = 0;
+        if (LAYER) {
+            fyc = a.cell[1][j];
+            fzc = a.cell[2][k];
+            fyf = a.face[1][j];
+            fzf = a.face[2][k];
+        }
+        if (sponge) {
+            wyc = g.w_cell[1][j];
+            wzc = g.w_cell[2][k];
+            wyf = g.w_face[1][j];
+            wzf = g.w_face[2][k];
+        }
+        const double mdt = -(double)g.dt;
+        real ld_prev = i0 > 0 ? g.ldiv[L.cell(i0 - 1)] : real(0);
+        for (int i = i0; i < i1; ++i) {
+            const int64_t c = L.cell(i);
+            const real ld = g.ldiv[c];
+            real fxc = 1, fxf = 1, wxc = 0, wxf = 0;
+            if (LAYER) {
+                fxc = a.cell[0][i];
+                fxf = a.face[0][i];
+            }
+            if (sponge) {
+                wxc = g.w_cell[0][i];
+                wxf = g.w_face[0][i];
+            }
+            {  // x face i: lower cell i - 1 (none at the wall), upper cell i
+                const real t = g.lvx[c] + ((i > 0 ? ld_prev : real(0)) - ld);
+                if (sponge) acc[kGradSponge] += mdt * (double)(wxf + wyc + wzc) * t * g.fwd_vx[c];
+                g.lvx[c] = LAYER ? ((t * fxf) * fyc) * fzc : t;
+            }
+            const int64_t cy = L.face_y(i);
+            {
+                const real t = g.lvy[cy] + ((j > 0 ? g.ldiv[c - a.nz] : real(0)) - ld);
+                if (sponge) acc[kGradSponge] += mdt * (double)(wxc + wyf + wzc) * t * g.fwd_vy[cy];
+                g.lvy[cy] = LAYER ? ((t * fxc) * fyf) * fzc : t;
+            }
+            const int64_t cz = L.face_z(i);
+            {
+                const real t = g.lvz[cz] + ((k > 0 ? g.ldiv[c - 1] : real(0)) - ld);
+                if (sponge) acc[kGradSponge] += mdt * (double)(wxc + wyc + wzf) * t * g.fwd_vz[cz];
+                g.lvz[cz] = LAYER ? ((t * fxc) * fyc) * fzf : t;
+            }
+            // Far wall faces: only the lower cell exists.
+            if (i == a.nx - 1) {
+                const int64_t f = c + L.plane;
+                const real t = g.lvx[f] + ld;
+                if (sponge)
+                    acc[kGradSponge] +=
+                        mdt * (double)(g.w_face[0][a.nx] + wyc + wzc) * t * g.fwd_vx[f];
+                g.lvx[f] = LAYER ? ((t * a.face[0][a.nx]) * fyc) * fzc : t;
+            }
+            if (j == a.ny - 1) {
+                const int64_t f = cy + a.nz;
+                const real t = g.lvy[f] + ld;
+                if (sponge)
+                    acc[kGradSponge] +=
+                        mdt * (double)(wxc + g.w_face[1][a.ny] + wzc) * t * g.fwd_vy[f];
+                g.lvy[f] = LAYER ? ((t * fxc) * a.face[1][a.ny]) * fzc : t;
+            }
+            if (k == a.nz - 1) {
+                const int64_t f = cz + 1;
+                const real t = g.lvz[f] + ld;
+                if (sponge)
+                    acc[kGradSponge] +=
+                        mdt * (double)(wxc + wyc + g.w_face[2][a.nz]) * t * g.fwd_vz[f];
+                g.lvz[f] = LAYER ? ((t * fxc) * fyc) * a.face[2][a.nz] : t;
+            }
+            ld_prev = ld;
+        }
+    }
+    if (sponge) flush_partials(acc, tile, slot, g.partials);
+}
+
+template <typename real, int NR, bool LAYER>
+__global__ __launch_bounds__(512) void adjoint_cell_sweep(const SweepArgs<real> a,
+                                                          const AdjointArgs<real> g,
+                                                          const int64_t step, const int stencil,
+                                                          const int local) {
+    double acc[kGradSlots] = {};
+    int kb = 0, jb = 0, xc = 0, slot = 0;
+    const bool tile = decode_tile_slot(a.tiles, kb, jb, xc, slot);
+    const int k = kb * 64 + (int)threadIdx.x;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    const bool active = tile && k < a.nz && j < a.ny;
+    const bool walls = local && (g.grads & ACFDTD_HIP_GRAD_WALLS) && a.wall_tape;
+    const bool sponge = local && LAYER && (g.grads & ACFDTD_HIP_GRAD_SPONGE) && g.fwd_p;
+    const bool air = local && NR > 0 && (g.grads & ACFDTD_HIP_GRAD_AIR) && g.fwd_vx;
+    if (active) {
+        const int i0 = xc * a.tiles.chunk;
+        const int i1 = min(i0 + a.tiles.chunk, a.nx);
+        const Layout L(a.nx, a.ny, a.nz, j, k);
+        const int64_t n_cells = L.plane * a.nx;
+        const int jk = j * a.nz + k;
+        const int code_yz = (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4);
+        real fyc = 1, fzc = 1, wyz = 0;
+        if (LAYER) {
+            fyc = a.cell[1][j];
+            fzc = a.cell[2][k];
+        }
+        if (sponge) wyz = g.w_cell[1][j] + g.w_cell[2][k];
+        real mg = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) mg += a.mean[r] * a.gain[r];
+        const real cdiv = (a.relax_scale - a.pres) - mg;
+        const TapePlanes planes(a.nx, a.ny, a.nz);
+        const real *tape_row =
+            walls ? a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride : nullptr;
+
+        for (int i = i0; i < i1; ++i) {
+            const int64_t c = L.cell(i);
+            real lpc = g.lp[c];
+            if (stencil) {
+                const real lo_x = i > 0 ? g.lvx[c] : real(0);
+                const real up_x = i + 1 < a.nx ? g.lvx[c + L.plane] : real(0);
+                const int64_t cy = L.face_y(i);
+                const real lo_y = j > 0 ? g.lvy[cy] : real(0);
+                const real up_y = j + 1 < a.ny ? g.lvy[cy + a.nz] : real(0);
+                const int64_t cz = L.face_z(i);
+                const real lo_z = k > 0 ? g.lvz[cz] : real(0);
+                const real up_z = k + 1 < a.nz ? g.lvz[cz + 1] : real(0);
+                lpc = lpc + a.vel * (((up_x - lo_x) + (up_y - lo_y)) + (up_z - lo_z));
+            }
+            if (!local) {
+                g.lp[c] = lpc;
+                continue;
+            }
+            // Local transpose of step `step`.
+            for (int r = 0; r < a.n_receivers; ++r)
+                if (a.receiver_cell[2 * r] == i && a.receiver_cell[2 * r + 1] == jk)
+                    lpc = lpc + g.grad_recording[r * g.grad_recording_stride +
+                                                 (step - g.grad_recording_step0)];
+            real injected = 0;
+            for (int s = 0; s < a.n_sources; ++s) {
+                if (a.source_cell[2 * s] == i && a.source_cell[2 * s + 1] == jk &&
+                    step < a.source_length[s]) {
+                    if (g.grad_signal) g.grad_signal[s * a.source_stride + step] = a.src * lpc;
+                    if (sponge) injected += a.src * a.source_signal[s * a.source_stride + step];
+                }
+            }
+            const real lhat = LAYER ? ((lpc * a.cell[0][i]) * fyc) * fzc : lpc;
+            if (sponge)
+                acc[kGradSponge] += -(double)g.dt * (double)(g.w_cell[0][i] + wyz) *
+                                    (double)lpc * (double)(g.fwd_p[c] - injected);
+            real ltil = lhat, lp_new = 0;
+            const int code = code_yz | wall_class(i, a.nx);
+            if (code && a.wall_table) {
+                const real fu = a.wall_table[code];
+                const real fp = a.wall_table[ACFDTD_HIP_WALL_CODES + code];
+                ltil = fu * lhat;
+                lp_new = -fp * lhat;
+                if (walls) {
+                    const double ga = -(double)lhat * (double)fu * (double)fu *
+                                      (double)tape_load(tape_row, planes, a.ny, a.nz, i, j, k, code);
+                    if (code & 1) acc[kGradWall + 0] += ga;
+                    if (code & 2) acc[kGradWall + 1] += ga;
+                    if (code & 4) acc[kGradWall + 2] += ga;
+                    if (code & 8) acc[kGradWall + 3] += ga;
+                    if (code & 16) acc[kGradWall + 4] += ga;
+                    if (code & 32) acc[kGradWall + 5] += ga;
+                }
+            }
+            lp_new = lp_new + ltil;
+            real ldv = ltil * cdiv;
+            double div = 0;
+            if (air) {
+                const int64_t cy = L.face_y(i), cz = L.face_z(i);
+                div = (double)(((g.fwd_vx[c + L.plane] - g.fwd_vx[c]) +
+                                (g.fwd_vy[cy + a.nz] - g.fwd_vy[cy])) +
+                               (g.fwd_vz[cz + 1] - g.fwd_vz[cz]));
+                acc[kGradScale] += div * (double)ltil;
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = g.lpsi + (int64_t)r * n_cells + c;
+                const real old = *state;
+                *state = a.mean[r] * ltil + a.decay[r] * old;
+                ldv = ldv + old * (a.gain[r] * (real(1) - a.decay[r]));
+                if (air)
+                    acc[kGradGain + r] +=
+                        div * (-(double)a.mean[r] * (double)ltil +
+                               (1.0 - (double)a.decay[r]) * (double)old);
+            }
+            g.lp[c] = lp_new;
+            g.ldiv[c] = ldv;
+        }
+    }
+    if (walls || sponge || air) flush_partials(acc, tile, slot, g.partials);
+}
+
+template <typename real>
+using FaceKernel = void (*)(const SweepArgs<real>, const AdjointArgs<real>);
+template <typename real>
+using CellKernel = void (*)(const SweepArgs<real>, const AdjointArgs<real>, const int64_t,
+                            const int, const int);
+
+template <typename real>
+CellKernel<real> pick_cell(int nr, bool layer) {
+    switch (nr) {
+        case 0: return layer ? adjoint_cell_sweep<real, 0, true> : adjoint_cell_sweep<real, 0, false>;
+        case 1: return layer ? adjoint_cell_sweep<real, 1, true> : adjoint_cell_sweep<real, 1, false>;
+        case 2: return layer ? adjoint_cell_sweep<real, 2, true> : adjoint_cell_sweep<real, 2, false>;
+        case 3: return layer ? adjoint_cell_sweep<real, 3, true> : adjoint_cell_sweep<real, 3, false>;
+        default: return layer ? adjoint_cell_sweep<real, 4, true> : adjoint_cell_sweep<real, 4, false>;
+    }
+}
+
+int validate_adjoint(const acfdtd_hip_desc *d, const acfdtd_hip_adjoint_desc *g) {
+    const int status = validate(d);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (g == nullptr) return fail(ACFDTD_HIP_EINVAL, "adjoint descriptor is NULL");
+    if (g->abi_version != ACFDTD_HIP_ABI_VERSION || g->struct_size != sizeof(acfdtd_hip_adjoint_desc))
+        return fail(ACFDTD_HIP_EABI, "adjoint descriptor ABI version or size mismatch");
+    if (!g->lp || !g->lvx || !g->lvy || !g->lvz || !g->ldiv)
+        return fail(ACFDTD_HIP_EINVAL, "adjoint field pointer is NULL");
+    if (d->n_relax > 0 && !g->lpsi) return fail(ACFDTD_HIP_EINVAL, "adjoint relaxation states are NULL");
+    if (d->n_receivers > 0 && !g->grad_recording)
+        return fail(ACFDTD_HIP_EINVAL, "recording gradient is NULL");
+    if ((g->grads & ~(ACFDTD_HIP_GRAD_WALLS | ACFDTD_HIP_GRAD_SPONGE | ACFDTD_HIP_GRAD_AIR)) != 0)
+        return fail(ACFDTD_HIP_EINVAL, "unknown gradient flags");
+    if (g->grads && !g->partials) return fail(ACFDTD_HIP_EINVAL, "gradient partials are NULL");
+    if ((g->grads & ACFDTD_HIP_GRAD_WALLS) && (!d->wall_tape || !d->wall_table))
+        return fail(ACFDTD_HIP_EINVAL, "wall gradients need the boundary tape and the wall table");
+    if ((g->grads & ACFDTD_HIP_GRAD_SPONGE) && d->has_layer)
+        for (int q = 0; q < 3; ++q)
+            if (!g->weight_cell[q] || !g->weight_face[q])
+                return fail(ACFDTD_HIP_EINVAL, "sponge gradients need the grading weights");
+    return ACFDTD_HIP_OK;
+}
+
+template <typename real>
+int face_sweep(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &gd, hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    AdjointArgs<real> g;
+    fill_adjoint(gd, g);
+    const FaceKernel<real> fk =
+        d.has_layer ? adjoint_face_sweep<real, true> : adjoint_face_sweep<real, false>;
+    hipLaunchKernelGGL(fk, dim3(a.tiles.launched), dim3(64, a.tiles.tile_j), 0, stream, a, g);
+    return check_hip(hipGetLastError(), "adjoint face sweep launch");
+}
+
+template <typename real>
+int cell_sweep(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &gd, int64_t step,
+               int stencil, int local, hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    AdjointArgs<real> g;
+    fill_adjoint(gd, g);
+    hipLaunchKernelGGL(pick_cell<real>(d.n_relax, d.has_layer != 0), dim3(a.tiles.launched),
+                       dim3(64, a.tiles.tile_j), 0, stream, a, g, step, stencil, local);
+    return check_hip(hipGetLastError(), "adjoint cell sweep launch");
+}
+
+}  // namespace acfdtd
+
+using namespace acfdtd;
+
+extern "C" size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc) {
+    if (validate(desc) != ACFDTD_HIP_OK) return 0;
+    return (size_t)tiling_for(*desc).tiles * kGradSlots;
+}
+
+extern "C" int acfdtd_hip_adjoint_face_sweep(const acfdtd_hip_desc *desc,
+                                             const acfdtd_hip_adjoint_desc *adjoint,
+                                             void *stream) {
+    const int status = validate_adjoint(desc, adjoint);
+    if (status != ACFDTD_HIP_OK) return status;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return face_sweep<double>(*desc, *adjoint, s);
+    return face_sweep<float>(*desc, *adjoint, s);
+}
+
+extern "C" int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
+                                             const acfdtd_hip_adjoint_desc *adjoint, int64_t step,
+                                             int stencil, int local, void *stream) {
+    const int status = validate_adjoint(desc, adjoint);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (local) {
+        if (step < 0) return fail(ACFDTD_HIP_EINVAL, "step must be >= 0");
+        if (desc->n_receivers > 0 && (step < adjoint->grad_recording_step0 ||
+                                      step - adjoint->grad_recording_step0 >=
+                                          adjoint->grad_recording_stride))
+            return fail(ACFDTD_HIP_EINVAL, "recording gradient does not cover this step");
+        if ((adjoint->grads & ACFDTD_HIP_GRAD_WALLS) &&
+            (step < desc->wall_tape_step0 || step - desc->wall_tape_step0 >= desc->wall_tape_rows))
+            return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8)
+        return cell_sweep<double>(*desc, *adjoint, step, stencil != 0, local != 0, s);
+    return cell_sweep<float>(*desc, *adjoint, step, stencil != 0, local != 0, s);
+}

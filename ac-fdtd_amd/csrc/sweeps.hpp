@@ -11,6 +11,7 @@
 namespace acfdtd {
 
 int fail(int code, const std::string &message);
+int check_hip(hipError_t status, const char *what);
 
 // Workgroup tiling of a room: 64 lanes along z, tile_j rows along y, chunk planes along x.
 // `launched` rounds the tile count up to a multiple of the 8 XCDs so that the remap in
@@ -106,7 +107,53 @@ struct SweepArgs {
     const int32_t *receiver_cell;
     real *recording;
     int64_t recording_stride;
+    real *wall_tape;
+    int64_t wall_tape_stride;
+    int64_t wall_tape_step0;
 };
+
+// Offsets of the six face planes in one row of the boundary tape, and the row length.
+struct TapePlanes {
+    int64_t x_lo, x_hi, y_lo, y_hi, z_lo, z_hi, stride;
+    __host__ __device__ TapePlanes(int nx, int ny, int nz) {
+        const int64_t yz = (int64_t)ny * nz, xz = (int64_t)nx * nz, xy = (int64_t)nx * ny;
+        x_lo = 0;
+        x_hi = yz;
+        y_lo = 2 * yz;
+        y_hi = 2 * yz + xz;
+        z_lo = 2 * (yz + xz);
+        z_hi = z_lo + xy;
+        stride = 2 * (yz + xz + xy);
+    }
+};
+
+// Write v on every face plane cell (i, j, k) lies on (wall code `code`, nonzero).
+template <typename real>
+__device__ inline void tape_store(real *row, const TapePlanes &t, int nx, int ny, int nz, int i,
+                                  int j, int k, int code, real v) {
+    (void)nx;
+    const int cx = code & 3, cy = (code >> 2) & 3, cz = (code >> 4) & 3;
+    const int64_t yz = (int64_t)j * nz + k, xz = (int64_t)i * nz + k, xy = (int64_t)i * ny + j;
+    if (cx & 1) row[t.x_lo + yz] = v;
+    if (cx & 2) row[t.x_hi + yz] = v;
+    if (cy & 1) row[t.y_lo + xz] = v;
+    if (cy & 2) row[t.y_hi + xz] = v;
+    if (cz & 1) row[t.z_lo + xy] = v;
+    if (cz & 2) row[t.z_hi + xy] = v;
+}
+
+// Read the value of cell (i, j, k) from the first face plane it lies on.
+template <typename real>
+__device__ inline real tape_load(const real *row, const TapePlanes &t, int ny, int nz, int i,
+                                 int j, int k, int code) {
+    const int cx = code & 3, cy = (code >> 2) & 3;
+    if (cx & 1) return row[t.x_lo + (int64_t)j * nz + k];
+    if (cx & 2) return row[t.x_hi + (int64_t)j * nz + k];
+    if (cy & 1) return row[t.y_lo + (int64_t)i * nz + k];
+    if (cy & 2) return row[t.y_hi + (int64_t)i * nz + k];
+    if (code & 16) return row[t.z_lo + (int64_t)i * ny + j];
+    return row[t.z_hi + (int64_t)i * ny + j];
+}
 
 template <typename real>
 void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
@@ -143,6 +190,9 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
     a.receiver_cell = d.receiver_cell;
     a.recording = static_cast<real *>(d.recording);
     a.recording_stride = d.recording_stride;
+    a.wall_tape = static_cast<real *>(d.wall_tape);
+    a.wall_tape_stride = d.wall_tape_stride;
+    a.wall_tape_step0 = d.wall_tape_step0;
 }
 
 inline int validate(const acfdtd_hip_desc *d) {
@@ -181,6 +231,9 @@ inline int validate(const acfdtd_hip_desc *d) {
         return fail(ACFDTD_HIP_EINVAL, "source arrays are NULL");
     if (d->n_receivers > 0 && (!d->receiver_cell || !d->recording))
         return fail(ACFDTD_HIP_EINVAL, "receiver arrays are NULL");
+    if (d->wall_tape && (d->wall_tape_stride < TapePlanes(d->nx, d->ny, d->nz).stride ||
+                         d->wall_tape_rows < 0 || d->wall_tape_step0 < 0))
+        return fail(ACFDTD_HIP_EINVAL, "wall tape stride, rows or first step out of range");
     const Tiling t = tiling_for(*d);
     if ((int64_t)t.nblk_k * t.nblk_j * t.nchunk > (int64_t)INT32_MAX - 8)
         return fail(ACFDTD_HIP_EINVAL, "grid too large for one launch");

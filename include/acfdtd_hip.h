@@ -93,6 +93,25 @@ typedef struct acfdtd_hip_desc {
     const int32_t *receiver_cell;
     void *recording;
     int64_t recording_stride;
+
+    /* Optional second set of p / vx / vy / vz buffers (device, same shapes). When all four are
+     * given, acfdtd_hip_run advances steps in pairs with one fused sweep per step, reading one
+     * set and writing the other (A -> B -> A), which moves (8 + 2 n_relax) elements per
+     * cell-update instead of (12 + 2 n_relax); a trailing odd step uses the two in-place sweeps.
+     * Either way the state ends in p / vx / vy / vz and the results are bit-identical. The
+     * swap contents are scratch on entry and exit. */
+    void *p_swap, *vx_swap, *vy_swap, *vz_swap;
+
+    /* Optional boundary tape for wall-admittance gradients (device, working precision). For
+     * absolute step n, row n - wall_tape_step0 (of wall_tape_rows rows, wall_tape_stride
+     * elements each) receives p_tmp + p_old of every boundary cell, where p_tmp is the
+     * pressure before the wall closed form, once per face plane the cell lies on. Planes in
+     * row order: x-low, x-high (ny*nz, index j*nz+k), y-low, y-high (nx*nz, i*nz+k), z-low,
+     * z-high (nx*ny, i*ny+j); acfdtd_hip_wall_tape_stride() gives the row length. */
+    void *wall_tape;
+    int64_t wall_tape_stride;
+    int64_t wall_tape_step0;
+    int64_t wall_tape_rows;
 } acfdtd_hip_desc;
 
 /* sizeof(acfdtd_hip_desc), for the caller's mirror check. Replaces acfdtd_struct_size
@@ -120,6 +139,16 @@ int acfdtd_hip_velocity_sweep(const acfdtd_hip_desc *desc, void *stream);
 int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t step, int64_t recording_column,
                               void *stream);
 
+/* One fused step (velocity and pressure in one sweep) for absolute step `step`, reading
+ * p / vx / vy / vz and writing p_swap / vx_swap / vy_swap / vz_swap (psi in place); with
+ * `reverse` nonzero the two sets swap roles. For per-kernel timing and tuning; no reference
+ * counterpart. */
+int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step, int64_t recording_column,
+                          int reverse, void *stream);
+
+/* Elements per step row of the boundary tape: 2 (ny nz + nx nz + nx ny). */
+int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
+
 /* Energy terms of the stored state, in fp64 on the device: out[0] = sum p^2,
  * out[1] = sum over all faces of v^2, out[2] = sum over interior faces of v_a (p_hi - p_lo).
  * `out` is a device pointer to 3 doubles; `workspace` a device buffer of at least
@@ -127,6 +156,52 @@ int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t step, int64_t
  * scheme.py:365-389 / c_backend.py:361-382. */
 size_t acfdtd_hip_energy_workspace_bytes(const acfdtd_hip_desc *desc);
 int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out, void *stream);
+
+/* ------------------------------------------------------------------------------ adjoint --
+ * The transposed, time-reversed step and the parameter gradients it yields (no reference
+ * counterpart: the reference is not differentiable). An adjoint step n is a face sweep
+ * followed by a cell sweep that applies the stencil part of step n and the cell-local part of
+ * step n - 1; the local part of the last step runs alone first. Gradients accumulate in fp64
+ * into `partials` (acfdtd_hip_adjoint_partials_count doubles, zeroed by the caller), 12 slots
+ * per workgroup: [0..5] dL/da summed over the cells of each face plane (FACES order),
+ * [6] dL/dsigma_max, [7] dL/drelax_scale, [8..11] dL/dgain_r. */
+#define ACFDTD_HIP_GRAD_WALLS 1u  /* needs the forward boundary tape (desc->wall_tape) */
+#define ACFDTD_HIP_GRAD_SPONGE 2u /* needs fwd_* and the grading weights */
+#define ACFDTD_HIP_GRAD_AIR 4u    /* needs fwd_vx / fwd_vy / fwd_vz */
+
+typedef struct acfdtd_hip_adjoint_desc {
+    uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
+    uint32_t struct_size; /* sizeof(acfdtd_hip_adjoint_desc) */
+    uint32_t grads;       /* ACFDTD_HIP_GRAD_* */
+    int32_t reserved;
+    /* Adjoint state, same shapes as p, vx, vy, vz, psi; ldiv is one cell field of scratch. */
+    void *lp, *lvx, *lvy, *lvz, *lpsi, *ldiv;
+    /* dL/d(recording): column (step - grad_recording_step0) of rows of
+     * grad_recording_stride samples, one row per receiver. */
+    const void *grad_recording;
+    int64_t grad_recording_stride;
+    int64_t grad_recording_step0;
+    /* dL/d(source signal), rows of desc->source_stride samples (written, not accumulated);
+     * NULL to skip. */
+    void *grad_signal;
+    /* Forward state after the step whose local part the call transposes (cell sweep) or
+     * after the step it transposes (face sweep): p (before nothing; sources are removed in the
+     * kernel), vx, vy, vz. Only read for sponge and air gradients. */
+    const void *fwd_p, *fwd_vx, *fwd_vy, *fwd_vz;
+    /* Sponge grading weights inside^order at cell centres and faces (the exponents of the
+     * profiles in desc->cell / desc->face divided by -sigma_max dt). */
+    const void *weight_cell[3];
+    const void *weight_face[3];
+    double dt;
+    double *partials;
+} acfdtd_hip_adjoint_desc;
+
+size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc);
+int acfdtd_hip_adjoint_face_sweep(const acfdtd_hip_desc *desc,
+                                  const acfdtd_hip_adjoint_desc *adjoint, void *stream);
+int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
+                                  const acfdtd_hip_adjoint_desc *adjoint, int64_t step,
+                                  int stencil, int local, void *stream);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);

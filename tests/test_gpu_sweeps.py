@@ -50,10 +50,11 @@ def assert_same_state(candidate, reference):
         np.testing.assert_array_equal(to_numpy(candidate.relaxation_states[n]), state)
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
 @pytest.mark.parametrize("name", list(PARITY))
-def test_fp64_is_bitwise_the_oracle(name):
+def test_fp64_is_bitwise_the_oracle(name, fused):
     reference = prepare(OracleFDTD(PARITY_GRID, **PARITY[name]))
-    candidate = prepare(hip(**PARITY[name]))
+    candidate = prepare(hip(fused=fused, **PARITY[name]))
     reference.run(PARITY_STEPS)
     candidate.run(PARITY_STEPS)
     assert_same_state(candidate, reference)
@@ -125,11 +126,12 @@ def test_running_in_pieces_is_bitwise_running_at_once():
     np.testing.assert_array_equal(pieces.recorded_pressure, whole.recorded_pressure)
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
 @pytest.mark.parametrize("tile_j,chunk_x", [(1, 1), (2, 3), (8, 7), (4, 64)])
-def test_tiling_does_not_change_the_answer(tile_j, chunk_x):
+def test_tiling_does_not_change_the_answer(tile_j, chunk_x, fused):
     """The reference's thread-count invariance (test_c_backend.py:68-82), for tile shapes."""
     base = prepare(hip(**PARITY["all"]))
-    tiled = prepare(hip(tile_j=tile_j, chunk_x=chunk_x, **PARITY["all"]))
+    tiled = prepare(hip(tile_j=tile_j, chunk_x=chunk_x, fused=fused, **PARITY["all"]))
     base.run(60)
     tiled.run(60)
     np.testing.assert_array_equal(to_numpy(tiled.p), to_numpy(base.p))
@@ -137,10 +139,11 @@ def test_tiling_does_not_change_the_answer(tile_j, chunk_x):
         np.testing.assert_array_equal(to_numpy(tiled.velocity[axis]), to_numpy(base.velocity[axis]))
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
 @pytest.mark.parametrize(
     "shape", [(1, 1, 1), (7, 1, 1), (1, 9, 1), (3, 1, 70), (5, 3, 130), (2, 66, 65), (33, 5, 64)]
 )
-def test_odd_shapes_match_the_oracle(shape):
+def test_odd_shapes_match_the_oracle(shape, fused):
     """Ducts, single cells, one-cell axes (both walls on one cell), z rows that are not a
     multiple of the 64-lane wavefront, with walls, a layer and air all on."""
     grid = Grid(dx=0.02, nx=shape[0], ny=shape[1], nz=shape[2])
@@ -150,11 +153,11 @@ def test_odd_shapes_match_the_oracle(shape):
         "air_absorption": AirAbsorption(),
     }
     reference = OracleFDTD(grid, **config)
-    candidate = hip(grid, **config)
+    candidate = hip(grid, fused=fused, **config)
     for solver in (reference, candidate):
         prepare(solver, grid, seed=7)
-    reference.run(40)
-    candidate.run(40)
+    reference.run(41)
+    candidate.run(41)
     assert_same_state(candidate, reference)
     np.testing.assert_array_equal(candidate.recorded_pressure, reference.recorded_pressure)
 
@@ -192,6 +195,27 @@ def test_a_receiver_added_part_way_records_from_then_on():
         np.testing.assert_array_equal(candidate.recorded_channel(channel), samples)
     with pytest.raises(ValueError):
         candidate.recorded_pressure  # noqa: B018  (ragged, as in the reference)
+
+
+def test_user_edits_between_runs_are_seen_by_both_kernel_paths():
+    """State written by the caller between runs (as the reference tests do through
+    ``solver.p[:] = ...``), including non-zero wall faces, is what the next run starts from."""
+    import torch
+
+    config = PARITY["all"]
+    reference = prepare(OracleFDTD(PARITY_GRID, **config))
+    candidate = prepare(hip(**config))
+    rng = np.random.default_rng(5)
+    for n_steps in (4, 3):
+        reference.run(n_steps)
+        candidate.run(n_steps)
+        for axis in range(3):
+            noise = rng.standard_normal(reference.velocity[axis].shape)
+            reference.velocity[axis][:] = noise
+            candidate.velocity[axis].copy_(torch.as_tensor(noise))
+    reference.run(6)
+    candidate.run(6)
+    assert_same_state(candidate, reference)
 
 
 def test_receivers_record_the_same_samples_as_probing_by_hand():
