@@ -20,6 +20,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PACKAGE_DIR = Path(__file__).resolve().parent
@@ -28,7 +30,7 @@ INCLUDE_DIR = PACKAGE_DIR.parent.parent / "include"
 #: The library the loader opens; ACFDTD_HIP_LIBRARY points it at another build of the same
 #: sources (tile experiments), the default is the in-tree build.
 LIBRARY = Path(os.environ.get("ACFDTD_HIP_LIBRARY", PACKAGE_DIR / "lib" / "libacfdtd_hip.so"))
-SOURCES = ("sweeps.hip", "adjoint.hip")
+SOURCES = ("sweeps.hip", "fused.hip", "batch.hip", "adjoint.hip")
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 
 FLAGS = [
@@ -36,7 +38,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang",
@@ -53,30 +54,41 @@ def hipcc() -> str:
     return found
 
 
-def command(output: Path) -> list[str]:
-    return [
-        hipcc(),
-        *FLAGS,
-        f"-I{INCLUDE_DIR}",
-        f"-I{SOURCE_DIR}",
-        *(str(SOURCE_DIR / name) for name in SOURCES),
-        "-o",
-        str(output),
-    ]
+def compile_command(source: Path, output: Path) -> list[str]:
+    return [hipcc(), *FLAGS, "-c", f"-I{INCLUDE_DIR}", f"-I{SOURCE_DIR}", str(source), "-o", str(output)]
 
 
-def build_library(verbose: bool = False) -> Path:
-    """Compile every source into :data:`LIBRARY`, replacing it atomically."""
-    LIBRARY.parent.mkdir(parents=True, exist_ok=True)
-    staging = LIBRARY.with_name(LIBRARY.name + f".tmp{os.getpid()}")
-    cmd = command(staging)
+def link_command(objects: list[Path], output: Path) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objects), "-o", str(output)]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     result = subprocess.run(cmd, capture_output=True, text=True)
     if result.returncode != 0:
-        staging.unlink(missing_ok=True)
         raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
-    os.replace(staging, LIBRARY)
+
+
+def build_library(verbose: bool = False) -> Path:
+    """Compile every source (in parallel) and link :data:`LIBRARY`, replacing it atomically."""
+    LIBRARY.parent.mkdir(parents=True, exist_ok=True)
+    with tempfile.TemporaryDirectory(prefix="acfdtd_build_") as scratch:
+        objects = [Path(scratch) / (Path(name).stem + ".o") for name in SOURCES]
+        jobs = min(len(SOURCES), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            for future in [
+                pool.submit(_run, compile_command(SOURCE_DIR / name, obj), verbose)
+                for name, obj in zip(SOURCES, objects)
+            ]:
+                future.result()
+        staging = LIBRARY.with_name(LIBRARY.name + f".tmp{os.getpid()}")
+        try:
+            _run(link_command(objects, staging), verbose)
+        except RuntimeError:
+            staging.unlink(missing_ok=True)
+            raise
+        os.replace(staging, LIBRARY)
     return LIBRARY
 
 

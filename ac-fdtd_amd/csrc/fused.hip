@@ -1,0 +1,156 @@
+// The fused one-sweep step (scalar and V-wide lanes): kernels, dispatch, launch.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "acfdtd_hip.h"
+#include "marches.hpp"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__global__ __launch_bounds__(512) void fused_step(const SweepArgs<real> a, const FusedIO<real> io,
+                                                  const int64_t step, const int64_t column) {
+    int kb, jb, xc;
+    if (!decode_tile(a.tiles, kb, jb, xc)) return;
+    const int k = kb * 64 + (int)threadIdx.x;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    if (k >= a.nz || j >= a.ny) return;
+    const int i0 = xc * a.tiles.chunk;
+    const int i1 = min(i0 + a.tiles.chunk, a.nx);
+    fused_march<real, NR, WALLS, LAYER, COMPENSATED>(
+        a, i0, i1, j, k, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
+        io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
+        a.face[1], a.cell[2], a.face[2], a.recording);
+}
+
+
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__global__ __launch_bounds__(512) void fused_step_vec(const SweepArgs<real> a,
+                                                      const FusedIO<real> io, const int64_t step,
+                                                      const int64_t column) {
+    int kb, jb, xc;
+    if (!decode_tile(a.tiles, kb, jb, xc)) return;
+    const int k0 = (kb * 64 + (int)threadIdx.x) * V;
+    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
+    if (k0 >= a.nz || j >= a.ny) return;
+    const int i0 = xc * a.tiles.chunk;
+    const int i1 = min(i0 + a.tiles.chunk, a.nx);
+    fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(
+        a, i0, i1, j, k0, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
+        io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
+        a.face[1], a.cell[2], a.face[2], a.recording);
+}
+
+
+template <typename real>
+using FusedKernel = void (*)(const SweepArgs<real>, const FusedIO<real>, const int64_t,
+                             const int64_t);
+
+template <typename real, int V, int NR, bool W, bool L, bool C>
+FusedKernel<real> fused_kernel() {
+    if constexpr (V == 1) return fused_step<real, NR, W, L, C>;
+    else return fused_step_vec<real, V, NR, W, L, C>;
+}
+
+// Compensated kernels exist for fp32 only, and 4-wide lanes only for fp32 (16 bytes).
+template <typename real, int V, int NR>
+FusedKernel<real> pick_fused_nr(bool walls, bool layer, bool comp) {
+    if constexpr (sizeof(real) == 4) {
+        if (comp) {
+            if (walls) return layer ? fused_kernel<real, V, NR, true, true, true>()
+                                    : fused_kernel<real, V, NR, true, false, true>();
+            return layer ? fused_kernel<real, V, NR, false, true, true>()
+                         : fused_kernel<real, V, NR, false, false, true>();
+        }
+    }
+    (void)comp;
+    if (walls) return layer ? fused_kernel<real, V, NR, true, true, false>()
+                            : fused_kernel<real, V, NR, true, false, false>();
+    return layer ? fused_kernel<real, V, NR, false, true, false>()
+                 : fused_kernel<real, V, NR, false, false, false>();
+}
+
+template <typename real, int V>
+FusedKernel<real> pick_fused_v(int nr, bool walls, bool layer, bool comp) {
+    switch (nr) {
+        case 0: return pick_fused_nr<real, V, 0>(walls, layer, comp);
+        case 1: return pick_fused_nr<real, V, 1>(walls, layer, comp);
+        case 2: return pick_fused_nr<real, V, 2>(walls, layer, comp);
+        case 3: return pick_fused_nr<real, V, 3>(walls, layer, comp);
+        default: return pick_fused_nr<real, V, 4>(walls, layer, comp);
+    }
+}
+
+template <typename real>
+FusedKernel<real> pick_fused(int vec, int nr, bool walls, bool layer, bool comp) {
+    if constexpr (sizeof(real) == 4) {
+        if (vec == 4) return pick_fused_v<real, 4>(nr, walls, layer, comp);
+    }
+    if (vec == 2) return pick_fused_v<real, 2>(nr, walls, layer, comp);
+    return pick_fused_v<real, 1>(nr, walls, layer, comp);
+}
+
+template <typename real>
+int64_t launch_fused_pairs(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
+                           int64_t column, hipStream_t stream) {
+    SweepArgs<real> f;
+    fill_args(d, f);
+    f.tiles = fused_tiling(d);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const FusedKernel<real> fk =
+        pick_fused<real>(f.tiles.vec, d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    const FusedIO<real> forward = fused_io<real>(d, false), backward = fused_io<real>(d, true);
+    const dim3 grid(f.tiles.launched), block(64, f.tiles.tile_j);
+    int64_t n = 0;
+    for (; n + 1 < n_steps; n += 2) {
+        hipLaunchKernelGGL(fk, grid, block, 0, stream, f, forward, step_offset + n, column + n);
+        hipLaunchKernelGGL(fk, grid, block, 0, stream, f, backward, step_offset + n + 1,
+                           column + n + 1);
+    }
+    return n;
+}
+
+template int64_t launch_fused_pairs<float>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
+                                           hipStream_t);
+template int64_t launch_fused_pairs<double>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
+                                            hipStream_t);
+
+template <typename real>
+int fused_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, bool reverse,
+               hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    a.tiles = fused_tiling(d);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const FusedKernel<real> fk = pick_fused<real>(a.tiles.vec, d.n_relax, d.wall_mask != 0,
+                                                  d.has_layer != 0, comp);
+    hipLaunchKernelGGL(fk, dim3(a.tiles.launched), dim3(64, a.tiles.tile_j), 0, stream, a,
+                       fused_io<real>(d, reverse), step, column);
+    return check_hip(hipGetLastError(), "fused launch");
+}
+
+}  // namespace acfdtd
+
+using namespace acfdtd;
+
+extern "C" int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step,
+                                     int64_t recording_column, int reverse, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (!has_swap(*desc)) return fail(ACFDTD_HIP_EINVAL, "fused step needs the four swap buffers");
+    if (step < 0 || recording_column < 0)
+        return fail(ACFDTD_HIP_EINVAL, "step and recording_column must be >= 0");
+    if (desc->n_receivers > 0 && recording_column >= desc->recording_stride)
+        return fail(ACFDTD_HIP_EINVAL, "recording column out of range");
+    if (desc->wall_tape &&
+        (step < desc->wall_tape_step0 || step - desc->wall_tape_step0 >= desc->wall_tape_rows))
+        return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return fused_once<double>(*desc, step, recording_column, reverse != 0, s);
+    return fused_once<float>(*desc, step, recording_column, reverse != 0, s);
+}

@@ -1,0 +1,608 @@
+// Device bodies of the time step, shared by the single-room, fused and batched kernels.
+//
+// Each body marches one lane's column of cells along x over [i0, i1): velocity_march and
+// pressure_march are the two in-place sweeps, fused_march the one-sweep step from an input set
+// of fields to an output set, fused_march_vec the same with V z-cells per lane. Pointers are
+// passed as restrict-qualified arguments of inlined functions, so the compiler may issue the
+// loads of the next planes before the stores of the current one retire.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "acfdtd_hip.h"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+// Planes marched per loop trip; with the restrict-qualified bodies below, the loads of the
+// next planes can be issued before the stores of the current one retire.
+#ifndef ACFDTD_X_UNROLL
+#define ACFDTD_X_UNROLL 2
+#endif
+
+template <typename real, bool LAYER, bool FUSED_FMA>
+__device__ __forceinline__ void velocity_march(
+    const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k,
+    const real *__restrict__ p, real *__restrict__ vx, real *__restrict__ vy,
+    real *__restrict__ vz, const real *__restrict__ cell_x, const real *__restrict__ face_x,
+    const real *__restrict__ cell_y, const real *__restrict__ face_y,
+    const real *__restrict__ cell_z, const real *__restrict__ face_z) {
+    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const real vel = a.vel;
+    real fyc = 1, fzc = 1, fyf = 1, fzf = 1;
+    if (LAYER) {
+        fyc = cell_y[j];
+        fzc = cell_z[k];
+        fyf = face_y[j];
+        fzf = face_z[k];
+    }
+    real p_prev = i0 > 0 ? p[L.cell(i0 - 1)] : real(0);
+#pragma unroll ACFDTD_X_UNROLL
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real pc = p[c];
+        real fxc = 1, fxf = 1;
+        if (LAYER) {
+            fxc = cell_x[i];
+            fxf = face_x[i];
+        }
+        // vx on face i (lower face of cell i); face 0 is a wall and is only ever damped.
+        if (LAYER || i > 0) {
+            real v = vx[c];
+            if (i > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p_prev);
+            if (LAYER) v = ((v * fxf) * fyc) * fzc;
+            vx[c] = v;
+        }
+        const int64_t cy = L.face_y(i);
+        if (LAYER || j > 0) {
+            real v = vy[cy];
+            if (j > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p[c - a.nz]);
+            if (LAYER) v = ((v * fxc) * fyf) * fzc;
+            vy[cy] = v;
+        }
+        const int64_t cz = L.face_z(i);
+        if (LAYER || k > 0) {
+            real v = vz[cz];
+            if (k > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p[c - 1]);
+            if (LAYER) v = ((v * fxc) * fyc) * fzf;
+            vz[cz] = v;
+        }
+        if (LAYER) {
+            // The far wall faces belong to the last cell on each axis; like face 0 they are
+            // never updated, only damped (the reference damps whole slabs, walls included).
+            if (i == a.nx - 1) {
+                const int64_t t = c + L.plane;
+                vx[t] = ((vx[t] * face_x[a.nx]) * fyc) * fzc;
+            }
+            if (j == a.ny - 1) {
+                const int64_t t = cy + a.nz;
+                vy[t] = ((vy[t] * fxc) * face_y[a.ny]) * fzc;
+            }
+            if (k == a.nz - 1) {
+                const int64_t t = cz + 1;
+                vz[t] = ((vz[t] * fxc) * fyc) * face_z[a.nz];
+            }
+        }
+        p_prev = pc;
+    }
+}
+
+
+template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__device__ __forceinline__ void pressure_march(
+    const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k,
+    const int64_t step, const int64_t column, real *__restrict__ p, real *__restrict__ psi,
+    const real *__restrict__ vx, const real *__restrict__ vy, const real *__restrict__ vz,
+    const real *__restrict__ wall_table, const real *__restrict__ cell_x,
+    const real *__restrict__ cell_y, const real *__restrict__ cell_z,
+    real *__restrict__ recording) {
+    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const int64_t n_cells = L.plane * a.nx;
+    const int jk = j * a.nz + k;
+    const int code_yz = WALLS ? (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4) : 0;
+    real fyc = 1, fzc = 1;
+    if (LAYER) {
+        fyc = cell_y[j];
+        fzc = cell_z[k];
+    }
+
+    real vx_lo_next = vx[L.cell(i0)];
+#pragma unroll ACFDTD_X_UNROLL
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real vx_lo = vx_lo_next;
+        const real vx_hi = vx[c + L.plane];
+        vx_lo_next = vx_hi;
+        const int64_t cy = L.face_y(i);
+        const int64_t cz = L.face_z(i);
+        const real div = ((vx_hi - vx_lo) + (vy[cy + a.nz] - vy[cy])) + (vz[cz + 1] - vz[cz]);
+
+        const real p_old = p[c];
+        real pn;
+        if (COMPENSATED) {
+            // p_old + sum_r off_r mean_r + div relax_scale - div pres, summed with TwoSum so the
+            // increment costs one rounding instead of NR + 2, and the dominant product exact.
+            real s = p_old, e = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                two_sum_acc(s, offset * a.mean[r], e);
+            }
+            if (NR > 0) two_sum_acc(s, div * a.relax_scale, e);
+            const real prod = div * a.pres;
+            const real prod_err = fma_real(div, a.pres, -prod);
+            two_sum_acc(s, -prod, e);
+            pn = s + (e - prod_err);
+        } else {
+            pn = p_old;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                pn = pn + offset * a.mean[r];
+            }
+            if (NR > 0) pn = pn + div * a.relax_scale;
+            pn = pn - div * a.pres;
+        }
+
+        if (a.wall_tape) {
+            const int code = (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4) |
+                             wall_class(i, a.nx);
+            if (code)
+                tape_store(a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride,
+                           TapePlanes(a.nx, a.ny, a.nz), a.nx, a.ny, a.nz, i, j, k, code,
+                           pn + p_old);
+        }
+        if (WALLS) {
+            const int code = code_yz | wall_class(i, a.nx);
+            if ((a.wall_mask >> code) & 1ull)
+                pn = pn * wall_table[code] - p_old * wall_table[ACFDTD_HIP_WALL_CODES + code];
+        }
+        if (LAYER) pn = ((pn * cell_x[i]) * fyc) * fzc;
+
+        // Sources, in insertion order; the plane test is wave-uniform (scalar).
+        for (int s = 0; s < a.n_sources; ++s) {
+            if (a.source_cell[2 * s] == i && a.source_cell[2 * s + 1] == jk &&
+                step < a.source_length[s])
+                pn = pn + a.src * a.source_signal[s * a.source_stride + step];
+        }
+        p[c] = pn;
+        for (int r = 0; r < a.n_receivers; ++r) {
+            if (a.receiver_cell[2 * r] == i && a.receiver_cell[2 * r + 1] == jk)
+                recording[r * a.recording_stride + column] = pn;
+        }
+    }
+}
+
+
+// ------------------------------------------------------------------ fused single sweep --
+//
+// One step in one pass over memory: read p^n and v^{n-1/2} from the `in` set, write
+// v^{n+1/2} and p^{n+1} to the `out` set, advance psi in place. Each cell computes the new
+// velocity on all six of its faces; a face shared with a neighbouring cell is computed by both
+// from the same unmodified inputs with the same operations, so both get the same bits, and
+// only the face's owner (the cell above it; the last cell for the far wall) writes it. The
+// x-neighbour face comes from a register carried along the march. Wall faces are copied
+// (damped when there is a sponge), as the two-sweep path leaves them in place.
+
+template <typename real>
+struct FusedIO {
+    const real *p_in, *vx_in, *vy_in, *vz_in;
+    real *p_out, *vx_out, *vy_out, *vz_out;
+};
+
+template <typename real, bool LAYER>
+__device__ __forceinline__ real damp3(real v, real f1, real f2, real f3) {
+    if (LAYER) return ((v * f1) * f2) * f3;
+    return v;
+}
+
+template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__device__ __forceinline__ void fused_march(
+    const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k,
+    const int64_t step, const int64_t column, const real *__restrict__ p_in,
+    const real *__restrict__ vx_in, const real *__restrict__ vy_in,
+    const real *__restrict__ vz_in, real *__restrict__ p_out, real *__restrict__ vx_out,
+    real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi,
+    const real *__restrict__ wall_table, const real *__restrict__ cell_x,
+    const real *__restrict__ face_x, const real *__restrict__ cell_y,
+    const real *__restrict__ face_y, const real *__restrict__ cell_z,
+    const real *__restrict__ face_z, real *__restrict__ recording) {
+    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const int64_t n_cells = L.plane * a.nx;
+    const int jk = j * a.nz + k;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const real vel = a.vel;
+    const int code_yz = WALLS ? (wall_class(j, ny) << 2) | (wall_class(k, nz) << 4) : 0;
+    real fyc = 1, fzc = 1, fyf_lo = 1, fyf_hi = 1, fzf_lo = 1, fzf_hi = 1;
+    if (LAYER) {
+        fyc = cell_y[j];
+        fzc = cell_z[k];
+        fyf_lo = face_y[j];
+        fyf_hi = face_y[j + 1];
+        fzf_lo = face_z[k];
+        fzf_hi = face_z[k + 1];
+    }
+
+    // New vx on face i0, the lower face of the first cell of the march.
+    real p_c = p_in[L.cell(i0)];
+    real vx_lo;
+    {
+        const real fxf = LAYER ? face_x[i0] : real(1);
+        real v = vx_in[L.cell(i0)];
+        if (i0 > 0) v = v - vel * (p_c - p_in[L.cell(i0 - 1)]);
+        vx_lo = damp3<real, LAYER>(v, fxf, fyc, fzc);
+        vx_out[L.cell(i0)] = vx_lo;
+    }
+
+#pragma unroll ACFDTD_X_UNROLL
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real fxc = LAYER ? cell_x[i] : real(1);
+
+        // x: upper face i + 1 (interior, or the far wall when i is the last cell).
+        real vx_hi;
+        real p_up = 0;
+        {
+            const real fxf = LAYER ? face_x[i + 1] : real(1);
+            real v = vx_in[c + L.plane];
+            if (i + 1 < nx) {
+                p_up = p_in[c + L.plane];
+                v = v - vel * (p_up - p_c);
+            }
+            vx_hi = damp3<real, LAYER>(v, fxf, fyc, fzc);
+            if (i + 1 < i1 || i + 1 == nx) vx_out[c + L.plane] = vx_hi;
+        }
+        // y: lower face j (owned), upper face j + 1 (written only for the far wall).
+        const int64_t cy = L.face_y(i);
+        real vy_lo, vy_hi;
+        {
+            real v = vy_in[cy];
+            if (j > 0) v = v - vel * (p_c - p_in[c - nz]);
+            vy_lo = damp3<real, LAYER>(v, fxc, fyf_lo, fzc);
+            vy_out[cy] = vy_lo;
+            real w = vy_in[cy + nz];
+            if (j + 1 < ny) w = w - vel * (p_in[c + nz] - p_c);
+            vy_hi = damp3<real, LAYER>(w, fxc, fyf_hi, fzc);
+            if (j + 1 == ny) vy_out[cy + nz] = vy_hi;
+        }
+        // z: lower face k (owned), upper face k + 1 (written only for the far wall).
+        const int64_t cz = L.face_z(i);
+        real vz_lo, vz_hi;
+        {
+            real v = vz_in[cz];
+            if (k > 0) v = v - vel * (p_c - p_in[c - 1]);
+            vz_lo = damp3<real, LAYER>(v, fxc, fyc, fzf_lo);
+            vz_out[cz] = vz_lo;
+            real w = vz_in[cz + 1];
+            if (k + 1 < nz) w = w - vel * (p_in[c + 1] - p_c);
+            vz_hi = damp3<real, LAYER>(w, fxc, fyc, fzf_hi);
+            if (k + 1 == nz) vz_out[cz + 1] = vz_hi;
+        }
+
+        const real div = ((vx_hi - vx_lo) + (vy_hi - vy_lo)) + (vz_hi - vz_lo);
+        const real p_old = p_c;
+        real pn;
+        if (COMPENSATED) {
+            real s = p_old, e = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                two_sum_acc(s, offset * a.mean[r], e);
+            }
+            if (NR > 0) two_sum_acc(s, div * a.relax_scale, e);
+            const real prod = div * a.pres;
+            const real prod_err = fma_real(div, a.pres, -prod);
+            two_sum_acc(s, -prod, e);
+            pn = s + (e - prod_err);
+        } else {
+            pn = p_old;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                real *state = psi + (int64_t)r * n_cells + c;
+                const real target = div * a.gain[r];
+                const real offset = *state - target;
+                *state = offset * a.decay[r] + target;
+                pn = pn + offset * a.mean[r];
+            }
+            if (NR > 0) pn = pn + div * a.relax_scale;
+            pn = pn - div * a.pres;
+        }
+        if (a.wall_tape) {
+            const int code = (wall_class(j, ny) << 2) | (wall_class(k, nz) << 4) |
+                             wall_class(i, nx);
+            if (code)
+                tape_store(a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride,
+                           TapePlanes(nx, ny, nz), nx, ny, nz, i, j, k, code, pn + p_old);
+        }
+        if (WALLS) {
+            const int code = code_yz | wall_class(i, nx);
+            if ((a.wall_mask >> code) & 1ull)
+                pn = pn * wall_table[code] - p_old * wall_table[ACFDTD_HIP_WALL_CODES + code];
+        }
+        if (LAYER) pn = ((pn * fxc) * fyc) * fzc;
+        for (int s = 0; s < a.n_sources; ++s) {
+            if (a.source_cell[2 * s] == i && a.source_cell[2 * s + 1] == jk &&
+                step < a.source_length[s])
+                pn = pn + a.src * a.source_signal[s * a.source_stride + step];
+        }
+        p_out[c] = pn;
+        for (int r = 0; r < a.n_receivers; ++r) {
+            if (a.receiver_cell[2 * r] == i && a.receiver_cell[2 * r + 1] == jk)
+                recording[r * a.recording_stride + column] = pn;
+        }
+        vx_lo = vx_hi;
+        p_c = p_up;
+    }
+}
+
+// ----------------------------------------------------------- fused sweep, V cells per lane --
+//
+// The same step as fused_march with each lane owning V consecutive z cells, so every field
+// access is a V-wide vector access (dwordx2 / dwordx4): fp32 moves too few bytes per memory
+// instruction with one cell per lane to keep HBM busy. The arithmetic per cell is exactly the
+// scalar kernel's, so the bits are the same. Used when nz is a multiple of V (then every row
+// start, and every vector, is V-element aligned in all fields except vz, whose rows are nz + 1
+// long; its accesses only need the 4-byte alignment gfx950 requires).
+
+template <typename real, int V>
+struct VecOf {
+    typedef real type __attribute__((ext_vector_type(V), aligned(sizeof(real))));
+};
+
+template <typename real, int V>
+__device__ __forceinline__ void vload(const real *__restrict__ p, real (&out)[V]) {
+    const typename VecOf<real, V>::type v = *reinterpret_cast<const typename VecOf<real, V>::type *>(p);
+#pragma unroll
+    for (int e = 0; e < V; ++e) out[e] = v[e];
+}
+
+template <typename real, int V>
+__device__ __forceinline__ void vstore(real *__restrict__ p, const real (&in)[V]) {
+    typename VecOf<real, V>::type v;
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = in[e];
+    *reinterpret_cast<typename VecOf<real, V>::type *>(p) = v;
+}
+
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__device__ __forceinline__ void fused_march_vec(
+    const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k0,
+    const int64_t step, const int64_t column, const real *__restrict__ p_in,
+    const real *__restrict__ vx_in, const real *__restrict__ vy_in,
+    const real *__restrict__ vz_in, real *__restrict__ p_out, real *__restrict__ vx_out,
+    real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi,
+    const real *__restrict__ wall_table, const real *__restrict__ cell_x,
+    const real *__restrict__ face_x, const real *__restrict__ cell_y,
+    const real *__restrict__ face_y, const real *__restrict__ cell_z,
+    const real *__restrict__ face_z, real *__restrict__ recording) {
+    const Layout L(a.nx, a.ny, a.nz, j, k0);
+    const int64_t n_cells = L.plane * a.nx;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int jk0 = j * nz + k0;
+    const real vel = a.vel;
+    const bool z_lo_wall = k0 == 0, z_hi_wall = k0 + V == nz;
+    real fyc = 1, fyf_lo = 1, fyf_hi = 1, fzc[V], fzf[V + 1];
+#pragma unroll
+    for (int e = 0; e < V; ++e) fzc[e] = 1;
+#pragma unroll
+    for (int e = 0; e <= V; ++e) fzf[e] = 1;
+    if (LAYER) {
+        fyc = cell_y[j];
+        fyf_lo = face_y[j];
+        fyf_hi = face_y[j + 1];
+#pragma unroll
+        for (int e = 0; e < V; ++e) fzc[e] = cell_z[k0 + e];
+#pragma unroll
+        for (int e = 0; e <= V; ++e) fzf[e] = face_z[k0 + e];
+    }
+
+    real p_c[V], vx_lo[V];
+    vload<real, V>(p_in + L.cell(i0), p_c);
+    {
+        real v[V], prev[V];
+        vload<real, V>(vx_in + L.cell(i0), v);
+        if (i0 > 0) vload<real, V>(p_in + L.cell(i0 - 1), prev);
+        const real fxf = LAYER ? face_x[i0] : real(1);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            real w = v[e];
+            if (i0 > 0) w = w - vel * (p_c[e] - prev[e]);
+            vx_lo[e] = damp3<real, LAYER>(w, fxf, fyc, fzc[e]);
+        }
+        vstore<real, V>(vx_out + L.cell(i0), vx_lo);
+    }
+
+#pragma unroll ACFDTD_X_UNROLL
+    for (int i = i0; i < i1; ++i) {
+        const int64_t c = L.cell(i);
+        const real fxc = LAYER ? cell_x[i] : real(1);
+
+        real vx_hi[V], p_up[V];
+        {
+            real v[V];
+            vload<real, V>(vx_in + c + L.plane, v);
+            const bool interior = i + 1 < nx;
+            if (interior) vload<real, V>(p_in + c + L.plane, p_up);
+            const real fxf = LAYER ? face_x[i + 1] : real(1);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                real w = v[e];
+                if (interior) w = w - vel * (p_up[e] - p_c[e]);
+                vx_hi[e] = damp3<real, LAYER>(w, fxf, fyc, fzc[e]);
+            }
+            if (!interior)
+#pragma unroll
+                for (int e = 0; e < V; ++e) p_up[e] = 0;
+            if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + c + L.plane, vx_hi);
+        }
+        const int64_t cy = L.face_y(i);
+        real vy_lo[V], vy_hi[V];
+        {
+            real v[V], w[V], q[V];
+            vload<real, V>(vy_in + cy, v);
+            if (j > 0) vload<real, V>(p_in + c - nz, q);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                real t = v[e];
+                if (j > 0) t = t - vel * (p_c[e] - q[e]);
+                vy_lo[e] = damp3<real, LAYER>(t, fxc, fyf_lo, fzc[e]);
+            }
+            vstore<real, V>(vy_out + cy, vy_lo);
+            vload<real, V>(vy_in + cy + nz, w);
+            if (j + 1 < ny) vload<real, V>(p_in + c + nz, q);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                real t = w[e];
+                if (j + 1 < ny) t = t - vel * (q[e] - p_c[e]);
+                vy_hi[e] = damp3<real, LAYER>(t, fxc, fyf_hi, fzc[e]);
+            }
+            if (j + 1 == ny) vstore<real, V>(vy_out + cy + nz, vy_hi);
+        }
+        const int64_t cz = L.face_z(i);
+        real vz_f[V + 1];  // faces k0 .. k0 + V of the lane's cells
+        {
+            real v[V];
+            vload<real, V>(vz_in + cz, v);
+            const real top = vz_in[cz + V];
+            const real p_lo = z_lo_wall ? real(0) : p_in[c - 1];
+            const real p_hi = z_hi_wall ? real(0) : p_in[c + V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                real t = v[e];
+                if (e > 0) t = t - vel * (p_c[e] - p_c[e - 1]);
+                else if (!z_lo_wall) t = t - vel * (p_c[0] - p_lo);
+                vz_f[e] = damp3<real, LAYER>(t, fxc, fyc, fzf[e]);
+            }
+            real t = top;
+            if (!z_hi_wall) t = t - vel * (p_hi - p_c[V - 1]);
+            vz_f[V] = damp3<real, LAYER>(t, fxc, fyc, fzf[V]);
+            real lower[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) lower[e] = vz_f[e];
+            vstore<real, V>(vz_out + cz, lower);
+            if (z_hi_wall) vz_out[cz + V] = vz_f[V];
+        }
+
+        real p_new[V];
+        real states[NR > 0 ? NR : 1][V];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vload<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
+        const int code_xy = WALLS || a.wall_tape
+                                ? (wall_class(j, ny) << 2) | wall_class(i, nx)
+                                : 0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const real div =
+                ((vx_hi[e] - vx_lo[e]) + (vy_hi[e] - vy_lo[e])) + (vz_f[e + 1] - vz_f[e]);
+            const real p_old = p_c[e];
+            real pn;
+            if (COMPENSATED) {
+                real s = p_old, err = 0;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const real target = div * a.gain[r];
+                    const real offset = states[r][e] - target;
+                    states[r][e] = offset * a.decay[r] + target;
+                    two_sum_acc(s, offset * a.mean[r], err);
+                }
+                if (NR > 0) two_sum_acc(s, div * a.relax_scale, err);
+                const real prod = div * a.pres;
+                const real prod_err = fma_real(div, a.pres, -prod);
+                two_sum_acc(s, -prod, err);
+                pn = s + (err - prod_err);
+            } else {
+                pn = p_old;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const real target = div * a.gain[r];
+                    const real offset = states[r][e] - target;
+                    states[r][e] = offset * a.decay[r] + target;
+                    pn = pn + offset * a.mean[r];
+                }
+                if (NR > 0) pn = pn + div * a.relax_scale;
+                pn = pn - div * a.pres;
+            }
+            const int k = k0 + e;
+            if (WALLS || a.wall_tape) {
+                const int code = code_xy | (wall_class(k, nz) << 4);
+                if (a.wall_tape && code)
+                    tape_store(a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride,
+                               TapePlanes(nx, ny, nz), nx, ny, nz, i, j, k, code, pn + p_old);
+                if (WALLS && ((a.wall_mask >> code) & 1ull))
+                    pn = pn * wall_table[code] -
+                         p_old * wall_table[ACFDTD_HIP_WALL_CODES + code];
+            }
+            if (LAYER) pn = ((pn * fxc) * fyc) * fzc[e];
+            p_new[e] = pn;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vstore<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
+        // Sources in insertion order, then receivers; the plane test is wave-uniform.
+        for (int s = 0; s < a.n_sources; ++s) {
+            const int e = a.source_cell[2 * s + 1] - jk0;
+            if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+                    if (q == e) p_new[q] = p_new[q] + a.src * a.source_signal[s * a.source_stride + step];
+            }
+        }
+        vstore<real, V>(p_out + c, p_new);
+        for (int r = 0; r < a.n_receivers; ++r) {
+            const int e = a.receiver_cell[2 * r + 1] - jk0;
+            if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+                    if (q == e) recording[r * a.recording_stride + column] = p_new[q];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            vx_lo[e] = vx_hi[e];
+            p_c[e] = p_up[e];
+        }
+    }
+}
+
+
+template <typename real>
+FusedIO<real> fused_io(const acfdtd_hip_desc &d, bool reverse) {
+    FusedIO<real> io;
+    real *a[4] = {static_cast<real *>(d.p), static_cast<real *>(d.vx), static_cast<real *>(d.vy),
+                  static_cast<real *>(d.vz)};
+    real *b[4] = {static_cast<real *>(d.p_swap), static_cast<real *>(d.vx_swap),
+                  static_cast<real *>(d.vy_swap), static_cast<real *>(d.vz_swap)};
+    real **in = reverse ? b : a;
+    real **out = reverse ? a : b;
+    io.p_in = in[0];
+    io.vx_in = in[1];
+    io.vy_in = in[2];
+    io.vz_in = in[3];
+    io.p_out = out[0];
+    io.vx_out = out[1];
+    io.vy_out = out[2];
+    io.vz_out = out[3];
+    return io;
+}
+
+inline bool has_swap(const acfdtd_hip_desc &d) {
+    return d.p_swap && d.vx_swap && d.vy_swap && d.vz_swap;
+}
+
+// Launch the fused step for the leading even number of n_steps (pairs A -> B -> A); returns the
+// number of steps launched. Defined in fused.hip.
+template <typename real>
+int64_t launch_fused_pairs(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
+                           int64_t column, hipStream_t stream);
+
+}  // namespace acfdtd

@@ -10,6 +10,8 @@
 
 namespace acfdtd {
 
+// Message of the last failure on this thread (defined in sweeps.hip).
+extern thread_local std::string g_last_error;
 int fail(int code, const std::string &message);
 int check_hip(hipError_t status, const char *what);
 
@@ -17,6 +19,7 @@ int check_hip(hipError_t status, const char *what);
 // `launched` rounds the tile count up to a multiple of the 8 XCDs so that the remap in
 // decode_tile is a bijection; the surplus workgroups exit at once.
 struct Tiling {
+    int vec;  // consecutive z cells per lane
     int tile_j;
     int chunk;
     int nblk_k;
@@ -28,10 +31,11 @@ struct Tiling {
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
-inline Tiling make_tiling(int nx, int ny, int nz, int tile_j, int chunk) {
+inline Tiling make_tiling(int nx, int ny, int nz, int tile_j, int chunk, int vec = 1) {
     Tiling t{};
+    t.vec = vec;
     t.tile_j = tile_j > 0 ? tile_j : 4;
-    t.nblk_k = ceil_div(nz, 64);
+    t.nblk_k = ceil_div(nz, 64 * vec);
     t.nblk_j = ceil_div(ny, t.tile_j);
     if (chunk <= 0) {
         // Enough workgroups to fill 256 CUs several times over, but never fewer than 4 planes
@@ -50,6 +54,19 @@ inline Tiling make_tiling(int nx, int ny, int nz, int tile_j, int chunk) {
 
 inline Tiling tiling_for(const acfdtd_hip_desc &d) {
     return make_tiling(d.nx, d.ny, d.nz, d.tile_j, d.chunk_x);
+}
+
+// z cells per lane of the fused step: as many as the rows allow (4 bytes x vec per access),
+// 4 for fp32 and 2 for fp64 when nz is a multiple, else fewer.
+inline int fused_vec(const acfdtd_hip_desc &d) {
+    const int widest = d.real_bytes == 4 ? 4 : 2;
+    for (int v = widest; v > 1; v /= 2)
+        if (d.nz % v == 0) return v;
+    return 1;
+}
+
+inline Tiling fused_tiling(const acfdtd_hip_desc &d) {
+    return make_tiling(d.nx, d.ny, d.nz, d.tile_j, d.chunk_x, fused_vec(d));
 }
 
 // Consecutive workgroup ids go to the 8 XCDs in turn; give XCD x the contiguous tile range

@@ -5,7 +5,8 @@
  * (src/ac_fdtd/c_backend.py:83-96, :302-356). This library is the device counterpart: the
  * same struct-and-run shape, plain C types only, device pointers instead of host pointers,
  * and an explicit HIP stream. Callers pass memory they own (torch-ROCm tensors, hipMalloc),
- * including the energy workspace; these entry points allocate nothing and keep no state.
+ * including every workspace; no entry point allocates device memory, and only the batch API
+ * keeps state (a small host handle).
  *
  * Field layout (shared with the reference): C order (nx, ny, nz), z unit-stride.
  *   p   : nx*ny*nz            vx : (nx+1)*ny*nz
@@ -145,6 +146,25 @@ int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t step, int64_t
  * counterpart. */
 int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step, int64_t recording_column,
                           int reverse, void *stream);
+
+/* ----------------------------------------------------------------------- batched rooms --
+ * Many rooms advance together, one launch per step (one fused launch per step with swap
+ * buffers, else two sweeps): a tile table covering every room is built once in a caller-owned
+ * device workspace, with each room's arguments beside it. Rooms must share real_bytes,
+ * n_relax, has_layer, flags, tile_j and whether swap buffers are given; sizes, walls, air,
+ * sources and receivers may differ. No reference counterpart (the reference runs one room). */
+typedef struct acfdtd_hip_batch acfdtd_hip_batch; /* opaque host handle */
+
+size_t acfdtd_hip_batch_workspace_bytes(const acfdtd_hip_desc *rooms, int32_t n_rooms);
+int acfdtd_hip_batch_create(const acfdtd_hip_desc *rooms, int32_t n_rooms, void *workspace,
+                            size_t workspace_bytes, void *stream, acfdtd_hip_batch **out);
+/* Re-upload the rooms' descriptors (new recording buffers, edited coefficients); the rooms'
+ * sizes and tiling must be those the batch was created with. Synchronises `stream`. */
+int acfdtd_hip_batch_update(acfdtd_hip_batch *batch, const acfdtd_hip_desc *rooms,
+                            int32_t n_rooms, void *stream);
+int acfdtd_hip_batch_run(acfdtd_hip_batch *batch, int64_t step_offset, int64_t n_steps,
+                         int64_t recording_column, void *stream);
+void acfdtd_hip_batch_destroy(acfdtd_hip_batch *batch);
 
 /* Elements per step row of the boundary tape: 2 (ny nz + nx nz + nx ny). */
 int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
