@@ -11,32 +11,15 @@
 
 namespace acfdtd {
 
-template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
-__global__ __launch_bounds__(512) void fused_step(const SweepArgs<real> a, const FusedIO<real> io,
-                                                  const int64_t step, const int64_t column) {
-    int kb, jb, xc;
-    if (!decode_tile(a.tiles, kb, jb, xc)) return;
-    const int k = kb * 64 + (int)threadIdx.x;
-    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
-    if (k >= a.nz || j >= a.ny) return;
-    const int i0 = xc * a.tiles.chunk;
-    const int i1 = min(i0 + a.tiles.chunk, a.nx);
-    fused_march<real, NR, WALLS, LAYER, COMPENSATED>(
-        a, i0, i1, j, k, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
-        io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
-        a.face[1], a.cell[2], a.face[2], a.recording);
-}
 
 
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
 __global__ __launch_bounds__(512) void fused_step_vec(const SweepArgs<real> a,
                                                       const FusedIO<real> io, const int64_t step,
                                                       const int64_t column) {
-    int kb, jb, xc;
+    int kb, jb, xc, j, k0;
     if (!decode_tile(a.tiles, kb, jb, xc)) return;
-    const int k0 = (kb * 64 + (int)threadIdx.x) * V;
-    const int j = jb * (int)blockDim.y + (int)threadIdx.y;
-    if (k0 >= a.nz || j >= a.ny) return;
+    if (!flat_lane(a.tiles, a.ny, a.nz, jb, j, k0)) return;
     const int i0 = xc * a.tiles.chunk;
     const int i1 = min(i0 + a.tiles.chunk, a.nx);
     fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(
@@ -52,8 +35,7 @@ using FusedKernel = void (*)(const SweepArgs<real>, const FusedIO<real>, const i
 
 template <typename real, int V, int NR, bool W, bool L, bool C>
 FusedKernel<real> fused_kernel() {
-    if constexpr (V == 1) return fused_step<real, NR, W, L, C>;
-    else return fused_step_vec<real, V, NR, W, L, C>;
+    return fused_step_vec<real, V, NR, W, L, C>;
 }
 
 // Compensated kernels exist for fp32 only, and 4-wide lanes only for fp32 (16 bytes).

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "acfdtd_hip.h"
 #include "sweeps.hpp"
@@ -21,6 +22,7 @@ namespace acfdtd {
 #ifndef ACFDTD_X_UNROLL
 #define ACFDTD_X_UNROLL 2
 #endif
+constexpr int kXUnroll = ACFDTD_X_UNROLL;
 
 template <typename real, bool LAYER, bool FUSED_FMA>
 __device__ __forceinline__ void velocity_march(
@@ -39,7 +41,7 @@ __device__ __forceinline__ void velocity_march(
         fzf = face_z[k];
     }
     real p_prev = i0 > 0 ? p[L.cell(i0 - 1)] : real(0);
-#pragma unroll ACFDTD_X_UNROLL
+#pragma unroll kXUnroll
     for (int i = i0; i < i1; ++i) {
         const int64_t c = L.cell(i);
         const real pc = p[c];
@@ -109,7 +111,7 @@ __device__ __forceinline__ void pressure_march(
     }
 
     real vx_lo_next = vx[L.cell(i0)];
-#pragma unroll ACFDTD_X_UNROLL
+#pragma unroll kXUnroll
     for (int i = i0; i < i1; ++i) {
         const int64_t c = L.cell(i);
         const real vx_lo = vx_lo_next;
@@ -242,7 +244,7 @@ __device__ __forceinline__ void fused_march(
         vx_out[L.cell(i0)] = vx_lo;
     }
 
-#pragma unroll ACFDTD_X_UNROLL
+#pragma unroll kXUnroll
     for (int i = i0; i < i1; ++i) {
         const int64_t c = L.cell(i);
         const real fxc = LAYER ? cell_x[i] : real(1);
@@ -348,12 +350,18 @@ __device__ __forceinline__ void fused_march(
 
 // ----------------------------------------------------------- fused sweep, V cells per lane --
 //
-// The same step as fused_march with each lane owning V consecutive z cells, so every field
-// access is a V-wide vector access (dwordx2 / dwordx4): fp32 moves too few bytes per memory
-// instruction with one cell per lane to keep HBM busy. The arithmetic per cell is exactly the
-// scalar kernel's, so the bits are the same. Used when nz is a multiple of V (then every row
-// start, and every vector, is V-element aligned in all fields except vz, whose rows are nz + 1
-// long; its accesses only need the 4-byte alignment gfx950 requires).
+// The same step as fused_march with each lane owning V consecutive z cells (V = 1, 2 or 4), so
+// every field access is a V-wide vector access (dwordx2 / dwordx4): fp32 with one cell per
+// lane issues too few bytes per memory instruction to keep HBM busy. The arithmetic per cell
+// is exactly the scalar kernel's, so the bits are the same. Used when nz is a multiple of V:
+// then each lane's cells lie in one row and every vector is V-element aligned in all fields
+// except vz (rows of nz + 1), whose accesses only need the 4-byte alignment gfx950 requires.
+//
+// The body is written so that no load depends on a branch: neighbours that do not exist at a
+// wall are loaded from a clamped (valid) address and discarded with a select, wall factors of
+// the interior planes are looked up once per lane, the two x-wall planes are peeled off the
+// march, and sources, receivers and the boundary tape sit behind per-lane flags computed once.
+// The compiler can then issue all of a plane's loads before the first wait.
 
 template <typename real, int V>
 struct VecOf {
@@ -362,17 +370,38 @@ struct VecOf {
 
 template <typename real, int V>
 __device__ __forceinline__ void vload(const real *__restrict__ p, real (&out)[V]) {
-    const typename VecOf<real, V>::type v = *reinterpret_cast<const typename VecOf<real, V>::type *>(p);
+    if constexpr (V == 1) {
+        out[0] = *p;
+    } else {
+        const typename VecOf<real, V>::type v =
+            *reinterpret_cast<const typename VecOf<real, V>::type *>(p);
 #pragma unroll
-    for (int e = 0; e < V; ++e) out[e] = v[e];
+        for (int e = 0; e < V; ++e) out[e] = v[e];
+    }
 }
 
 template <typename real, int V>
 __device__ __forceinline__ void vstore(real *__restrict__ p, const real (&in)[V]) {
-    typename VecOf<real, V>::type v;
+    if constexpr (V == 1) {
+        *p = in[0];
+    } else {
+        typename VecOf<real, V>::type v;
 #pragma unroll
-    for (int e = 0; e < V; ++e) v[e] = in[e];
-    *reinterpret_cast<typename VecOf<real, V>::type *>(p) = v;
+        for (int e = 0; e < V; ++e) v[e] = in[e];
+        *reinterpret_cast<typename VecOf<real, V>::type *>(p) = v;
+    }
+}
+
+// Lane position in the flattened (y, z / V) plane used by the V-wide fused kernels: consecutive
+// lanes take consecutive V-cell groups of a row and run on into the next row, so no lane idles
+// whatever nz is. Returns false for lanes past the plane.
+__device__ inline bool flat_lane(const Tiling &t, int ny, int nz, int jb, int &j, int &k0) {
+    const int groups = nz / t.vec;
+    const int64_t flat = (int64_t)jb * (64 * t.tile_j) + threadIdx.y * 64 + threadIdx.x;
+    if (flat >= (int64_t)ny * groups) return false;
+    j = (int)(flat / groups);
+    k0 = (int)(flat - (int64_t)j * groups) * t.vec;
+    return true;
 }
 
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
@@ -386,12 +415,18 @@ __device__ __forceinline__ void fused_march_vec(
     const real *__restrict__ face_x, const real *__restrict__ cell_y,
     const real *__restrict__ face_y, const real *__restrict__ cell_z,
     const real *__restrict__ face_z, real *__restrict__ recording) {
-    const Layout L(a.nx, a.ny, a.nz, j, k0);
-    const int64_t n_cells = L.plane * a.nx;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const Layout L(nx, ny, nz, j, k0);
+    const int64_t n_cells = L.plane * nx;
     const int jk0 = j * nz + k0;
     const real vel = a.vel;
-    const bool z_lo_wall = k0 == 0, z_hi_wall = k0 + V == nz;
+
+    // Neighbours that exist, and clamped offsets to read in their place when they do not.
+    const bool has_ylo = j > 0, has_yhi = j + 1 < ny;
+    const bool has_zlo = k0 > 0, has_zhi = k0 + V < nz;
+    const int64_t dy_lo = has_ylo ? nz : 0, dy_hi = has_yhi ? nz : 0;
+    const int64_t dz_lo = has_zlo ? 1 : 0, dz_hi = has_zhi ? V : V - 1;
+
     real fyc = 1, fyf_lo = 1, fyf_hi = 1, fzc[V], fzf[V + 1];
 #pragma unroll
     for (int e = 0; e < V; ++e) fzc[e] = 1;
@@ -407,100 +442,114 @@ __device__ __forceinline__ void fused_march_vec(
         for (int e = 0; e <= V; ++e) fzf[e] = face_z[k0 + e];
     }
 
+    // Wall classes: y and z part per cell; the factors of interior planes (no x wall).
+    int code_yz[V];
+    bool wall_in[V];
+    real fu_in[V], fp_in[V];
+    const bool yz_boundary = !has_ylo || !has_yhi || !has_zlo || !has_zhi;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        code_yz[e] = (wall_class(j, ny) << 2) | (wall_class(k0 + e, nz) << 4);
+        wall_in[e] = WALLS && ((a.wall_mask >> code_yz[e]) & 1ull);
+        fu_in[e] = 1;
+        fp_in[e] = 0;
+    }
+    if (WALLS && yz_boundary) {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            if (wall_in[e]) {
+                fu_in[e] = wall_table[code_yz[e]];
+                fp_in[e] = wall_table[ACFDTD_HIP_WALL_CODES + code_yz[e]];
+            }
+    }
+
+    // Does any source or receiver lie in this lane's cells (in some plane)?
+    bool special = false;
+    for (int s = 0; s < a.n_sources; ++s) {
+        const int e = a.source_cell[2 * s + 1] - jk0;
+        special = special || (e >= 0 && e < V);
+    }
+    for (int r = 0; r < a.n_receivers; ++r) {
+        const int e = a.receiver_cell[2 * r + 1] - jk0;
+        special = special || (e >= 0 && e < V);
+    }
+
+    // New vx on face i0, the lower face of the first cell of the march.
     real p_c[V], vx_lo[V];
     vload<real, V>(p_in + L.cell(i0), p_c);
     {
         real v[V], prev[V];
         vload<real, V>(vx_in + L.cell(i0), v);
-        if (i0 > 0) vload<real, V>(p_in + L.cell(i0 - 1), prev);
+        vload<real, V>(p_in + L.cell(i0 > 0 ? i0 - 1 : i0), prev);
         const real fxf = LAYER ? face_x[i0] : real(1);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-            real w = v[e];
-            if (i0 > 0) w = w - vel * (p_c[e] - prev[e]);
+            const real w = i0 > 0 ? v[e] - vel * (p_c[e] - prev[e]) : v[e];
             vx_lo[e] = damp3<real, LAYER>(w, fxf, fyc, fzc[e]);
         }
         vstore<real, V>(vx_out + L.cell(i0), vx_lo);
     }
 
-#pragma unroll ACFDTD_X_UNROLL
-    for (int i = i0; i < i1; ++i) {
+    // One plane; XWALL marks the planes i = 0 and i = nx - 1, where x walls and the missing
+    // upper neighbour are handled.
+    auto plane = [&](const int i, auto xwall_tag) {
+        constexpr bool XWALL = decltype(xwall_tag)::value;
         const int64_t c = L.cell(i);
+        const bool up_interior = !XWALL || i + 1 < nx;
         const real fxc = LAYER ? cell_x[i] : real(1);
+        const real fxf_hi = LAYER ? face_x[i + 1] : real(1);
 
-        real vx_hi[V], p_up[V];
-        {
-            real v[V];
-            vload<real, V>(vx_in + c + L.plane, v);
-            const bool interior = i + 1 < nx;
-            if (interior) vload<real, V>(p_in + c + L.plane, p_up);
-            const real fxf = LAYER ? face_x[i + 1] : real(1);
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                real w = v[e];
-                if (interior) w = w - vel * (p_up[e] - p_c[e]);
-                vx_hi[e] = damp3<real, LAYER>(w, fxf, fyc, fzc[e]);
-            }
-            if (!interior)
-#pragma unroll
-                for (int e = 0; e < V; ++e) p_up[e] = 0;
-            if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + c + L.plane, vx_hi);
-        }
+        // Loads, none behind a branch.
+        real vx_v[V], p_up[V], vy_lo_in[V], vy_hi_in[V], q_lo[V], q_hi[V], vz_v[V];
+        vload<real, V>(vx_in + c + L.plane, vx_v);
+        vload<real, V>(p_in + c + (up_interior ? L.plane : 0), p_up);
         const int64_t cy = L.face_y(i);
-        real vy_lo[V], vy_hi[V];
-        {
-            real v[V], w[V], q[V];
-            vload<real, V>(vy_in + cy, v);
-            if (j > 0) vload<real, V>(p_in + c - nz, q);
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                real t = v[e];
-                if (j > 0) t = t - vel * (p_c[e] - q[e]);
-                vy_lo[e] = damp3<real, LAYER>(t, fxc, fyf_lo, fzc[e]);
-            }
-            vstore<real, V>(vy_out + cy, vy_lo);
-            vload<real, V>(vy_in + cy + nz, w);
-            if (j + 1 < ny) vload<real, V>(p_in + c + nz, q);
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                real t = w[e];
-                if (j + 1 < ny) t = t - vel * (q[e] - p_c[e]);
-                vy_hi[e] = damp3<real, LAYER>(t, fxc, fyf_hi, fzc[e]);
-            }
-            if (j + 1 == ny) vstore<real, V>(vy_out + cy + nz, vy_hi);
-        }
+        vload<real, V>(vy_in + cy, vy_lo_in);
+        vload<real, V>(vy_in + cy + nz, vy_hi_in);
+        vload<real, V>(p_in + c - dy_lo, q_lo);
+        vload<real, V>(p_in + c + dy_hi, q_hi);
         const int64_t cz = L.face_z(i);
-        real vz_f[V + 1];  // faces k0 .. k0 + V of the lane's cells
-        {
-            real v[V];
-            vload<real, V>(vz_in + cz, v);
-            const real top = vz_in[cz + V];
-            const real p_lo = z_lo_wall ? real(0) : p_in[c - 1];
-            const real p_hi = z_hi_wall ? real(0) : p_in[c + V];
+        vload<real, V>(vz_in + cz, vz_v);
+        const real vz_top = vz_in[cz + V];
+        const real pz_lo = p_in[c - dz_lo];
+        const real pz_hi = p_in[c + dz_hi];
+        real states[NR > 0 ? NR : 1][V];
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-                real t = v[e];
-                if (e > 0) t = t - vel * (p_c[e] - p_c[e - 1]);
-                else if (!z_lo_wall) t = t - vel * (p_c[0] - p_lo);
-                vz_f[e] = damp3<real, LAYER>(t, fxc, fyc, fzf[e]);
-            }
-            real t = top;
-            if (!z_hi_wall) t = t - vel * (p_hi - p_c[V - 1]);
-            vz_f[V] = damp3<real, LAYER>(t, fxc, fyc, fzf[V]);
+        for (int r = 0; r < NR; ++r) vload<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
+
+        // New velocities on the six faces of each cell.
+        real vx_hi[V], vy_lo[V], vy_hi[V], vz_f[V + 1];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const real wx = up_interior ? vx_v[e] - vel * (p_up[e] - p_c[e]) : vx_v[e];
+            vx_hi[e] = damp3<real, LAYER>(wx, fxf_hi, fyc, fzc[e]);
+            const real wl = has_ylo ? vy_lo_in[e] - vel * (p_c[e] - q_lo[e]) : vy_lo_in[e];
+            vy_lo[e] = damp3<real, LAYER>(wl, fxc, fyf_lo, fzc[e]);
+            const real wh = has_yhi ? vy_hi_in[e] - vel * (q_hi[e] - p_c[e]) : vy_hi_in[e];
+            vy_hi[e] = damp3<real, LAYER>(wh, fxc, fyf_hi, fzc[e]);
+            real wz;
+            if (e > 0) wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
+            else wz = has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+            vz_f[e] = damp3<real, LAYER>(wz, fxc, fyc, fzf[e]);
+        }
+        {
+            const real wz = has_zhi ? vz_top - vel * (pz_hi - p_c[V - 1]) : vz_top;
+            vz_f[V] = damp3<real, LAYER>(wz, fxc, fyc, fzf[V]);
+        }
+        if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + c + L.plane, vx_hi);
+        vstore<real, V>(vy_out + cy, vy_lo);
+        if (!has_yhi) vstore<real, V>(vy_out + cy + nz, vy_hi);
+        {
             real lower[V];
 #pragma unroll
             for (int e = 0; e < V; ++e) lower[e] = vz_f[e];
             vstore<real, V>(vz_out + cz, lower);
-            if (z_hi_wall) vz_out[cz + V] = vz_f[V];
         }
+        if (!has_zhi) vz_out[cz + V] = vz_f[V];
 
-        real p_new[V];
-        real states[NR > 0 ? NR : 1][V];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) vload<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
-        const int code_xy = WALLS || a.wall_tape
-                                ? (wall_class(j, ny) << 2) | wall_class(i, nx)
-                                : 0;
+        // Pressure with the relaxation states, walls and sponge.
+        const int cx = XWALL ? wall_class(i, nx) : 0;
+        real p_new[V], tape_value[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real div =
@@ -533,37 +582,56 @@ __device__ __forceinline__ void fused_march_vec(
                 if (NR > 0) pn = pn + div * a.relax_scale;
                 pn = pn - div * a.pres;
             }
-            const int k = k0 + e;
-            if (WALLS || a.wall_tape) {
-                const int code = code_xy | (wall_class(k, nz) << 4);
-                if (a.wall_tape && code)
-                    tape_store(a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride,
-                               TapePlanes(nx, ny, nz), nx, ny, nz, i, j, k, code, pn + p_old);
-                if (WALLS && ((a.wall_mask >> code) & 1ull))
-                    pn = pn * wall_table[code] -
-                         p_old * wall_table[ACFDTD_HIP_WALL_CODES + code];
+            tape_value[e] = pn + p_old;
+            if (WALLS) {
+                bool is_wall = wall_in[e];
+                real fu = fu_in[e], fp = fp_in[e];
+                if (XWALL) {
+                    const int code = code_yz[e] | cx;
+                    is_wall = (a.wall_mask >> code) & 1ull;
+                    if (is_wall) {
+                        fu = wall_table[code];
+                        fp = wall_table[ACFDTD_HIP_WALL_CODES + code];
+                    }
+                }
+                if (is_wall) pn = pn * fu - p_old * fp;
             }
             if (LAYER) pn = ((pn * fxc) * fyc) * fzc[e];
             p_new[e] = pn;
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) vstore<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
-        // Sources in insertion order, then receivers; the plane test is wave-uniform.
-        for (int s = 0; s < a.n_sources; ++s) {
-            const int e = a.source_cell[2 * s + 1] - jk0;
-            if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
+        if (a.wall_tape && (XWALL || yz_boundary)) {
+            real *row = a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride;
 #pragma unroll
-                for (int q = 0; q < V; ++q)
-                    if (q == e) p_new[q] = p_new[q] + a.src * a.source_signal[s * a.source_stride + step];
+            for (int e = 0; e < V; ++e) {
+                const int code = code_yz[e] | cx;
+                if (code)
+                    tape_store(row, TapePlanes(nx, ny, nz), nx, ny, nz, i, j, k0 + e, code,
+                               tape_value[e]);
+            }
+        }
+        if (special) {
+            // Sources in insertion order, after the sponge (scheme.py:360-363).
+            for (int s = 0; s < a.n_sources; ++s) {
+                const int e = a.source_cell[2 * s + 1] - jk0;
+                if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
+                    const real add = a.src * a.source_signal[s * a.source_stride + step];
+#pragma unroll
+                    for (int q = 0; q < V; ++q)
+                        if (q == e) p_new[q] = p_new[q] + add;
+                }
             }
         }
         vstore<real, V>(p_out + c, p_new);
-        for (int r = 0; r < a.n_receivers; ++r) {
-            const int e = a.receiver_cell[2 * r + 1] - jk0;
-            if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
+        if (special) {
+            for (int r = 0; r < a.n_receivers; ++r) {
+                const int e = a.receiver_cell[2 * r + 1] - jk0;
+                if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
 #pragma unroll
-                for (int q = 0; q < V; ++q)
-                    if (q == e) recording[r * a.recording_stride + column] = p_new[q];
+                    for (int q = 0; q < V; ++q)
+                        if (q == e) recording[r * a.recording_stride + column] = p_new[q];
+                }
             }
         }
 #pragma unroll
@@ -571,9 +639,20 @@ __device__ __forceinline__ void fused_march_vec(
             vx_lo[e] = vx_hi[e];
             p_c[e] = p_up[e];
         }
-    }
-}
+    };
 
+    using Wall = std::integral_constant<bool, true>;
+    using Interior = std::integral_constant<bool, false>;
+    int i = i0;
+    if (i == 0) {
+        plane(0, Wall{});
+        ++i;
+    }
+    const int stop = i1 == nx ? nx - 1 : i1;
+#pragma unroll kXUnroll
+    for (; i < stop; ++i) plane(i, Interior{});
+    if (i1 == nx && i == nx - 1) plane(nx - 1, Wall{});
+}
 
 template <typename real>
 FusedIO<real> fused_io(const acfdtd_hip_desc &d, bool reverse) {

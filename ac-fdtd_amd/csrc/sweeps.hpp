@@ -65,8 +65,15 @@ inline int fused_vec(const acfdtd_hip_desc &d) {
     return 1;
 }
 
+// Tiling of the fused kernels: the (y, z / vec) plane is flattened and cut into workgroups of
+// 64 * tile_j lanes (see flat_lane), so nblk_k is 1 and nblk_j counts those workgroups.
 inline Tiling fused_tiling(const acfdtd_hip_desc &d) {
-    return make_tiling(d.nx, d.ny, d.nz, d.tile_j, d.chunk_x, fused_vec(d));
+    const int vec = fused_vec(d);
+    const int64_t lanes = (int64_t)d.ny * (d.nz / vec);
+    const int tile_j = d.tile_j > 0 ? d.tile_j : 4;
+    Tiling t = make_tiling(d.nx, ceil_div(lanes, 64 * tile_j) * tile_j, 64 * vec, tile_j,
+                           d.chunk_x, vec);
+    return t;
 }
 
 // Consecutive workgroup ids go to the 8 XCDs in turn; give XCD x the contiguous tile range

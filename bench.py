@@ -182,6 +182,13 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-extra", action="store_true")
+    parser.add_argument(
+        "--mode",
+        choices=("bench", "copy"),
+        default="bench",
+        help="copy: five plain 512 MiB device-to-device copies, the known-size reference a "
+        "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by",
+    )
     args = parser.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -199,6 +206,15 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
+
+    if args.mode == "copy":
+        source = torch.ones(1 << 27, dtype=torch.float32, device="cuda")  # 512 MiB
+        target = torch.empty_like(source)
+        for _ in range(5):
+            target.copy_(source)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": source.numel() * 4, "copies": 5}), flush=True)
+        return 0
 
     log(f"fp32 {args.size}^3, {args.steps} steps")
     f32 = gpu_throughput(args.size, args.steps, args.warmup, np.float32)
