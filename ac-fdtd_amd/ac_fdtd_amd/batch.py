@@ -1,0 +1,121 @@
+"""Many rooms, one launch per step: :class:`BatchedRooms` over a list of solvers.
+
+Each room is an ordinary :class:`~ac_fdtd_amd.scheme.AcousticFDTD` (its own grid, walls, air,
+sources, receivers and state tensors); the batch advances them together through the library's
+batch API (``acfdtd_hip_batch_*`` in ``include/acfdtd_hip.h``), one fused launch per step over
+a tile table that covers every room. Results are bit-identical to running each room alone.
+There is no reference counterpart: the reference runs one room at a time.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native
+from .scheme import AcousticFDTD
+
+__all__ = ["BatchedRooms"]
+
+
+class BatchedRooms:
+    """Advance ``solvers`` together. They must share device, precision, relaxation count,
+    sponge presence, compensation and tiling, use the fused path, and be at the same step."""
+
+    def __init__(self, solvers: list[AcousticFDTD]) -> None:
+        if not solvers:
+            raise ValueError("a batch needs at least one room")
+        first = solvers[0]
+        for solver in solvers:
+            if solver.device != first.device or solver.dtype != first.dtype:
+                raise ValueError("rooms of a batch must share device and dtype")
+            if not solver.fused:
+                raise ValueError("batched rooms use the fused path (fused=True)")
+            if solver.step_index != first.step_index:
+                raise ValueError("rooms of a batch must be at the same step")
+        self.solvers = list(solvers)
+        self.device = first.device
+        self._lib = _native.library()
+        self._lib.acfdtd_hip_batch_workspace_bytes.restype = ctypes.c_size_t
+        self._lib.acfdtd_hip_batch_workspace_bytes.argtypes = [ctypes.POINTER(_native.Desc),
+                                                               ctypes.c_int32]
+        self._lib.acfdtd_hip_batch_create.restype = ctypes.c_int
+        self._lib.acfdtd_hip_batch_create.argtypes = [
+            ctypes.POINTER(_native.Desc), ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t,
+            ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        self._lib.acfdtd_hip_batch_update.restype = ctypes.c_int
+        self._lib.acfdtd_hip_batch_update.argtypes = [
+            ctypes.c_void_p, ctypes.POINTER(_native.Desc), ctypes.c_int32, ctypes.c_void_p]
+        self._lib.acfdtd_hip_batch_run.restype = ctypes.c_int
+        self._lib.acfdtd_hip_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64,
+                                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]
+        self._lib.acfdtd_hip_batch_destroy.restype = None
+        self._lib.acfdtd_hip_batch_destroy.argtypes = [ctypes.c_void_p]
+
+        descs = self._descriptors(None)
+        size = self._lib.acfdtd_hip_batch_workspace_bytes(descs, len(self.solvers))
+        if size == 0:
+            raise RuntimeError(
+                "these rooms cannot be batched: "
+                + self._lib.acfdtd_hip_last_error().decode(errors="replace")
+            )
+        self._workspace = torch.empty(size, dtype=torch.uint8, device=self.device)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            status = self._lib.acfdtd_hip_batch_create(
+                descs, len(self.solvers), ctypes.c_void_p(self._workspace.data_ptr()), size,
+                ctypes.c_void_p(self._stream()), ctypes.byref(handle))
+        _native.check(status, "acfdtd_hip_batch_create")
+        self._handle = handle
+
+    @property
+    def step_index(self) -> int:
+        return self.solvers[0].step_index
+
+    @property
+    def n_cells(self) -> int:
+        return sum(solver.grid.n_cells for solver in self.solvers)
+
+    def _stream(self) -> int:
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _descriptors(self, recordings):
+        array = (_native.Desc * len(self.solvers))()
+        for n, solver in enumerate(self.solvers):
+            array[n] = solver._descriptor(None if recordings is None else recordings[n], tape=False)
+        return array
+
+    def run(self, n_steps: int) -> None:
+        """Advance every room ``n_steps``; receivers record as with ``AcousticFDTD.run``."""
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            return
+        if self._handle is None:
+            raise RuntimeError("the batch has been closed")
+        recordings = []
+        for solver in self.solvers:
+            recordings.append(
+                torch.zeros((len(solver._receivers), n_steps), **solver._opts)
+                if solver._receivers else None
+            )
+        descs = self._descriptors(recordings)
+        stream = ctypes.c_void_p(self._stream())
+        with torch.cuda.device(self.device):
+            _native.check(self._lib.acfdtd_hip_batch_update(self._handle, descs,
+                                                            len(self.solvers), stream),
+                          "acfdtd_hip_batch_update")
+            _native.check(self._lib.acfdtd_hip_batch_run(self._handle, self.step_index, n_steps,
+                                                         0, stream), "acfdtd_hip_batch_run")
+        for solver, recording in zip(self.solvers, recordings):
+            solver.step_index += n_steps
+            if recording is not None:
+                solver._runs.append(recording)
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None:
+            self._lib.acfdtd_hip_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
