@@ -7,11 +7,14 @@ width 4-8 cells at a seeded interior cell, 8 receivers at seeded interior cells,
 steps. All state is resident in HBM before timing; the timed region is one call of the library
 advancing ``--steps`` steps, bracketed by HIP events and synchronisations.
 
-Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3; the fp64 plain
-kernel on the same workload, a rigid-box fp64 energy drift and a 384^3 fp32 run are under
-``extra``. ``roofline`` is for the dominant kernel (the pressure sweep), from HIP events around
-each of its launches; ``cpu_baseline`` is the reference's own compiled C loop (``oracle/_ref``,
-all host cores this process may use) on a bounded sample of the same workload.
+Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3. Under ``extra``:
+the fp64 plain kernel and the compensated fp32 kernel on the same workload, the error of every
+fp32 run against the fp64 run (receiver relative L2, T20 per octave band), rigid-box energy
+drift in fp64 and fp32, a 384^3 fp32 run, 64 uneven rooms advanced as one batch (each checked
+against its own fp64 single-room run), and the cost of a wall-admittance gradient against a
+forward pass at 128^3. ``roofline`` is for the dominant kernel (the fused step), from HIP events
+around each of its launches; ``cpu_baseline`` is the reference's own compiled C loop
+(``oracle/_ref``, all host cores this process may use) on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -120,17 +123,48 @@ def gpu_throughput(size, steps, warmup, dtype, compensated=False):
         "kernels": kernels,
         "n_relax": solver.coefficients.n_relax,
         "cells": cells,
+        "recorded": recorded.astype(np.float64),
+        "sample_rate": solver.sample_rate,
     }
 
 
-def rigid_energy_drift(size, steps):
-    """fp64 rigid box from a seeded random pressure: max |E/E0 - 1| over the run."""
+def relative_l2(test: np.ndarray, reference: np.ndarray) -> float:
+    return float(np.linalg.norm(test - reference) / np.linalg.norm(reference))
+
+
+OCTAVES = (500.0, 1000.0, 2000.0, 4000.0)
+
+
+def error_against(test: dict, reference: dict) -> dict:
+    """Receiver relative L2 of ``test`` against ``reference`` and the largest relative T20
+    deviation per octave band over the receivers (bands whose decay is too short for a T20
+    in either run are left out)."""
+    from ac_fdtd_amd.metrics import octave_band_reverberation
+
+    a, b, rate = test["recorded"], reference["recorded"], reference["sample_rate"]
+    worst = {}
+    for channel in range(b.shape[0]):
+        t_test = octave_band_reverberation(a[channel], rate, OCTAVES)
+        t_ref = octave_band_reverberation(b[channel], rate, OCTAVES)
+        for band in OCTAVES:
+            if np.isfinite(t_test[band]) and np.isfinite(t_ref[band]):
+                deviation = abs(t_test[band] / t_ref[band] - 1.0)
+                worst[band] = max(worst.get(band, 0.0), deviation)
+    return {
+        "receiver_rel_l2": relative_l2(a, b),
+        "t20_max_rel_deviation_by_octave": {f"{int(k)}": float(v) for k, v in worst.items()},
+    }
+
+
+def rigid_energy_drift(size, steps, dtype=np.float64):
+    """Rigid box from a seeded random pressure: max |E/E0 - 1| over the run (energy reduced in
+    fp64 on the device for either precision)."""
     import torch
 
     from ac_fdtd_amd import Grid
     from ac_fdtd_amd.scheme import AcousticFDTD
 
-    solver = AcousticFDTD(Grid(dx=DX, nx=size, ny=size, nz=size), dtype=np.float64)
+    solver = AcousticFDTD(Grid(dx=DX, nx=size, ny=size, nz=size), dtype=dtype)
     solver.p.copy_(torch.as_tensor(np.random.default_rng(0).standard_normal(solver.grid.shape)))
     initial = solver.energy()
     drift = 0.0
@@ -142,6 +176,147 @@ def rigid_energy_drift(size, steps):
         done += n
         drift = max(drift, abs(solver.energy() / initial - 1.0))
     return drift
+
+
+def batched_rooms(n_rooms, steps, warmup, check=True, seed=7):
+    """Rooms of sides U[3, 10] m at 1 kHz (the spacing ``Grid.from_max_frequency`` gives:
+    c / (10 x 1 kHz) = 34.3 mm), six per-wall absorptions U[0.05, 0.5], air at T U[15, 25] C and
+    RH U[30, 70] %, one source and 4 receivers at least 0.5 m from every wall, the repo's
+    band-limited impulse at 1 kHz; advanced together in fp32, then each room alone in fp64 for
+    the error."""
+    import torch
+
+    from ac_fdtd_amd import AIR, AirAbsorption, Grid, WallAdmittances, admittance_from_absorption
+    from ac_fdtd_amd.batch import BatchedRooms
+    from ac_fdtd_amd.scheme import AcousticFDTD
+    from ac_fdtd_amd.sources import band_limited_impulse
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n_rooms):
+        sides = rng.uniform(3.0, 10.0, 3)
+        walls = [admittance_from_absorption(a) for a in rng.uniform(0.05, 0.5, 6)]
+        air = (rng.uniform(15.0, 25.0), rng.uniform(30.0, 70.0))
+        points = [tuple(0.5 + rng.uniform(0.0, 1.0, 3) * (sides - 1.0)) for _ in range(5)]
+        specs.append((tuple(sides), walls, air, points))
+
+    def room(spec, dtype):
+        sides, walls, air, points = spec
+        grid = Grid.from_max_frequency(sides, 1000.0, AIR.sound_speed)
+        solver = AcousticFDTD(grid, dtype=dtype, walls=WallAdmittances(*walls),
+                              air_absorption=AirAbsorption(*air))
+        solver.add_volume_source(points[0], band_limited_impulse(warmup + steps, solver.dt, 1000.0))
+        for point in points[1:]:
+            solver.add_pressure_receiver(point)
+        return solver
+
+    rooms = [room(spec, np.float32) for spec in specs]
+    cells = sum(r.grid.n_cells for r in rooms)
+    shapes = [r.grid.shape for r in rooms]
+    batch = BatchedRooms(rooms)
+    batch.run(warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    batch.run(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    groups = len(batch._groups)
+    batch.close()
+    recorded = [r.recorded_pressure.astype(np.float64) for r in rooms]
+    out = {
+        "rooms": n_rooms,
+        "cells": cells,
+        "cells_per_axis_range": [int(min(min(s) for s in shapes)), int(max(max(s) for s in shapes))],
+        "steps": steps,
+        "lane_width_groups": groups,
+        "gcells": round(cells * steps / (ms * 1e-3) / 1e9, 3),
+        "ms_per_step": round(ms / steps, 4),
+    }
+    del rooms, batch
+    torch.cuda.empty_cache()
+    if check:
+        errors = []
+        for spec, mine in zip(specs, recorded):
+            reference = room(spec, np.float64)
+            reference.run(warmup + steps)
+            errors.append(relative_l2(mine, reference.recorded_pressure))
+            del reference
+        torch.cuda.empty_cache()
+        out["receiver_rel_l2_vs_fp64_max"] = float(max(errors))
+        out["receiver_rel_l2_vs_fp64_median"] = float(np.median(errors))
+    return out
+
+
+def gradient_cost(size, steps, seed=11):
+    """Wall-admittance gradient of an L2 receiver loss at ``size``^3, fp32: a hidden target set
+    of six absorptions U[0.05, 0.5], the model at uniform 0.2, 4 receivers, the band-limited
+    impulse at 1 kHz on a 34.3 mm grid. Returns seconds for a forward pass and for forward +
+    backward, with walls only (boundary tape, no checkpoints) and with walls and the air
+    strengths (state checkpoints every 100 steps plus recomputation)."""
+    import torch
+
+    from ac_fdtd_amd import AIR, AirAbsorption, Grid, WallAdmittances, admittance_from_absorption
+    from ac_fdtd_amd.autograd import RoomResponse
+    from ac_fdtd_amd.sources import band_limited_impulse
+
+    rng = np.random.default_rng(seed)
+    dx = AIR.sound_speed / (10.0 * 1000.0)
+    grid = Grid(dx=dx, nx=size, ny=size, nz=size)
+    lengths = np.array(grid.lengths)
+    points = [tuple((0.15 + 0.7 * rng.uniform(0.0, 1.0, 3)) * lengths) for _ in range(5)]
+    hidden = [admittance_from_absorption(a) for a in rng.uniform(0.05, 0.5, 6)]
+    start = admittance_from_absorption(0.2)
+
+    def model(air):
+        from ac_fdtd_amd.grid import max_stable_time_step
+
+        dt = max_stable_time_step(dx, AIR.sound_speed)
+        return RoomResponse(
+            grid, steps, sources=[(points[0], band_limited_impulse(steps, dt, 1000.0))],
+            receivers=points[1:], dtype=np.float32,
+            walls=WallAdmittances.from_absorption(0.2),
+            air_absorption=AirAbsorption(20.0, 50.0) if air else None, checkpoint_every=100,
+        )
+
+    def seconds(call):
+        torch.cuda.synchronize()
+        began = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        return time.perf_counter() - began
+
+    out = {"grid": [size] * 3, "steps": steps}
+    for air in (False, True):
+        room = model(air)
+        target = room(torch.tensor(hidden, dtype=torch.float64)).detach()
+        xi = torch.full((6,), start, dtype=torch.float64, requires_grad=True)
+        strengths = room.default_air_strength().requires_grad_(True) if air else None
+
+        def forward():
+            with torch.no_grad():
+                room(xi.detach(), air_strength=None if strengths is None else strengths.detach())
+
+        def forward_backward():
+            loss = ((room(xi, air_strength=strengths) - target) ** 2).sum()
+            loss.backward()
+
+        forward()
+        forward_backward()
+        t_fwd = min(seconds(forward) for _ in range(3))
+        t_grad = min(seconds(forward_backward) for _ in range(3))
+        key = "walls_and_air_checkpointed" if air else "walls_tape"
+        out[key] = {
+            "forward_s": round(t_fwd, 4),
+            "forward_backward_s": round(t_grad, 4),
+            "grad_over_forward": round(t_grad / t_fwd, 3),
+            "recomputed_steps": room.report.recomputed_steps,
+            "checkpoints": room.report.checkpoints,
+        }
+        if not np.all(np.isfinite(xi.grad.cpu().numpy())):
+            raise RuntimeError("non-finite admittance gradient")
+    return out
 
 
 def cpu_baseline(size, seconds):
@@ -182,6 +357,10 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-extra", action="store_true")
+    parser.add_argument("--batch-rooms", type=int, default=64)
+    parser.add_argument("--batch-steps", type=int, default=3000)
+    parser.add_argument("--skip-check", action="store_true",
+                        help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
         "--mode",
         choices=("bench", "copy"),
@@ -241,14 +420,30 @@ def main() -> int:
         log("fp64 plain")
         f64 = gpu_throughput(args.size, args.steps, args.warmup, np.float64)
         extra["fp64_plain_gcells"] = round(f64["gcells"], 3)
+        extra["fp32_over_fp64_speed"] = round(f32["gcells"] / f64["gcells"], 3)
         extra["fp64_two_sweep_equivalent_hbm_fraction"] = round(
             bytes_per_cell(f64["n_relax"], 8)["two_sweep_step"] * f64["gcells"] / HBM_PEAK_GBS, 4
         )
+        extra["fp32_error_vs_fp64"] = error_against(f32, f64)
+        extra["fp32_compensated_error_vs_fp64"] = error_against(comp, f64)
+        del comp, f64
         log("fp32 384^3")
         big = gpu_throughput(384, args.steps, args.warmup, np.float32)
         extra["fp32_384_gcells"] = round(big["gcells"], 3)
-        log("fp64 rigid energy drift")
+        del big
+        log("rigid-box energy drift, fp64 and fp32")
         extra["fp64_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.steps)
+        extra["fp32_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.steps,
+                                                                  np.float32)
+        torch.cuda.empty_cache()
+        if args.batch_rooms > 0:
+            log(f"{args.batch_rooms} batched rooms, {args.batch_steps} steps")
+            extra["batched_rooms_fp32"] = batched_rooms(args.batch_rooms, args.batch_steps,
+                                                        args.warmup, check=not args.skip_check)
+            log(f"batched: {extra['batched_rooms_fp32']}")
+        log("gradient cost")
+        extra["wall_gradient_cost_fp32"] = gradient_cost(128, 2000)
+        log(f"gradient: {extra['wall_gradient_cost_fp32']}")
         torch.cuda.empty_cache()
 
     baseline = None
