@@ -123,6 +123,12 @@ class AdjointDesc(_c.Structure):
         ("weight_face", _ptr * 3),
         ("dt", _c.c_double),
         ("partials", _ptr),
+        ("ldiv_swap", _ptr),
+        ("lvx_swap", _ptr),
+        ("lvy_swap", _ptr),
+        ("lvz_swap", _ptr),
+        ("fwd_step0", _c.c_int64),
+        ("fwd_count", _c.c_int64),
     ]
 
 
@@ -162,6 +168,14 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_adjoint_cell_sweep.restype = _c.c_int
     lib.acfdtd_hip_adjoint_cell_sweep.argtypes = [
         _c.POINTER(Desc), _c.POINTER(AdjointDesc), _c.c_int64, _c.c_int, _c.c_int, _ptr
+    ]
+    lib.acfdtd_hip_adjoint_run.restype = _c.c_int
+    lib.acfdtd_hip_adjoint_run.argtypes = [
+        _c.POINTER(Desc), _c.POINTER(AdjointDesc), _c.c_int64, _c.c_int64, _ptr
+    ]
+    lib.acfdtd_hip_run_states.restype = _c.c_int
+    lib.acfdtd_hip_run_states.argtypes = [
+        _c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr, _ptr, _ptr, _ptr, _ptr
     ]
     lib.acfdtd_hip_energy_workspace_bytes.restype = _c.c_size_t
     lib.acfdtd_hip_energy_workspace_bytes.argtypes = [_c.POINTER(Desc)]

@@ -282,50 +282,67 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
             adj.weight_cell[a] = weights[a].data_ptr()
             adj.weight_face[a] = weights[3 + a].data_ptr()
 
+    if room.fused:
+        # Second set for the fused adjoint steps (acfdtd_hip_adjoint_run ping-pongs ldiv and
+        # the adjoint velocities between the two).
+        swaps = [torch.zeros(g.shape, **opts)] + [torch.zeros(g.velocity_shape(a), **opts)
+                                                  for a in range(3)]
+        adj.ldiv_swap, adj.lvx_swap, adj.lvy_swap, adj.lvz_swap = (t.data_ptr() for t in swaps)
+
     stream = torch.cuda.current_stream(solver.device).cuda_stream
     handle = ctypes.c_void_p(stream)
 
-    def set_forward(state):
-        if state is None:
+    def use_ring(ring, step0, count):
+        """Forward states for the adjoint: slot q of each ring field holds step step0 + q."""
+        if ring is None:
+            adj.fwd_p = adj.fwd_vx = adj.fwd_vy = adj.fwd_vz = None
+            adj.fwd_step0 = adj.fwd_count = 0
+        else:
+            adj.fwd_p, adj.fwd_vx, adj.fwd_vy, adj.fwd_vz = (t.data_ptr() for t in ring)
+            adj.fwd_step0, adj.fwd_count = step0, count
+
+    def last_local(ring, slot):
+        """The local part of the last forward step, before any adjoint step."""
+        if ring is None:
             adj.fwd_p = adj.fwd_vx = adj.fwd_vy = adj.fwd_vz = None
         else:
-            adj.fwd_p, adj.fwd_vx, adj.fwd_vy, adj.fwd_vz = (t.data_ptr() for t in state)
-
-    def cell(step, stencil, local, state):
-        set_forward(state)
+            adj.fwd_p, adj.fwd_vx, adj.fwd_vy, adj.fwd_vz = (t[slot].data_ptr() for t in ring)
         _native.check(lib.acfdtd_hip_adjoint_cell_sweep(ctypes.byref(fwd), ctypes.byref(adj),
-                                                        step, stencil, local, handle),
+                                                        n_steps - 1, 0, 1, handle),
                       "acfdtd_hip_adjoint_cell_sweep")
 
-    def face(state):
-        set_forward(state)
-        _native.check(lib.acfdtd_hip_adjoint_face_sweep(ctypes.byref(fwd), ctypes.byref(adj),
-                                                        handle), "acfdtd_hip_adjoint_face_sweep")
-
-    def transpose(n, state_before, state_after, last_state):
-        """Adjoint step n; state_before = x^n, state_after = x^{n+1}."""
-        if n == n_steps - 1:
-            cell(n_steps - 1, 0, 1, last_state)
-        face(state_after)
-        cell(n - 1, 1, 1 if n >= 1 else 0, state_before)
+    def adjoint_steps(step_hi, step_lo):
+        _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), step_hi,
+                                                 step_lo, handle), "acfdtd_hip_adjoint_run")
 
     with torch.cuda.device(solver.device):
         if checkpoints:
             K = room.checkpoint_every
+            # One ring of K + 1 states, reused by every segment: slot q = step start + q.
+            ring = [torch.empty((K + 1, *g.shape), **opts)] + [
+                torch.empty((K + 1, *g.velocity_shape(a)), **opts) for a in range(3)
+            ]
+            report.checkpoint_bytes += sum(t.numel() * t.element_size() for t in ring)
+            state_desc = solver._descriptor(None, tape=False)
             for seg in range(len(checkpoints) - 1, -1, -1):
                 start, stop = seg * K, min(seg * K + K, n_steps)
                 solver.load_state(checkpoints[seg], start)
-                states = [solver.save_state(with_relaxation=False)]
-                for _ in range(start, stop):
-                    solver.run(1, record=False)
-                    states.append(solver.save_state(with_relaxation=False))
+                for field, saved in zip(ring, checkpoints[seg][:4]):
+                    field[0].copy_(saved)
+                _native.check(lib.acfdtd_hip_run_states(
+                    ctypes.byref(state_desc), start, stop - start,
+                    *(ctypes.c_void_p(t.data_ptr()) for t in ring), handle),
+                    "acfdtd_hip_run_states")
                 report.recomputed_steps += stop - start
-                for n in range(stop - 1, start - 1, -1):
-                    transpose(n, states[n - start], states[n + 1 - start], states[-1])
-                del states
+                if stop == n_steps:
+                    last_local(ring, n_steps - start)
+                use_ring(ring, start, stop - start + 1)
+                adjoint_steps(stop - 1, start)
+            del ring
         else:
-            for n in range(n_steps - 1, -1, -1):
-                transpose(n, None, None, None)
+            use_ring(None, 0, 0)
+            last_local(None, 0)
+            adjoint_steps(n_steps - 1, 0)
     report.adjoint_steps = n_steps
 
     sums = partials.view(-1, _native.GRAD_SLOTS).sum(dim=0).cpu().numpy()

@@ -30,7 +30,14 @@ INCLUDE_DIR = PACKAGE_DIR.parent.parent / "include"
 #: The library the loader opens; ACFDTD_HIP_LIBRARY points it at another build of the same
 #: sources (tile experiments), the default is the in-tree build.
 LIBRARY = Path(os.environ.get("ACFDTD_HIP_LIBRARY", PACKAGE_DIR / "lib" / "libacfdtd_hip.so"))
-SOURCES = ("sweeps.hip", "fused.hip", "batch.hip", "adjoint.hip")
+SOURCES = (
+    "sweeps.hip",
+    "fused.hip",
+    "batch.hip",
+    "adjoint.hip",
+    "adjoint_fused.hip",
+    "adjoint_fused64.hip",
+)
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 
 FLAGS = [

@@ -31,88 +31,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 
 #include "acfdtd_hip.h"
+#include "adjoint.hpp"
 #include "sweeps.hpp"
 
 namespace acfdtd {
-
-enum : int { kGradWall = 0, kGradSponge = 6, kGradScale = 7, kGradGain = 8, kGradSlots = 12 };
-
-template <typename real>
-struct AdjointArgs {
-    real *lp, *lvx, *lvy, *lvz, *lpsi, *ldiv;
-    const real *grad_recording;
-    int64_t grad_recording_stride;
-    int64_t grad_recording_step0;
-    real *grad_signal;
-    const real *fwd_p, *fwd_vx, *fwd_vy, *fwd_vz;
-    const real *w_cell[3];
-    const real *w_face[3];
-    double *partials;
-    uint32_t grads;
-    real dt;
-};
-
-template <typename real>
-void fill_adjoint(const acfdtd_hip_adjoint_desc &g, AdjointArgs<real> &a) {
-    a.lp = static_cast<real *>(g.lp);
-    a.lvx = static_cast<real *>(g.lvx);
-    a.lvy = static_cast<real *>(g.lvy);
-    a.lvz = static_cast<real *>(g.lvz);
-    a.lpsi = static_cast<real *>(g.lpsi);
-    a.ldiv = static_cast<real *>(g.ldiv);
-    a.grad_recording = static_cast<const real *>(g.grad_recording);
-    a.grad_recording_stride = g.grad_recording_stride;
-    a.grad_recording_step0 = g.grad_recording_step0;
-    a.grad_signal = static_cast<real *>(g.grad_signal);
-    a.fwd_p = static_cast<const real *>(g.fwd_p);
-    a.fwd_vx = static_cast<const real *>(g.fwd_vx);
-    a.fwd_vy = static_cast<const real *>(g.fwd_vy);
-    a.fwd_vz = static_cast<const real *>(g.fwd_vz);
-    for (int q = 0; q < 3; ++q) {
-        a.w_cell[q] = static_cast<const real *>(g.weight_cell[q]);
-        a.w_face[q] = static_cast<const real *>(g.weight_face[q]);
-    }
-    a.partials = g.partials;
-    a.grads = g.grads;
-    a.dt = (real)g.dt;
-}
-
-// Like decode_tile, also returning the tile's own index (its gradient slot).
-__device__ inline bool decode_tile_slot(const Tiling &t, int &kb, int &jb, int &xc, int &slot) {
-    const unsigned n = (unsigned)t.launched;
-    unsigned b = blockIdx.x;
-    b = (b & 7u) * (n >> 3) + (b >> 3);
-    if (b >= (unsigned)t.tiles) return false;
-    slot = (int)b;
-    kb = (int)(b % (unsigned)t.nblk_k);
-    b /= (unsigned)t.nblk_k;
-    jb = (int)(b % (unsigned)t.nblk_j);
-    xc = (int)(b / (unsigned)t.nblk_j);
-    return true;
-}
-
-// Add each thread's kGradSlots accumulators into partials[slot * kGradSlots + q], in a fixed
-// order. Every thread of the workgroup must call it.
-__device__ inline void flush_partials(double (&acc)[kGradSlots], bool active, int slot,
-                                      double *partials) {
-    __shared__ double lds[kGradSlots][8];
-    for (int q = 0; q < kGradSlots; ++q) {
-        double v = wave_sum(acc[q]);
-        if (threadIdx.x == 0) lds[q][threadIdx.y] = v;
-    }
-    __syncthreads();
-    if (active && threadIdx.x == 0 && threadIdx.y == 0) {
-        for (int q = 0; q < kGradSlots; ++q) {
-            double v = 0;
-            for (unsigned w = 0; w < blockDim.y; ++w) v += lds[q][w];
-            if (v != 0) partials[(int64_t)slot * kGradSlots + q] += v;
-        }
-    }
-}
 
 template <typename real, bool LAYER>
 __global__ __launch_bounds__(512) void adjoint_face_sweep(const SweepArgs<real> a,
@@ -200,7 +127,7 @@ __global__ __launch_bounds__(512) void adjoint_face_sweep(const SweepArgs<real> 
             ld_prev = ld;
         }
     }
-    if (sponge) flush_partials(acc, tile, slot, g.partials);
+    if (sponge) flush_partials<kGradSponge, 1>(acc, tile, slot, g.partials);
 }
 
 template <typename real, int NR, bool LAYER>
@@ -383,13 +310,63 @@ int cell_sweep(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &gd, int6
     return check_hip(hipGetLastError(), "adjoint cell sweep launch");
 }
 
+// One adjoint step m by the face and cell sweeps, forward states from the ring (if any).
+template <typename real>
+int two_sweep_adjoint_step(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &gd, int64_t m,
+                           hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    AdjointArgs<real> g;
+    fill_adjoint(gd, g);
+    const bool ring = gd.fwd_count > 0;
+    const dim3 grid(a.tiles.launched), block(64, a.tiles.tile_j);
+    ForwardState<real> after{}, before{};
+    if (ring) {
+        after = forward_slot<real>(d, gd, m + 1 - gd.fwd_step0);
+        before = forward_slot<real>(d, gd, m - gd.fwd_step0);
+    }
+    g.fwd_p = after.p;
+    g.fwd_vx = after.vx;
+    g.fwd_vy = after.vy;
+    g.fwd_vz = after.vz;
+    const FaceKernel<real> fk =
+        d.has_layer ? adjoint_face_sweep<real, true> : adjoint_face_sweep<real, false>;
+    hipLaunchKernelGGL(fk, grid, block, 0, stream, a, g);
+    g.fwd_p = before.p;
+    g.fwd_vx = before.vx;
+    g.fwd_vy = before.vy;
+    g.fwd_vz = before.vz;
+    hipLaunchKernelGGL(pick_cell<real>(d.n_relax, d.has_layer != 0), grid, block, 0, stream, a, g,
+                       m - 1, 1, m >= 1 ? 1 : 0);
+    return check_hip(hipGetLastError(), "adjoint sweep launch");
+}
+
+template <typename real>
+int adjoint_run(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g, int64_t step_hi,
+                int64_t step_lo, hipStream_t stream) {
+    const int64_t n_steps = step_hi - step_lo + 1;
+    int64_t done = 0;
+    if (has_adjoint_swap(g)) {
+        done = launch_adjoint_pairs<real>(d, g, step_hi, n_steps, stream);
+        const int status = check_hip(hipGetLastError(), "fused adjoint launch");
+        if (status != ACFDTD_HIP_OK) return status;
+    }
+    for (int64_t m = step_hi - done; m >= step_lo; --m) {
+        const int status = two_sweep_adjoint_step<real>(d, g, m, stream);
+        if (status != ACFDTD_HIP_OK) return status;
+    }
+    return ACFDTD_HIP_OK;
+}
+
 }  // namespace acfdtd
 
 using namespace acfdtd;
 
 extern "C" size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc) {
     if (validate(desc) != ACFDTD_HIP_OK) return 0;
-    return (size_t)tiling_for(*desc).tiles * kGradSlots;
+    const Tiling t = tiling_for(*desc);
+    const int64_t waves = std::max<int64_t>((int64_t)t.tiles * t.tile_j, adjoint_fused_waves(*desc));
+    return (size_t)waves * kGradSlots;
 }
 
 extern "C" int acfdtd_hip_adjoint_face_sweep(const acfdtd_hip_desc *desc,
@@ -421,4 +398,40 @@ extern "C" int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
     if (desc->real_bytes == 8)
         return cell_sweep<double>(*desc, *adjoint, step, stencil != 0, local != 0, s);
     return cell_sweep<float>(*desc, *adjoint, step, stencil != 0, local != 0, s);
+}
+
+extern "C" int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc,
+                                      const acfdtd_hip_adjoint_desc *adjoint, int64_t step_hi,
+                                      int64_t step_lo, void *stream) {
+    const int status = validate_adjoint(desc, adjoint);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (step_lo < 0 || step_hi < step_lo)
+        return fail(ACFDTD_HIP_EINVAL, "need 0 <= step_lo <= step_hi");
+    // Steps whose local part runs: m - 1 for m in [max(step_lo, 1), step_hi].
+    const int64_t local_lo = step_lo > 0 ? step_lo - 1 : 0, local_hi = step_hi - 1;
+    if (local_hi >= local_lo) {
+        if (desc->n_receivers > 0 &&
+            (local_lo < adjoint->grad_recording_step0 ||
+             local_hi - adjoint->grad_recording_step0 >= adjoint->grad_recording_stride))
+            return fail(ACFDTD_HIP_EINVAL, "recording gradient does not cover these steps");
+        if ((adjoint->grads & ACFDTD_HIP_GRAD_WALLS) &&
+            (local_lo < desc->wall_tape_step0 ||
+             local_hi - desc->wall_tape_step0 >= desc->wall_tape_rows))
+            return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these steps");
+    }
+    const bool sponge = (adjoint->grads & ACFDTD_HIP_GRAD_SPONGE) && desc->has_layer;
+    const bool air = (adjoint->grads & ACFDTD_HIP_GRAD_AIR) && desc->n_relax > 0;
+    if (adjoint->fwd_count < 0) return fail(ACFDTD_HIP_EINVAL, "fwd_count must be >= 0");
+    if (sponge || air) {
+        if (adjoint->fwd_count == 0 || !adjoint->fwd_vx || !adjoint->fwd_vy || !adjoint->fwd_vz ||
+            (sponge && !adjoint->fwd_p))
+            return fail(ACFDTD_HIP_EINVAL, "sponge and air gradients need the forward states");
+        if (step_lo < adjoint->fwd_step0 ||
+            step_hi + 1 - adjoint->fwd_step0 >= adjoint->fwd_count)
+            return fail(ACFDTD_HIP_EINVAL, "forward states do not cover these steps");
+    }
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8) return adjoint_run<double>(*desc, *adjoint, step_hi, step_lo, s);
+    return adjoint_run<float>(*desc, *adjoint, step_hi, step_lo, s);
 }

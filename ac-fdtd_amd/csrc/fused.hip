@@ -136,3 +136,48 @@ extern "C" int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step,
     if (desc->real_bytes == 8) return fused_once<double>(*desc, step, recording_column, reverse != 0, s);
     return fused_once<float>(*desc, step, recording_column, reverse != 0, s);
 }
+
+extern "C" int acfdtd_hip_run_states(const acfdtd_hip_desc *desc, int64_t step_offset,
+                                     int64_t n_steps, void *p_states, void *vx_states,
+                                     void *vy_states, void *vz_states, void *stream) {
+    if (desc == nullptr) return fail(ACFDTD_HIP_EINVAL, "descriptor is NULL");
+    if (!p_states || !vx_states || !vy_states || !vz_states)
+        return fail(ACFDTD_HIP_EINVAL, "state ring pointer is NULL");
+    if (step_offset < 0 || n_steps < 0)
+        return fail(ACFDTD_HIP_EINVAL, "step_offset and n_steps must be >= 0");
+    // The ring replaces the fields; nothing is recorded and no tape is written.
+    acfdtd_hip_desc d = *desc;
+    d.p = p_states;
+    d.vx = vx_states;
+    d.vy = vy_states;
+    d.vz = vz_states;
+    d.n_receivers = 0;
+    d.receiver_cell = nullptr;
+    d.recording = nullptr;
+    d.recording_stride = 0;
+    d.wall_tape = nullptr;
+    d.wall_tape_stride = 0;
+    d.wall_tape_step0 = 0;
+    d.wall_tape_rows = 0;
+    const int status = validate(&d);
+    if (status != ACFDTD_HIP_OK) return status;
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t sizes[4] = {n, (int64_t)(d.nx + 1) * d.ny * d.nz,
+                              (int64_t)d.nx * (d.ny + 1) * d.nz, (int64_t)d.nx * d.ny * (d.nz + 1)};
+    char *bases[4] = {static_cast<char *>(p_states), static_cast<char *>(vx_states),
+                      static_cast<char *>(vy_states), static_cast<char *>(vz_states)};
+    void **in[4] = {&d.p, &d.vx, &d.vy, &d.vz};
+    void **out[4] = {&d.p_swap, &d.vx_swap, &d.vy_swap, &d.vz_swap};
+    for (int64_t q = 0; q < n_steps; ++q) {
+        for (int f = 0; f < 4; ++f) {
+            *in[f] = bases[f] + q * sizes[f] * d.real_bytes;
+            *out[f] = bases[f] + (q + 1) * sizes[f] * d.real_bytes;
+        }
+        const int rc = d.real_bytes == 8 ? fused_once<double>(d, step_offset + q, 0, false, s)
+                                         : fused_once<float>(d, step_offset + q, 0, false, s);
+        if (rc != ACFDTD_HIP_OK) return rc;
+    }
+    return ACFDTD_HIP_OK;
+}

@@ -183,7 +183,7 @@ int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out,
  * followed by a cell sweep that applies the stencil part of step n and the cell-local part of
  * step n - 1; the local part of the last step runs alone first. Gradients accumulate in fp64
  * into `partials` (acfdtd_hip_adjoint_partials_count doubles, zeroed by the caller), 12 slots
- * per workgroup: [0..5] dL/da summed over the cells of each face plane (FACES order),
+ * per wavefront: [0..5] dL/da summed over the cells of each face plane (FACES order),
  * [6] dL/dsigma_max, [7] dL/drelax_scale, [8..11] dL/dgain_r. */
 #define ACFDTD_HIP_GRAD_WALLS 1u  /* needs the forward boundary tape (desc->wall_tape) */
 #define ACFDTD_HIP_GRAD_SPONGE 2u /* needs fwd_* and the grading weights */
@@ -214,6 +214,15 @@ typedef struct acfdtd_hip_adjoint_desc {
     const void *weight_face[3];
     double dt;
     double *partials;
+    /* acfdtd_hip_adjoint_run only. A second set of ldiv, lvx, lvy, lvz (same shapes) lets it
+     * run adjoint steps as one fused sweep each, in pairs from the first set to the second and
+     * back; with any of them NULL it runs the face and cell sweeps. */
+    void *ldiv_swap, *lvx_swap, *lvy_swap, *lvz_swap;
+    /* acfdtd_hip_adjoint_run only: fwd_p / fwd_vx / fwd_vy / fwd_vz are the bases of
+     * fwd_count consecutive forward states (each field's states back to back): slot q holds
+     * (p^n, v^(n-1/2)) with n = fwd_step0 + q. Needed for sponge and air gradients. */
+    int64_t fwd_step0;
+    int64_t fwd_count;
 } acfdtd_hip_adjoint_desc;
 
 size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc);
@@ -222,6 +231,25 @@ int acfdtd_hip_adjoint_face_sweep(const acfdtd_hip_desc *desc,
 int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
                                   const acfdtd_hip_adjoint_desc *adjoint, int64_t step,
                                   int stencil, int local, void *stream);
+
+/* Adjoint steps step_hi, step_hi - 1, ..., step_lo in one call: each is the face sweep and
+ * stencil of step m and the local part of step m - 1 (none for m = 0), as the two entry
+ * points above would do it, with the same results bit for bit in lp, lvx, lvy, lvz, lpsi and
+ * grad_signal (the fp64 gradient partials are summed in another order). The local part of
+ * step step_hi + 1 must already have been applied (the cell sweep with stencil = 0 for the
+ * last step of a run). The state ends in the first set (lp, ldiv, lvx, lvy, lvz). */
+int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc, const acfdtd_hip_adjoint_desc *adjoint,
+                           int64_t step_hi, int64_t step_lo, void *stream);
+
+/* Forward recomputation into a ring of states: slot 0 of p_states, vx_states, vy_states,
+ * vz_states (each field's states back to back) holds (p, v) at step step_offset; advances
+ * n_steps fused steps, writing the state after step step_offset + q into slot q + 1, and the
+ * relaxation states in place in desc->psi. desc->p, vx, vy, vz and the swap buffers are not
+ * used, nothing is recorded and no boundary tape is written. Used to recompute a checkpoint
+ * segment for the adjoint. */
+int acfdtd_hip_run_states(const acfdtd_hip_desc *desc, int64_t step_offset, int64_t n_steps,
+                          void *p_states, void *vx_states, void *vy_states, void *vz_states,
+                          void *stream);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);
