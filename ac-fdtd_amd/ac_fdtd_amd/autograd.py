@@ -67,6 +67,9 @@ class RoomResponse:
         receivers: receiver points.
         checkpoint_every: state checkpoint interval K, used when sponge or air gradients are
             requested (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field sets).
+        autotune: pick the tilings of the forward and of the adjoint step by timing them
+            (:mod:`ac_fdtd_amd.autotune`); gradients then differ from an untuned run only in
+            the summation order of their fp64 partials.
     """
 
     def __init__(
@@ -85,6 +88,7 @@ class RoomResponse:
         checkpoint_every: int = 100,
         fused: bool = True,
         compensated: bool = False,
+        autotune: bool = False,
     ) -> None:
         if n_steps < 1:
             raise ValueError("n_steps must be >= 1")
@@ -101,6 +105,7 @@ class RoomResponse:
         self.device = device
         self.checkpoint_every = int(checkpoint_every)
         self.fused, self.compensated = fused, compensated
+        self.autotune = autotune
         self.source_points = [point for point, _ in sources]
         self.signals = [np.asarray(signal, dtype=np.float64) for _, signal in sources]
         self.receivers = list(receivers)
@@ -161,6 +166,7 @@ class RoomResponse:
             sponge_strength=sigma,
             relaxation_strengths=delta,
             wall_table_always=wall_table_always,
+            autotune=self.autotune,
         )
         signals = self.signals if signal is None else list(
             torch.as_tensor(signal).detach().cpu().double().numpy()
@@ -250,6 +256,10 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
     G = grad_recording.to(**opts).contiguous()
     recording = solver.recording_tensor()
     fwd = solver._descriptor(recording)
+    if room.autotune:
+        from .autotune import adjoint_tiling
+
+        fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, fwdgrad=want_sponge or want_air)
     n_partials = lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd))
     partials = torch.zeros(n_partials, dtype=torch.float64, device=solver.device)
     sources = solver._source_arrays()

@@ -1,0 +1,216 @@
+"""Tile shapes of the fused kernels, picked by timing them on the GPU.
+
+A fused launch cuts each x-plane into workgroups of ``64 x tile_j`` lanes and gives every
+workgroup ``chunk_x`` consecutive planes to march. Which shape is fastest depends on the grid
+(small grids need many short marches to fill 256 CUs; long marches re-read fewer halo planes),
+on the precision and on the terms the step carries, and it differs between the forward step
+and the adjoint step, so it is measured: every candidate shape runs a few steps on scratch
+fields that share the solver's descriptor (coefficients, walls, sponge, relaxation) and is
+timed with HIP events on the launch stream; the fastest wins.
+
+Results are kept per configuration for the life of the process and, when the environment
+variable ``ACFDTD_TUNE_CACHE`` names a JSON file, across processes. A tiling never changes the
+fields a step computes (each cell's arithmetic is the same whatever the tile), only the order
+in which the adjoint's fp64 gradient partials are summed.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from pathlib import Path
+
+import torch
+
+from . import _native
+
+__all__ = ["CANDIDATES", "adjoint_tiling", "forward_tiling"]
+
+#: (tile_j, chunk_x) shapes tried.
+CANDIDATES = tuple((tile_j, chunk) for tile_j in (1, 2, 4, 8) for chunk in (1, 2, 3, 4, 6, 8))
+
+_cache: dict[str, tuple[int, int]] = {}
+_loaded = False
+
+
+def _cache_file() -> Path | None:
+    name = os.environ.get("ACFDTD_TUNE_CACHE")
+    return Path(name) if name else None
+
+
+def _lookup(key: str):
+    global _loaded
+    if not _loaded:
+        _loaded = True
+        path = _cache_file()
+        if path is not None and path.exists():
+            try:
+                stored = json.loads(path.read_text())
+                _cache.update({k: tuple(v) for k, v in stored.items()})
+            except (OSError, ValueError):
+                pass
+    return _cache.get(key)
+
+
+def _store(key: str, value: tuple[int, int]) -> None:
+    _cache[key] = value
+    path = _cache_file()
+    if path is not None:
+        try:
+            stored = json.loads(path.read_text()) if path.exists() else {}
+        except (OSError, ValueError):
+            stored = {}
+        stored[key] = list(value)
+        path.write_text(json.dumps(stored, indent=1, sort_keys=True))
+
+
+def _key(kind: str, solver) -> str:
+    g, k = solver.grid, solver.coefficients
+    name = torch.cuda.get_device_name(solver.device).replace(" ", "_")
+    return (f"{kind}:{name}:{g.nx}x{g.ny}x{g.nz}:f{solver.np_dtype.itemsize * 8}:r{k.n_relax}"
+            f":w{int(solver._wall_mask != 0)}:l{int(k.has_layer)}:c{int(solver.compensated)}")
+
+
+def _steps_per_sample(solver) -> int:
+    """Enough steps for about 2 ms per timing, at least 4, at most 60, even."""
+    bytes_per_cell = solver.np_dtype.itemsize * (8 + 2 * solver.coefficients.n_relax)
+    seconds = solver.grid.n_cells * bytes_per_cell / 4e12
+    steps = int(min(60, max(4, 2e-3 / max(seconds, 1e-9))))
+    return steps + steps % 2
+
+
+class _Scratch:
+    """Scratch forward fields (both sets) with a copy of the solver's descriptor on them; no
+    sources, receivers or tape."""
+
+    def __init__(self, solver) -> None:
+        g, k = solver.grid, solver.coefficients
+        opts = {"device": solver.device, "dtype": solver.dtype}
+        gen = torch.Generator(device=solver.device).manual_seed(0)
+
+        def field(shape):
+            return torch.randn(shape, generator=gen, **opts) * 1e-3
+
+        self.sets = [[field(g.shape)] + [field(g.velocity_shape(a)) for a in range(3)]
+                     for _ in range(2)]
+        self.psi = torch.zeros((k.n_relax, *g.shape), **opts)
+        d = solver._descriptor(None, tape=False)
+        d.n_sources = d.n_receivers = 0
+        d.source_cell = d.source_length = d.source_signal = None
+        d.receiver_cell = d.recording = None
+        d.p, d.vx, d.vy, d.vz = (t.data_ptr() for t in self.sets[0])
+        d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.sets[1])
+        d.psi = self.psi.data_ptr() if k.n_relax else None
+        self.desc = d
+
+
+def _time(solver, launch, shape, steps: int, repeats: int = 2) -> float:
+    """Mean ms per step of ``launch(steps)`` with ``shape`` = (tile_j, chunk_x); best of
+    ``repeats`` timings after one warm-up."""
+    stream = torch.cuda.current_stream(solver.device)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    launch(min(steps, 4), shape)
+    best = float("inf")
+    for _ in range(repeats):
+        start.record(stream)
+        launch(steps, shape)
+        stop.record(stream)
+        stop.synchronize()
+        best = min(best, start.elapsed_time(stop) / steps)
+    return best
+
+
+def _pick(solver, kind, launch, candidates):
+    key = _key(kind, solver)
+    found = _lookup(key)
+    if found is not None:
+        return found
+    steps = _steps_per_sample(solver)
+    with torch.cuda.device(solver.device):
+        timings = {shape: _time(solver, launch, shape, steps) for shape in candidates}
+    best = min(timings, key=timings.get)
+    _store(key, best)
+    return best
+
+
+def forward_tiling(solver, candidates=CANDIDATES) -> tuple[int, int]:
+    """Fastest (tile_j, chunk_x) of the fused forward step for ``solver``'s configuration."""
+    if not solver.fused:
+        return solver.tile_j, solver.chunk_x
+    scratch = _Scratch(solver)
+    lib = solver._lib
+    handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
+
+    def launch(steps, shape):
+        scratch.desc.tile_j, scratch.desc.chunk_x = shape
+        _native.check(lib.acfdtd_hip_run(ctypes.byref(scratch.desc), 0, steps, 0, handle),
+                      "acfdtd_hip_run")
+
+    return _pick(solver, "forward", launch, candidates)
+
+
+def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tuple[int, int]:
+    """Fastest (tile_j, chunk_x) of the fused adjoint step: with the wall gradient when the
+    solver has a wall table, and with the sponge and air gradients too when ``fwdgrad``."""
+    if not solver.fused:
+        return solver.tile_j, solver.chunk_x
+    g, k = solver.grid, solver.coefficients
+    lib = solver._lib
+    opts = {"device": solver.device, "dtype": solver.dtype}
+    scratch = _Scratch(solver)
+    steps = _steps_per_sample(solver)
+    fwd = scratch.desc
+    walls = solver._wall_table is not None
+    tape = None
+    if walls:
+        tape = torch.zeros((steps + 1, solver.wall_tape_stride()), **opts)
+        fwd.wall_tape = tape.data_ptr()
+        fwd.wall_tape_stride, fwd.wall_tape_step0, fwd.wall_tape_rows = tape.shape[1], 0, tape.shape[0]
+    # Nonzero adjoint fields, so that boundary waves reduce their wall partials as in a run.
+    gen = torch.Generator(device=solver.device).manual_seed(1)
+    lam = [[torch.randn(g.shape, generator=gen, **opts)] +
+           [torch.randn(g.velocity_shape(a), generator=gen, **opts) for a in range(3)]
+           for _ in range(2)]
+    lpsi = torch.zeros((k.n_relax, *g.shape), **opts)
+    adj = _native.AdjointDesc()
+    adj.abi_version = _native.ABI_VERSION
+    adj.struct_size = ctypes.sizeof(_native.AdjointDesc)
+    adj.grads = _native.GRAD_WALLS if walls else 0
+    adj.lp, adj.lvx, adj.lvy, adj.lvz = (t.data_ptr() for t in lam[0])
+    ldiv = [torch.randn(g.shape, generator=gen, **opts) for _ in range(2)]
+    adj.ldiv, adj.ldiv_swap = ldiv[0].data_ptr(), ldiv[1].data_ptr()
+    adj.lvx_swap, adj.lvy_swap, adj.lvz_swap = (t.data_ptr() for t in lam[1][1:])
+    adj.lpsi = lpsi.data_ptr() if k.n_relax else None
+    adj.dt = solver.dt
+    ring = weights = None
+    if fwdgrad and (k.has_layer or k.n_relax):
+        # Adjoint steps n .. 1 read forward states 1 .. n + 1.
+        ring = [torch.zeros((steps + 2, *g.shape), **opts)] + [
+            torch.zeros((steps + 2, *g.velocity_shape(a)), **opts) for a in range(3)]
+        adj.fwd_p, adj.fwd_vx, adj.fwd_vy, adj.fwd_vz = (t.data_ptr() for t in ring)
+        adj.fwd_step0, adj.fwd_count = 0, steps + 2
+        if k.n_relax:
+            adj.grads |= _native.GRAD_AIR
+        if k.has_layer:
+            adj.grads |= _native.GRAD_SPONGE
+            weights = [torch.zeros(g.shape[a] + extra, **opts) for extra in (0, 1) for a in range(3)]
+            for a in range(3):
+                adj.weight_cell[a] = weights[a].data_ptr()
+                adj.weight_face[a] = weights[3 + a].data_ptr()
+    size = 0
+    for shape in candidates:
+        fwd.tile_j, fwd.chunk_x = shape
+        size = max(size, lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd)))
+    partials = torch.zeros(max(size, 1), dtype=torch.float64, device=solver.device)
+    adj.partials = partials.data_ptr()
+    handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
+
+    def launch(n, shape):
+        # Adjoint steps n .. 1: local parts of steps n - 1 .. 0 (tape rows 0 .. n - 1).
+        fwd.tile_j, fwd.chunk_x = shape
+        _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), n, 1,
+                                                 handle), "acfdtd_hip_adjoint_run")
+
+    kind = "adjoint-fwdgrad" if ring is not None else "adjoint"
+    return _pick(solver, kind, launch, candidates)

@@ -69,6 +69,9 @@ class AcousticFDTD:
         compensated: fp32 only — error-free summation of the pressure increment.
         fused: one sweep per step through a second set of buffers (default), or two in place.
         tile_j, chunk_x: kernel tiling (rows per workgroup, planes per march); 0 = default.
+        autotune: time the candidate tilings of the fused step on scratch fields of this
+            configuration and keep the fastest (overrides ``tile_j`` / ``chunk_x``; see
+            :mod:`ac_fdtd_amd.autotune`).
     """
 
     def __init__(
@@ -88,6 +91,7 @@ class AcousticFDTD:
         sponge_strength: float | None = None,
         relaxation_strengths=None,
         wall_table_always: bool = False,
+        autotune: bool = False,
     ) -> None:
         self.device = _require_gpu(device)
         self._lib = _native.library()
@@ -149,6 +153,10 @@ class AcousticFDTD:
         self._receiver_tensor = None
         self._runs: list[torch.Tensor] = []  # per run(): (receivers at that time, n_steps)
         self._energy_workspace = None
+        if autotune:
+            from .autotune import forward_tiling
+
+            self.tile_j, self.chunk_x = forward_tiling(self)
 
     # -- reference API ----------------------------------------------------------------------
     @property

@@ -95,12 +95,13 @@ def bytes_per_cell(n_relax: int, itemsize: int) -> dict[str, int]:
     }
 
 
-def gpu_throughput(size, steps, warmup, dtype, compensated=False):
+def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True):
     import torch
 
     from ac_fdtd_amd.scheme import AcousticFDTD
 
-    solver = make_solver(AcousticFDTD, size, steps + warmup + 40, dtype, compensated=compensated)
+    solver = make_solver(AcousticFDTD, size, steps + warmup + 40, dtype, compensated=compensated,
+                         autotune=autotune)
     solver.run(warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -125,6 +126,7 @@ def gpu_throughput(size, steps, warmup, dtype, compensated=False):
         "cells": cells,
         "recorded": recorded.astype(np.float64),
         "sample_rate": solver.sample_rate,
+        "tiling": [solver.tile_j, solver.chunk_x],
     }
 
 
@@ -278,6 +280,7 @@ def gradient_cost(size, steps, seed=11):
             receivers=points[1:], dtype=np.float32,
             walls=WallAdmittances.from_absorption(0.2),
             air_absorption=AirAbsorption(20.0, 50.0) if air else None, checkpoint_every=100,
+            autotune=True,
         )
 
     def seconds(call):
@@ -405,6 +408,7 @@ def main() -> int:
     achieved = sweep_bytes * f32["cells"] / (f32["kernels"][dominant] * 1e-3) / 1e9
 
     extra = {
+        "tiling_tile_j_chunk_x": f32["tiling"],
         "kernel_ms": {k: round(v, 4) for k, v in f32["kernels"].items()},
         "bytes_per_cell_update": {k: per_cell[k] for k in (*f32["kernels"], "two_sweep_step")},
         # Gcell/s against the two-sweep byte count (12 + 2 n_relax elements per cell-update),
