@@ -177,6 +177,8 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_run_states.argtypes = [
         _c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr, _ptr, _ptr, _ptr, _ptr
     ]
+    lib.acfdtd_hip_stream_copy.restype = _c.c_int
+    lib.acfdtd_hip_stream_copy.argtypes = [_ptr, _ptr, _c.c_size_t, _ptr]
     lib.acfdtd_hip_energy_workspace_bytes.restype = _c.c_size_t
     lib.acfdtd_hip_energy_workspace_bytes.argtypes = [_c.POINTER(Desc)]
     lib.acfdtd_hip_energy.restype = _c.c_int

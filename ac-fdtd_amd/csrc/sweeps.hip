@@ -328,3 +328,27 @@ extern "C" int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, d
     if (desc->real_bytes == 8) return energy_terms<double>(*desc, workspace, out, s);
     return energy_terms<float>(*desc, workspace, out, s);
 }
+
+namespace acfdtd {
+
+__global__ __launch_bounds__(256) void stream_copy(const float4 *__restrict__ src,
+                                                   float4 *__restrict__ dst, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+}  // namespace acfdtd
+
+extern "C" int acfdtd_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream) {
+    if (!src || !dst) return fail(ACFDTD_HIP_EINVAL, "stream copy pointer is NULL");
+    if (bytes % 16 || reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(dst) % 16)
+        return fail(ACFDTD_HIP_EINVAL, "stream copy needs 16-byte sizes and alignment");
+    const size_t n = bytes / 16;
+    if (n == 0) return ACFDTD_HIP_OK;
+    if ((n + 255) / 256 > 0x7fffffffu) return fail(ACFDTD_HIP_EINVAL, "stream copy too large");
+    g_last_error.clear();
+    hipLaunchKernelGGL(stream_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const float4 *>(src),
+                       static_cast<float4 *>(dst), n);
+    return check_hip(hipGetLastError(), "stream copy launch");
+}

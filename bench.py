@@ -180,7 +180,7 @@ def rigid_energy_drift(size, steps, dtype=np.float64):
     return drift
 
 
-def batched_rooms(n_rooms, steps, warmup, check=True, seed=7):
+def batched_rooms(n_rooms, steps, warmup, check=True, seed=3):
     """Rooms of sides U[3, 10] m at 1 kHz (the spacing ``Grid.from_max_frequency`` gives:
     c / (10 x 1 kHz) = 34.3 mm), six per-wall absorptions U[0.05, 0.5], air at T U[15, 25] C and
     RH U[30, 70] %, one source and 4 receivers at least 0.5 m from every wall, the repo's
@@ -251,7 +251,7 @@ def batched_rooms(n_rooms, steps, warmup, check=True, seed=7):
     return out
 
 
-def gradient_cost(size, steps, seed=11):
+def gradient_cost(size, steps, seed=4):
     """Wall-admittance gradient of an L2 receiver loss at ``size``^3, fp32: a hidden target set
     of six absorptions U[0.05, 0.5], the model at uniform 0.2, 4 receivers, the band-limited
     impulse at 1 kHz on a 34.3 mm grid. Returns seconds for a forward pass and for forward +
@@ -267,8 +267,8 @@ def gradient_cost(size, steps, seed=11):
     dx = AIR.sound_speed / (10.0 * 1000.0)
     grid = Grid(dx=dx, nx=size, ny=size, nz=size)
     lengths = np.array(grid.lengths)
-    points = [tuple((0.15 + 0.7 * rng.uniform(0.0, 1.0, 3)) * lengths) for _ in range(5)]
     hidden = [admittance_from_absorption(a) for a in rng.uniform(0.05, 0.5, 6)]
+    points = [tuple((0.15 + 0.7 * rng.uniform(0.0, 1.0, 3)) * lengths) for _ in range(5)]
     start = admittance_from_absorption(0.2)
 
     def model(air):
@@ -322,6 +322,169 @@ def gradient_cost(size, steps, seed=11):
     return out
 
 
+LARGE_OCTAVES = (250.0, 500.0, 1000.0, 2000.0, 4000.0)
+
+
+def large_room(size, steps, seed=5):
+    """``size``^3 at dx = 1 cm: the 16-cell sponge on the two x faces (rigid behind it), walls
+    of absorption 0.1 on the four others, air at 20 C / 30 % RH, a seeded Gaussian-derivative
+    pulse, 32 seeded receivers, ``steps`` steps. fp32 through the differentiable room (boundary
+    tape attached, so the forward pass is the gradient's) and its wall-admittance gradient,
+    then fp64 for the T20 deviation per octave band."""
+    import torch
+
+    from ac_fdtd_amd import (AIR, AbsorbingLayer, AirAbsorption, Grid, WallAdmittances,
+                             admittance_from_absorption)
+    from ac_fdtd_amd.autograd import RoomResponse
+    from ac_fdtd_amd.grid import max_stable_time_step
+    from ac_fdtd_amd.metrics import octave_band_reverberation
+    from ac_fdtd_amd.scheme import AcousticFDTD
+    from ac_fdtd_amd.sources import gaussian_derivative_pulse
+
+    grid = Grid(dx=DX, nx=size, ny=size, nz=size)
+    rng = np.random.default_rng(seed)
+    width = rng.uniform(4.0, 8.0) * DX / AIR.sound_speed
+
+    def point():
+        return tuple((rng.integers(MARGIN, size - MARGIN, size=3) + 0.5) * DX)
+
+    source = point()
+    receivers = [point() for _ in range(32)]
+    xi = admittance_from_absorption(0.1)
+    config = {
+        "walls": WallAdmittances(0.0, 0.0, xi, xi, xi, xi),
+        "absorbing_layer": AbsorbingLayer(thickness=SPONGE, faces=((0, 0), (0, 1))),
+        "air_absorption": AirAbsorption(temperature=20.0, relative_humidity=30.0),
+    }
+    dt = max_stable_time_step(DX, AIR.sound_speed)
+    signal = gaussian_derivative_pulse(steps, dt, width)
+
+    def seconds(call):
+        torch.cuda.synchronize()
+        began = time.perf_counter()
+        out = call()
+        torch.cuda.synchronize()
+        return time.perf_counter() - began, out
+
+    room = RoomResponse(grid, steps, [(source, signal)], receivers, dtype=np.float32,
+                        autotune=True, **config)
+    # Tune the forward and adjoint tiles now, outside the timed passes.
+    from ac_fdtd_amd.autotune import adjoint_tiling
+
+    adjoint_tiling(room.solver(wall_table_always=True))
+    torch.cuda.empty_cache()
+    admittance = room.default_admittance(device="cuda").requires_grad_()
+    t_forward, recording = seconds(lambda: room(admittance))
+    weights = torch.as_tensor(np.random.default_rng(seed + 1).standard_normal(recording.shape),
+                              dtype=recording.dtype, device=recording.device)
+    t_backward, (gradient,) = seconds(lambda: torch.autograd.grad((recording * weights).sum(),
+                                                                  [admittance]))
+    report = room.report
+    f32 = recording.detach().double().cpu().numpy()
+    del recording, weights
+    torch.cuda.empty_cache()
+
+    solver = AcousticFDTD(grid, dtype=np.float64, autotune=True, **config)
+    solver.add_volume_source(source, signal)
+    for r in receivers:
+        solver.add_pressure_receiver(r)
+    t_f64, _ = seconds(lambda: solver.run(steps))
+    f64 = solver.recorded_pressure
+    del solver
+    torch.cuda.empty_cache()
+
+    rate = 1.0 / dt
+    deviation, reference = {}, {}
+    for channel in range(f64.shape[0]):
+        t_a = octave_band_reverberation(f32[channel], rate, LARGE_OCTAVES)
+        t_b = octave_band_reverberation(f64[channel], rate, LARGE_OCTAVES)
+        for band in LARGE_OCTAVES:
+            if np.isfinite(t_a[band]) and np.isfinite(t_b[band]):
+                deviation[band] = max(deviation.get(band, 0.0), abs(t_a[band] / t_b[band] - 1.0))
+                reference.setdefault(band, []).append(t_b[band])
+    cells = grid.n_cells
+    return {
+        "grid": [size] * 3,
+        "steps": steps,
+        "receivers": len(receivers),
+        "fp32_forward_with_tape_gcells": round(cells * steps / t_forward / 1e9, 3),
+        "fp64_forward_gcells": round(cells * steps / t_f64 / 1e9, 3),
+        "wall_gradient_backward_s": round(t_backward, 3),
+        "wall_gradient_over_forward": round((t_forward + t_backward) / t_forward, 3),
+        "wall_gradient": [float(g) for g in gradient.cpu()],
+        "hbm_gradient_storage_bytes": report.tape_bytes + report.checkpoint_bytes,
+        "receiver_rel_l2_fp32_vs_fp64": relative_l2(f32, f64),
+        "t20_fp64_median_s_by_octave": {f"{int(b)}": float(np.median(v))
+                                        for b, v in reference.items()},
+        "t20_max_rel_deviation_by_octave": {f"{int(b)}": float(v) for b, v in deviation.items()},
+    }
+
+def stream_ceiling(gib: float = 1.0, copies: int = 20) -> float:
+    """GB/s (read + write) of the library's 16-byte-per-lane copy over ``gib`` GiB, four times
+    the 256 MB MALL: the HBM bandwidth a plain streaming kernel reaches here."""
+    import ctypes
+
+    import torch
+
+    from ac_fdtd_amd import _native
+
+    lib = _native.library()
+    n_bytes = int(gib * 2**30)
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    handle = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def copy():
+        _native.check(lib.acfdtd_hip_stream_copy(ctypes.c_void_p(src.data_ptr()),
+                                                 ctypes.c_void_p(dst.data_ptr()), n_bytes,
+                                                 handle), "acfdtd_hip_stream_copy")
+
+    for _ in range(3):
+        copy()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(copies):
+        copy()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / copies
+    return 2.0 * n_bytes / (ms * 1e-3) / 1e9
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as info:
+            for line in info:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def numpy_baseline(size, seconds):
+    """The oracle's NumPy restatement of the reference loop (one thread, as NumPy runs), fp32,
+    same workload, bounded sample."""
+    from oracle.scheme import OracleFDTD
+
+    solver = make_solver(OracleFDTD, size, 4000, np.float32)
+    solver.run(1)
+    started = time.perf_counter()
+    solver.run(1)
+    per_step = time.perf_counter() - started
+    steps = max(2, min(200, int(seconds / per_step)))
+    started = time.perf_counter()
+    solver.run(steps)
+    elapsed = time.perf_counter() - started
+    return {
+        "value": round(solver.grid.n_cells * steps / elapsed / 1e9, 5),
+        "unit": "Gcell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy restatement of the reference step (oracle/scheme.py), fp32, same "
+        f"{size}^3 workload, {steps} steps after 2 warm-up steps, {elapsed:.1f} s",
+    }
+
 def cpu_baseline(size, seconds):
     """The reference C loop (oracle/_ref) on the host, fp32, same workload, bounded sample."""
     from oracle import ref_c
@@ -347,7 +510,7 @@ def cpu_baseline(size, seconds):
         "kind": "reference",
         "sample": f"reference C loop (acfdtd.c, fp32, -O3 -ffp-contract=off, pthreads) on the "
         f"same {size}^3 walls+sponge+air workload, {steps} steps after 4 warm-up steps, "
-        f"{elapsed:.1f} s",
+        f"{elapsed:.1f} s, on {cpu_model()}",
     }
 
 
@@ -361,6 +524,8 @@ def main() -> int:
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-extra", action="store_true")
     parser.add_argument("--batch-rooms", type=int, default=64)
+    parser.add_argument("--large-steps", type=int, default=8000,
+                        help="steps of the 512^3 room (0 skips it)")
     parser.add_argument("--batch-steps", type=int, default=3000)
     parser.add_argument("--skip-check", action="store_true",
                         help="skip the fp64 single-room runs that check the batched rooms")
@@ -449,11 +614,22 @@ def main() -> int:
         extra["wall_gradient_cost_fp32"] = gradient_cost(128, 2000)
         log(f"gradient: {extra['wall_gradient_cost_fp32']}")
         torch.cuda.empty_cache()
+        if args.large_steps > 0:
+            log(f"large room 512^3, {args.large_steps} steps")
+            extra["large_room_512"] = large_room(512, args.large_steps)
+            log(f"large room: {extra['large_room_512']}")
+
+    log("stream ceiling")
+    ceiling = stream_ceiling()
+    extra["stream_copy_ceiling_gbs"] = round(ceiling, 1)
+    extra["fused_fraction_of_stream_ceiling"] = round(achieved / ceiling, 4)
 
     baseline = None
     if not args.skip_cpu:
         log("cpu baseline")
         baseline = cpu_baseline(args.size, args.cpu_seconds)
+        log("numpy baseline")
+        extra["numpy_baseline"] = numpy_baseline(args.size, args.cpu_seconds)
 
     line = {
         "metric": METRIC,

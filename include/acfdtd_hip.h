@@ -177,6 +177,12 @@ int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
 size_t acfdtd_hip_energy_workspace_bytes(const acfdtd_hip_desc *desc);
 int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out, void *stream);
 
+/* Stream ceiling for the roofline: copy `bytes` (a multiple of 16, both pointers 16-byte
+ * aligned) from src to dst, one 16-byte vector per lane. Measures the HBM bandwidth a plain
+ * streaming kernel reaches on this device; no reference counterpart (the reference times a
+ * NumPy add, benchmarks/backend_benchmark.py:86-115). */
+int acfdtd_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
+
 /* ------------------------------------------------------------------------------ adjoint --
  * The transposed, time-reversed step and the parameter gradients it yields (no reference
  * counterpart: the reference is not differentiable). An adjoint step n is a face sweep
