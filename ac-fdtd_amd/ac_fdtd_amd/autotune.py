@@ -67,7 +67,10 @@ def _store(key: str, value: tuple[int, int]) -> None:
 
 def _key(kind: str, solver) -> str:
     g, k = solver.grid, solver.coefficients
-    name = torch.cuda.get_device_name(solver.device).replace(" ", "_")
+    props = torch.cuda.get_device_properties(solver.device)
+    # The architecture and CU count identify the device type; the marketing name is not
+    # reliable (it comes from a table some installations lack).
+    name = f"{getattr(props, 'gcnArchName', 'gpu').split(':')[0]}x{props.multi_processor_count}"
     return (f"{kind}:{name}:{g.nx}x{g.ny}x{g.nz}:f{solver.np_dtype.itemsize * 8}:r{k.n_relax}"
             f":w{int(solver._wall_mask != 0)}:l{int(k.has_layer)}:c{int(solver.compensated)}")
 

@@ -95,6 +95,20 @@ def bytes_per_cell(n_relax: int, itemsize: int) -> dict[str, int]:
     }
 
 
+PROFILE = ROOT / "profiles" / "fused_step_256cubed_fp32.json"
+
+
+def measured_traffic(size: int, kernel: str):
+    """HBM bytes per launch of the fused step measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a copy of known size) for this workload, from the committed
+    profile; None for another grid or kernel."""
+    if kernel != "fused" or not PROFILE.exists():
+        return None
+    record = json.loads(PROFILE.read_text())
+    if record.get("grid") != [size] * 3:
+        return None
+    return record["traffic_bytes_per_launch"]
+
 def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True):
     import torch
 
@@ -531,10 +545,11 @@ def main() -> int:
                         help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
         "--mode",
-        choices=("bench", "copy"),
+        choices=("bench", "copy", "gradient"),
         default="bench",
         help="copy: five plain 512 MiB device-to-device copies, the known-size reference a "
-        "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by",
+        "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by; gradient: only the "
+        "wall-gradient cost at 128^3 (for profiling the adjoint kernels)",
     )
     args = parser.parse_args()
 
@@ -562,6 +577,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": source.numel() * 4, "copies": 5}), flush=True)
         return 0
+    if args.mode == "gradient":
+        print(json.dumps(gradient_cost(128, 2000)), flush=True)
+        return 0
 
     log(f"fp32 {args.size}^3, {args.steps} steps")
     f32 = gpu_throughput(args.size, args.steps, args.warmup, np.float32)
@@ -571,6 +589,7 @@ def main() -> int:
     dominant = max(f32["kernels"], key=f32["kernels"].get)
     sweep_bytes = per_cell[dominant]
     achieved = sweep_bytes * f32["cells"] / (f32["kernels"][dominant] * 1e-3) / 1e9
+    traffic = measured_traffic(args.size, dominant)
 
     extra = {
         "tiling_tile_j_chunk_x": f32["tiling"],
@@ -656,7 +675,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": dominant,
             "algorithmic_bytes_per_launch": sweep_bytes * f32["cells"],
         },
