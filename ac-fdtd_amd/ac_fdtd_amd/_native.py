@@ -56,7 +56,7 @@ class Desc(_c.Structure):
         ("flags", _c.c_uint32),
         ("tile_j", _c.c_int32),
         ("chunk_x", _c.c_int32),
-        ("reserved", _c.c_int32),
+        ("time_block", _c.c_int32),
         ("vel_coef", _c.c_double),
         ("pres_coef", _c.c_double),
         ("src_coef", _c.c_double),
@@ -88,13 +88,14 @@ class Desc(_c.Structure):
         ("wall_tape_stride", _c.c_int64),
         ("wall_tape_step0", _c.c_int64),
         ("wall_tape_rows", _c.c_int64),
+        ("psi_swap", _ptr),
     ]
 
 
 GRAD_WALLS = 1
 GRAD_SPONGE = 2
 GRAD_AIR = 4
-GRAD_SLOTS = 12  # per-workgroup gradient accumulators: 6 walls, sponge, scale, 4 gains
+GRAD_SLOTS = 12  # per-wavefront gradient accumulators: 6 walls, sponge, scale, 4 gains
 
 
 class AdjointDesc(_c.Structure):

@@ -37,6 +37,8 @@ SOURCES = (
     "adjoint.hip",
     "adjoint_fused.hip",
     "adjoint_fused64.hip",
+    "fused2.hip",
+    "fused2_64.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

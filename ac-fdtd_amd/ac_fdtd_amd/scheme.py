@@ -72,6 +72,9 @@ class AcousticFDTD:
         autotune: time the candidate tilings of the fused step on scratch fields of this
             configuration and keep the fastest (overrides ``tile_j`` / ``chunk_x``; see
             :mod:`ac_fdtd_amd.autotune`).
+        time_block: steps per sweep over memory, 1 or 2. With 2 (fused path only) the run
+            advances two steps per sweep wherever it can (a second set of relaxation states
+            is allocated for it); results are bit-identical to single steps.
     """
 
     def __init__(
@@ -92,7 +95,10 @@ class AcousticFDTD:
         relaxation_strengths=None,
         wall_table_always: bool = False,
         autotune: bool = False,
+        time_block: int = 1,
     ) -> None:
+        if time_block not in (1, 2):
+            raise ValueError(f"time_block must be 1 or 2, got {time_block}")
         self.device = _require_gpu(device)
         self._lib = _native.library()
         self.grid = grid
@@ -128,11 +134,15 @@ class AcousticFDTD:
         k = self.coefficients
         self.relaxation_states = torch.zeros((k.n_relax, *grid.shape), **opts)
         self.fused = fused
+        self.time_block = time_block if fused else 1
         self._swap = None
+        self._psi_swap = None
         if fused:
             self._swap = [torch.zeros(grid.shape, **opts)] + [
                 torch.zeros(grid.velocity_shape(a), **opts) for a in range(3)
             ]
+            if self.time_block == 2 and k.n_relax:
+                self._psi_swap = torch.zeros((k.n_relax, *grid.shape), **opts)
 
         self._wall_table = None
         self._wall_mask = 0
@@ -401,6 +411,7 @@ class AcousticFDTD:
         d.has_layer = int(k.has_layer)
         d.flags = _native.COMPENSATED if self.compensated else 0
         d.tile_j, d.chunk_x = self.tile_j, self.chunk_x
+        d.time_block = self.time_block
         d.vel_coef, d.pres_coef, d.src_coef = k.velocity, k.pressure, k.source
         d.relax_scale = k.relax_scale
         for r in range(k.n_relax):
@@ -444,4 +455,6 @@ class AcousticFDTD:
             d.p_swap = self._checked(self._swap[0], g.shape, "p swap")
             for a, name in enumerate(("vx_swap", "vy_swap", "vz_swap")):
                 setattr(d, name, self._checked(self._swap[a + 1], g.velocity_shape(a), name))
+        if self._psi_swap is not None:
+            d.psi_swap = self._checked(self._psi_swap, (k.n_relax, *g.shape), "psi swap")
         return d

@@ -1,0 +1,10 @@
+// fp32 instantiation of the two-step sweep (templates in fused2.hpp and marches2.hpp).
+
+#include "fused2.hpp"
+
+namespace acfdtd {
+
+template int64_t launch_fused2_quads<float>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
+                                            hipStream_t);
+
+}  // namespace acfdtd

@@ -1,0 +1,117 @@
+// The two-step sweep: kernel, dispatch, launch of the leading multiple of four steps. Templates
+// only; fused2.hip and fused2_64.hip instantiate them for fp32 (4 cells per lane) and fp64
+// (2 cells per lane).
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "acfdtd_hip.h"
+#include "marches.hpp"
+#include "marches2.hpp"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__global__ __launch_bounds__(kFused2Threads) void fused2_step(const SweepArgs<real> a,
+                                                             const Fused2IO<real> io,
+                                                             const Tiling2 t2, const int64_t step,
+                                                             const int64_t column) {
+    __shared__ real lds[3 * kFused2Threads * V];
+    const unsigned n = (unsigned)t2.launched;
+    unsigned b = blockIdx.x;
+    b = (b & 7u) * (n >> 3) + (b >> 3);  // contiguous tiles per XCD, as decode_tile
+    if (b >= (unsigned)t2.tiles) return;  // whole workgroup, before any barrier
+    const int jb = (int)(b % (unsigned)t2.nblk);
+    const int xc = (int)(b / (unsigned)t2.nblk);
+    const int t = (int)threadIdx.x;
+    const int64_t f = (int64_t)jb * t2.main - t2.groups + t;
+    const bool valid = t < t2.main + 2 * t2.groups && f >= 0 && f < (int64_t)a.ny * t2.groups;
+    const bool owner = valid && t >= t2.groups && t < t2.groups + t2.main;
+    const int j = valid ? (int)(f / t2.groups) : 0;
+    const int k0 = valid ? (int)(f - (int64_t)j * t2.groups) * V : 0;
+    const int i0 = xc * t2.chunk;
+    const int i1 = min(i0 + t2.chunk, a.nx);
+    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED>(
+        a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
+        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds);
+}
+
+template <typename real>
+using Fused2Kernel = void (*)(const SweepArgs<real>, const Fused2IO<real>, const Tiling2,
+                              const int64_t, const int64_t);
+
+// Lane width of the two-step sweep: 16-byte lanes (4 cells fp32, 2 fp64).
+template <typename real>
+constexpr int fused2_vec() {
+    return sizeof(real) == 4 ? 4 : 2;
+}
+
+template <typename real, int NR>
+Fused2Kernel<real> pick_fused2_nr(bool walls, bool layer, bool comp) {
+    constexpr int V = fused2_vec<real>();
+    if constexpr (sizeof(real) == 4) {
+        if (comp) {
+            if (walls) return layer ? fused2_step<real, V, NR, true, true, true>
+                                    : fused2_step<real, V, NR, true, false, true>;
+            return layer ? fused2_step<real, V, NR, false, true, true>
+                         : fused2_step<real, V, NR, false, false, true>;
+        }
+    }
+    (void)comp;
+    if (walls) return layer ? fused2_step<real, V, NR, true, true, false>
+                            : fused2_step<real, V, NR, true, false, false>;
+    return layer ? fused2_step<real, V, NR, false, true, false>
+                 : fused2_step<real, V, NR, false, false, false>;
+}
+
+template <typename real>
+Fused2Kernel<real> pick_fused2(int nr, bool walls, bool layer, bool comp) {
+    switch (nr) {
+        case 0: return pick_fused2_nr<real, 0>(walls, layer, comp);
+        case 1: return pick_fused2_nr<real, 1>(walls, layer, comp);
+        case 2: return pick_fused2_nr<real, 2>(walls, layer, comp);
+        case 3: return pick_fused2_nr<real, 3>(walls, layer, comp);
+        default: return pick_fused2_nr<real, 4>(walls, layer, comp);
+    }
+}
+
+template <typename real>
+int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
+                            int64_t column, hipStream_t stream) {
+    Tiling2 t2;
+    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2)) return 0;
+    SweepArgs<real> a;
+    fill_args(d, a);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const Fused2Kernel<real> k = pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    Fused2IO<real> forward{}, backward{};
+    forward.p_in = backward.p_out = static_cast<real *>(d.p);
+    forward.vx_in = backward.vx_out = static_cast<real *>(d.vx);
+    forward.vy_in = backward.vy_out = static_cast<real *>(d.vy);
+    forward.vz_in = backward.vz_out = static_cast<real *>(d.vz);
+    forward.psi_in = backward.psi_out = static_cast<real *>(d.psi);
+    forward.p_out = static_cast<real *>(d.p_swap);
+    forward.vx_out = static_cast<real *>(d.vx_swap);
+    forward.vy_out = static_cast<real *>(d.vy_swap);
+    forward.vz_out = static_cast<real *>(d.vz_swap);
+    forward.psi_out = static_cast<real *>(d.psi_swap);
+    backward.p_in = forward.p_out;
+    backward.vx_in = forward.vx_out;
+    backward.vy_in = forward.vy_out;
+    backward.vz_in = forward.vz_out;
+    backward.psi_in = forward.psi_out;
+    const dim3 grid(t2.launched), block(t2.threads);
+    int64_t n = 0;
+    for (; n + 3 < n_steps; n += 4) {
+        hipLaunchKernelGGL(k, grid, block, 0, stream, a, forward, t2, step_offset + n, column + n);
+        hipLaunchKernelGGL(k, grid, block, 0, stream, a, backward, t2, step_offset + n + 2,
+                           column + n + 2);
+    }
+    return n;
+}
+
+}  // namespace acfdtd

@@ -684,4 +684,11 @@ template <typename real>
 int64_t launch_fused_pairs(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
                            int64_t column, hipStream_t stream);
 
+// Launch two-step sweeps for the leading multiple of four of n_steps (pairs A -> B -> A of two
+// steps each) when the descriptor asks for them and allows them; returns the number of steps
+// launched (0 otherwise). Defined in fused2.hpp, instantiated in fused2.hip and fused2_64.hip.
+template <typename real>
+int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
+                            int64_t column, hipStream_t stream);
+
 }  // namespace acfdtd

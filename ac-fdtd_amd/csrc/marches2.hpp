@@ -1,0 +1,441 @@
+// Two steps per sweep (temporal blocking): the body of the fused2 kernels.
+//
+// One sweep reads (p^n, v^(n-1/2), psi^n) from the `in` set and writes
+// (p^(n+2), v^(n+3/2), psi^(n+2)) to the `out` set, so each field crosses HBM once per two
+// steps. A workgroup owns `main` consecutive lane groups of the flattened (y, z / V) plane (V
+// z cells each, as in fused_march_vec) and marches a chunk of x planes. Along x it runs the
+// first step one plane ahead of the second: step 1 of plane i + 1, then step 2 of plane i,
+// which needs p^(n+1) of planes i - 1, i, i + 1 (registers of the same lane) and of its four
+// y and z neighbours in plane i (other lanes: exchanged through LDS). Its y neighbours lie a
+// row (nz / V lane groups) away, so the workgroup also runs step 1, without storing anything,
+// on one row of halo lanes on each side, and on one halo plane at each end of its chunk.
+//
+// Every cell's arithmetic is that of fused_march_vec term by term, and a halo cell's first
+// step reads the same unmodified inputs as its owner's, so p^(n+1), v^(n+1/2) and psi^(n+1)
+// are the same bits in every lane that computes them, and the sweep is bit-identical to two
+// single-step sweeps. The relaxation states are read by halo lanes too, so unlike the
+// single-step sweep they ping-pong with the other fields.
+//
+// LDS holds p^(n+1) of three planes (written for plane i + 1 while plane i is read, and one
+// more so that one barrier per plane suffices): 3 x 512 lanes x V values, 24 KiB.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "acfdtd_hip.h"
+#include "marches.hpp"
+#include "sweeps.hpp"
+
+namespace acfdtd {
+
+// Largest workgroup of the two-step sweep (launch bound and LDS size).
+constexpr int kFused2Threads = 512;
+
+template <typename real>
+struct Fused2IO {
+    const real *p_in, *vx_in, *vy_in, *vz_in, *psi_in;
+    real *p_out, *vx_out, *vy_out, *vz_out, *psi_out;
+};
+
+// Workgroup shape of the two-step sweep.
+struct Tiling2 {
+    int vec;      // z cells per lane (V)
+    int groups;   // lane groups per row (nz / V), also the halo on each side
+    int main;     // lane groups a workgroup advances (64 * rows)
+    int threads;  // main + 2 * groups, rounded up to whole wavefronts
+    int chunk;    // x planes per march
+    int nblk, nchunk, tiles, launched;
+};
+
+// Whether acfdtd_hip_run may use the two-step sweep for this descriptor.
+inline bool fused2_applies(const acfdtd_hip_desc &d) {
+    return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr) &&
+           d.wall_tape == nullptr;
+}
+
+// Shape of the two-step sweep: tile_j wavefronts of own lanes (fewer if the halo rows would
+// push the workgroup past kFused2Threads), chunk_x planes per march (default 8). False when
+// nz is not a multiple of the lane width or one row of halo on each side does not fit.
+inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
+    if (d.nz % vec != 0) return false;
+    t.vec = vec;
+    t.groups = d.nz / vec;
+    int rows = d.tile_j > 0 ? d.tile_j : 4;
+    while (rows > 1 && 64 * rows + 2 * t.groups > kFused2Threads) --rows;
+    t.main = 64 * rows;
+    if (t.main + 2 * t.groups > kFused2Threads) return false;
+    t.threads = ceil_div(t.main + 2 * t.groups, 64) * 64;
+    t.chunk = d.chunk_x > 0 ? d.chunk_x : 8;
+    t.nblk = ceil_div((int64_t)d.ny * t.groups, t.main);
+    t.nchunk = ceil_div(d.nx, t.chunk);
+    const int64_t tiles = (int64_t)t.nblk * t.nchunk;
+    if (tiles > (int64_t)INT32_MAX - 8) return false;
+    t.tiles = (int)tiles;
+    t.launched = ceil_div(tiles, 8) * 8;
+    return true;
+}
+
+// Per-lane constants of the step: position, neighbour existence, sponge factors, wall classes.
+template <typename real, int V>
+struct LaneConsts {
+    int j, k0, jk0;
+    bool has_ylo, has_yhi, has_zlo, has_zhi, yz_boundary, special;
+    int64_t dy_lo, dy_hi, dz_lo, dz_hi;
+    real fyc, fyf_lo, fyf_hi, fzc[V], fzf[V + 1];
+    int code_yz[V];
+    bool wall_in[V];
+    real fu_in[V], fp_in[V];
+};
+
+template <typename real, int V, bool WALLS, bool LAYER>
+__device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int j, const int k0,
+                                            LaneConsts<real, V> &c) {
+    const int ny = a.ny, nz = a.nz;
+    c.j = j;
+    c.k0 = k0;
+    c.jk0 = j * nz + k0;
+    c.has_ylo = j > 0;
+    c.has_yhi = j + 1 < ny;
+    c.has_zlo = k0 > 0;
+    c.has_zhi = k0 + V < nz;
+    c.dy_lo = c.has_ylo ? nz : 0;
+    c.dy_hi = c.has_yhi ? nz : 0;
+    c.dz_lo = c.has_zlo ? 1 : 0;
+    c.dz_hi = c.has_zhi ? V : V - 1;
+    c.fyc = 1;
+    c.fyf_lo = 1;
+    c.fyf_hi = 1;
+#pragma unroll
+    for (int e = 0; e < V; ++e) c.fzc[e] = 1;
+#pragma unroll
+    for (int e = 0; e <= V; ++e) c.fzf[e] = 1;
+    if (LAYER) {
+        c.fyc = a.cell[1][j];
+        c.fyf_lo = a.face[1][j];
+        c.fyf_hi = a.face[1][j + 1];
+#pragma unroll
+        for (int e = 0; e < V; ++e) c.fzc[e] = a.cell[2][k0 + e];
+#pragma unroll
+        for (int e = 0; e <= V; ++e) c.fzf[e] = a.face[2][k0 + e];
+    }
+    c.yz_boundary = !c.has_ylo || !c.has_yhi || !c.has_zlo || !c.has_zhi;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        c.code_yz[e] = (wall_class(j, ny) << 2) | (wall_class(k0 + e, nz) << 4);
+        c.wall_in[e] = WALLS && ((a.wall_mask >> c.code_yz[e]) & 1ull);
+        c.fu_in[e] = 1;
+        c.fp_in[e] = 0;
+    }
+    if (WALLS && c.yz_boundary) {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            if (c.wall_in[e]) {
+                c.fu_in[e] = a.wall_table[c.code_yz[e]];
+                c.fp_in[e] = a.wall_table[ACFDTD_HIP_WALL_CODES + c.code_yz[e]];
+            }
+    }
+    bool special = false;
+    for (int s = 0; s < a.n_sources; ++s) {
+        const int e = a.source_cell[2 * s + 1] - c.jk0;
+        special = special || (e >= 0 && e < V);
+    }
+    for (int r = 0; r < a.n_receivers; ++r) {
+        const int e = a.receiver_cell[2 * r + 1] - c.jk0;
+        special = special || (e >= 0 && e < V);
+    }
+    c.special = special;
+}
+
+// What one step leaves in a lane for one plane: the new pressure, the new velocity on the
+// cell faces the next step needs again (upper x, both y, all z), the new relaxation states.
+template <typename real, int V, int NR>
+struct PlaneState {
+    real p[V], vx_hi[V], vy_lo[V], vy_hi[V], vz[V + 1];
+    real psi[NR > 0 ? NR : 1][V];
+};
+
+// One step of the lane's V cells in plane i, the arithmetic of fused_march_vec's plane body:
+// from the pressures p_c (plane i), p_up (plane i + 1), q_lo / q_hi (rows j -/+ 1), pz_lo /
+// pz_hi (cells k0 - 1 and k0 + V), the new velocity on the lower x face, the old velocities on
+// the other faces and the relaxation states (advanced in place), produce the new velocities,
+// the new pressure after walls, sponge and sources of `step`.
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool XWALL>
+__device__ __forceinline__ void advance_cells(
+    const SweepArgs<real> &a, const LaneConsts<real, V> &c, const int i, const int64_t step,
+    const real (&p_c)[V], const real (&p_up)[V], const real (&q_lo)[V], const real (&q_hi)[V],
+    const real pz_lo, const real pz_hi, const real (&vx_lo)[V], const real (&vx_v)[V],
+    const real (&vy_lo_in)[V], const real (&vy_hi_in)[V], const real (&vz_v)[V],
+    const real vz_top, real (&states)[NR > 0 ? NR : 1][V], PlaneState<real, V, NR> &o) {
+    const int nx = a.nx;
+    const real vel = a.vel;
+    const bool up_interior = !XWALL || i + 1 < nx;
+    const real fxc = LAYER ? a.cell[0][i] : real(1);
+    const real fxf_hi = LAYER ? a.face[0][i + 1] : real(1);
+
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const real wx = up_interior ? vx_v[e] - vel * (p_up[e] - p_c[e]) : vx_v[e];
+        o.vx_hi[e] = damp3<real, LAYER>(wx, fxf_hi, c.fyc, c.fzc[e]);
+        const real wl = c.has_ylo ? vy_lo_in[e] - vel * (p_c[e] - q_lo[e]) : vy_lo_in[e];
+        o.vy_lo[e] = damp3<real, LAYER>(wl, fxc, c.fyf_lo, c.fzc[e]);
+        const real wh = c.has_yhi ? vy_hi_in[e] - vel * (q_hi[e] - p_c[e]) : vy_hi_in[e];
+        o.vy_hi[e] = damp3<real, LAYER>(wh, fxc, c.fyf_hi, c.fzc[e]);
+        real wz;
+        if (e > 0) wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
+        else wz = c.has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+        o.vz[e] = damp3<real, LAYER>(wz, fxc, c.fyc, c.fzf[e]);
+    }
+    {
+        const real wz = c.has_zhi ? vz_top - vel * (pz_hi - p_c[V - 1]) : vz_top;
+        o.vz[V] = damp3<real, LAYER>(wz, fxc, c.fyc, c.fzf[V]);
+    }
+
+    const int cx = XWALL ? wall_class(i, nx) : 0;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const real div =
+            ((o.vx_hi[e] - vx_lo[e]) + (o.vy_hi[e] - o.vy_lo[e])) + (o.vz[e + 1] - o.vz[e]);
+        const real p_old = p_c[e];
+        real pn;
+        if (COMPENSATED) {
+            real s = p_old, err = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const real target = div * a.gain[r];
+                const real offset = states[r][e] - target;
+                states[r][e] = offset * a.decay[r] + target;
+                two_sum_acc(s, offset * a.mean[r], err);
+            }
+            if (NR > 0) two_sum_acc(s, div * a.relax_scale, err);
+            const real prod = div * a.pres;
+            const real prod_err = fma_real(div, a.pres, -prod);
+            two_sum_acc(s, -prod, err);
+            pn = s + (err - prod_err);
+        } else {
+            pn = p_old;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const real target = div * a.gain[r];
+                const real offset = states[r][e] - target;
+                states[r][e] = offset * a.decay[r] + target;
+                pn = pn + offset * a.mean[r];
+            }
+            if (NR > 0) pn = pn + div * a.relax_scale;
+            pn = pn - div * a.pres;
+        }
+        if (WALLS) {
+            bool is_wall = c.wall_in[e];
+            real fu = c.fu_in[e], fp = c.fp_in[e];
+            if (XWALL) {
+                const int code = c.code_yz[e] | cx;
+                is_wall = (a.wall_mask >> code) & 1ull;
+                if (is_wall) {
+                    fu = a.wall_table[code];
+                    fp = a.wall_table[ACFDTD_HIP_WALL_CODES + code];
+                }
+            }
+            if (is_wall) pn = pn * fu - p_old * fp;
+        }
+        if (LAYER) pn = ((pn * fxc) * c.fyc) * c.fzc[e];
+        o.p[e] = pn;
+    }
+    if (c.special) {
+        // Sources in insertion order, after the sponge (scheme.py:360-363).
+        for (int s = 0; s < a.n_sources; ++s) {
+            const int e = a.source_cell[2 * s + 1] - c.jk0;
+            if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
+                const real add = a.src * a.source_signal[s * a.source_stride + step];
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+                    if (q == e) o.p[q] = o.p[q] + add;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) o.psi[r][e] = states[r][e];
+}
+
+template <typename real, int V>
+__device__ __forceinline__ void record_cells(const SweepArgs<real> &a, const LaneConsts<real, V> &c,
+                                             const int i, const int64_t column,
+                                             const real (&p)[V]) {
+    for (int r = 0; r < a.n_receivers; ++r) {
+        const int e = a.receiver_cell[2 * r + 1] - c.jk0;
+        if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                if (q == e) a.recording[r * a.recording_stride + column] = p[q];
+        }
+    }
+}
+
+// The march of one lane over planes [i0, i1). `valid`: the lane maps to cells of the grid;
+// `owner`: it is one of the workgroup's own lanes (not halo), which alone store and record.
+// Every thread of the workgroup runs the same loop and reaches every barrier.
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+__device__ __forceinline__ void fused2_march(
+    const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
+    const int k0, const bool valid, const bool owner, const int t, const int64_t step,
+    const int64_t column, const real *__restrict__ p_in, const real *__restrict__ vx_in,
+    const real *__restrict__ vy_in, const real *__restrict__ vz_in,
+    const real *__restrict__ psi_in, real *__restrict__ p_out, real *__restrict__ vx_out,
+    real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi_out,
+    real *__restrict__ lds) {
+    using Wall = std::integral_constant<bool, true>;
+    using Interior = std::integral_constant<bool, false>;
+    const int nx = a.nx, nz = a.nz;
+    const Layout L(nx, a.ny, nz, j, k0);
+    const int64_t n_cells = L.plane * nx;
+    const real vel = a.vel;
+    const int G = t2.groups, T = t2.threads;
+    LaneConsts<real, V> c;
+    if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, c);
+
+    auto slot = [&](const int plane, const int lane) { return lds + ((plane % 3) * T + lane) * V; };
+    auto share = [&](const int plane, const real (&p)[V]) {
+        real *d = slot(plane, t);
+        vstore<real, V>(d, p);
+    };
+
+    // First step: p^n of the plane being advanced, and the new lower x face velocity.
+    real pn_c[V], s1_vx_lo[V];
+    auto first_face = [&](const int s) {
+        real v[V], prev[V];
+        vload<real, V>(p_in + L.cell(s), pn_c);
+        vload<real, V>(vx_in + L.cell(s), v);
+        vload<real, V>(p_in + L.cell(s > 0 ? s - 1 : s), prev);
+        const real fxf = LAYER ? a.face[0][s] : real(1);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const real w = s > 0 ? v[e] - vel * (pn_c[e] - prev[e]) : v[e];
+            s1_vx_lo[e] = damp3<real, LAYER>(w, fxf, c.fyc, c.fzc[e]);
+        }
+    };
+    // Step 1 of plane s from global memory into o (p^n of plane s in pn_c, new lower face in
+    // s1_vx_lo); then carry plane s + 1's p^n and lower face.
+    auto step1 = [&](const int s, PlaneState<real, V, NR> &o, auto xwall_tag) {
+        constexpr bool XWALL = decltype(xwall_tag)::value;
+        const int64_t cc = L.cell(s);
+        const bool up_interior = !XWALL || s + 1 < nx;
+        real vx_v[V], p_up[V], vy_lo_in[V], vy_hi_in[V], q_lo[V], q_hi[V], vz_v[V];
+        vload<real, V>(vx_in + cc + L.plane, vx_v);
+        vload<real, V>(p_in + cc + (up_interior ? L.plane : 0), p_up);
+        const int64_t cy = L.face_y(s);
+        vload<real, V>(vy_in + cy, vy_lo_in);
+        vload<real, V>(vy_in + cy + nz, vy_hi_in);
+        vload<real, V>(p_in + cc - c.dy_lo, q_lo);
+        vload<real, V>(p_in + cc + c.dy_hi, q_hi);
+        const int64_t cz = L.face_z(s);
+        vload<real, V>(vz_in + cz, vz_v);
+        const real vz_top = vz_in[cz + V];
+        const real pz_lo = p_in[cc - c.dz_lo];
+        const real pz_hi = p_in[cc + c.dz_hi];
+        real states[NR > 0 ? NR : 1][V];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vload<real, V>(psi_in + (int64_t)r * n_cells + cc, states[r]);
+        advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
+            a, c, s, step, pn_c, p_up, q_lo, q_hi, pz_lo, pz_hi, s1_vx_lo, vx_v, vy_lo_in,
+            vy_hi_in, vz_v, vz_top, states, o);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            pn_c[e] = p_up[e];
+            s1_vx_lo[e] = o.vx_hi[e];
+        }
+    };
+    auto step1_at = [&](const int s, PlaneState<real, V, NR> &o) {
+        if (s == 0 || s == nx - 1) step1(s, o, Wall{});
+        else step1(s, o, Interior{});
+    };
+
+    // Second step of plane i from the first step's results: A (plane i), p_next (p^(n+1) of
+    // plane i + 1), the lower face's new velocity s2_vx_lo and the neighbours in LDS.
+    real s2_vx_lo[V];
+    auto step2 = [&](const int i, PlaneState<real, V, NR> &A, const real (&p_next)[V],
+                     auto xwall_tag) {
+        constexpr bool XWALL = decltype(xwall_tag)::value;
+        const int64_t cc = L.cell(i);
+        real q_lo[V], q_hi[V];
+        vload<real, V>(slot(i, t - G), q_lo);
+        vload<real, V>(slot(i, t + G), q_hi);
+        const real pz_lo = slot(i, t - 1)[V - 1];
+        const real pz_hi = slot(i, t + 1)[0];
+        real vz_v[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) vz_v[e] = A.vz[e];
+        PlaneState<real, V, NR> o;
+        advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
+            a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi, A.vy_lo,
+            A.vy_hi, vz_v, A.vz[V], A.psi, o);
+        const int64_t cy = L.face_y(i), cz = L.face_z(i);
+        if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
+        vstore<real, V>(vy_out + cy, o.vy_lo);
+        if (!c.has_yhi) vstore<real, V>(vy_out + cy + nz, o.vy_hi);
+        {
+            real lower[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) lower[e] = o.vz[e];
+            vstore<real, V>(vz_out + cz, lower);
+        }
+        if (!c.has_zhi) vz_out[cz + V] = o.vz[V];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vstore<real, V>(psi_out + (int64_t)r * n_cells + cc, o.psi[r]);
+        vstore<real, V>(p_out + cc, o.p);
+        if (c.special) record_cells<real, V>(a, c, i, column + 1, o.p);
+#pragma unroll
+        for (int e = 0; e < V; ++e) s2_vx_lo[e] = o.vx_hi[e];
+    };
+
+    // Prologue: step 1 of the halo plane i0 - 1 (its p^(n+1) only) and of plane i0.
+    PlaneState<real, V, NR> A, B;
+    real p_prev[V];
+    real v_half_lo[V];  // v^(n+1/2) on face i0
+    if (valid) {
+        const int s0 = i0 > 0 ? i0 - 1 : i0;
+        first_face(s0);
+        if (i0 > 0) {
+            step1_at(i0 - 1, B);
+#pragma unroll
+            for (int e = 0; e < V; ++e) p_prev[e] = B.p[e];
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) v_half_lo[e] = s1_vx_lo[e];
+        step1_at(i0, A);
+        if (owner && c.special) record_cells<real, V>(a, c, i0, column, A.p);
+        share(i0, A.p);
+    }
+    __syncthreads();
+    if (owner) {
+        // v^(n+3/2) on face i0, the lower face of the first cell (the prologue of
+        // fused_march_vec with the first step's results as inputs).
+        const real fxf = LAYER ? a.face[0][i0] : real(1);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const real w = i0 > 0 ? v_half_lo[e] - vel * (A.p[e] - p_prev[e]) : v_half_lo[e];
+            s2_vx_lo[e] = damp3<real, LAYER>(w, fxf, c.fyc, c.fzc[e]);
+        }
+        vstore<real, V>(vx_out + L.cell(i0), s2_vx_lo);
+    }
+
+    for (int i = i0; i < i1; ++i) {
+        const bool more = i + 1 < nx;
+        if (valid && more) {
+            step1_at(i + 1, B);
+            if (owner && c.special && i + 1 < i1) record_cells<real, V>(a, c, i + 1, column, B.p);
+            share(i + 1, B.p);
+        }
+        __syncthreads();
+        if (owner) {
+            if (i == 0 || i == nx - 1) step2(i, A, more ? B.p : A.p, Wall{});
+            else step2(i, A, more ? B.p : A.p, Interior{});
+        }
+        if (valid && more) A = B;
+    }
+}
+
+}  // namespace acfdtd

@@ -200,7 +200,9 @@ int run_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, in
     const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
     const bool walls = d.wall_mask != 0, layer = d.has_layer != 0;
     int64_t n = 0;
-    if (has_swap(d) && n_steps > 1) n = launch_fused_pairs<real>(d, step_offset, n_steps, column, stream);
+    if (n_steps > 3) n = launch_fused2_quads<real>(d, step_offset, n_steps, column, stream);
+    if (has_swap(d) && n_steps - n > 1)
+        n += launch_fused_pairs<real>(d, step_offset + n, n_steps - n, column + n, stream);
     const dim3 block(64, a.tiles.tile_j);
     const dim3 grid(a.tiles.launched);
     const VelocityKernel<real> vk = pick_velocity<real>(layer, comp);

@@ -244,6 +244,8 @@ inline int validate(const acfdtd_hip_desc *d) {
     if (!(d->tile_j == 0 || d->tile_j == 1 || d->tile_j == 2 || d->tile_j == 4 || d->tile_j == 8))
         return fail(ACFDTD_HIP_EINVAL, "tile_j must be 0, 1, 2, 4 or 8");
     if (d->chunk_x < 0) return fail(ACFDTD_HIP_EINVAL, "chunk_x must be >= 0");
+    if (d->time_block < 0 || d->time_block > 2)
+        return fail(ACFDTD_HIP_EINVAL, "time_block must be 0, 1 or 2");
     if (!d->p || !d->vx || !d->vy || !d->vz) return fail(ACFDTD_HIP_EINVAL, "field pointer is NULL");
     if (d->n_relax > 0 && !d->psi) return fail(ACFDTD_HIP_EINVAL, "relaxation states are NULL");
     if (d->wall_mask != 0 && !d->wall_table) return fail(ACFDTD_HIP_EINVAL, "wall table is NULL");

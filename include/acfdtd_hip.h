@@ -57,7 +57,7 @@ typedef struct acfdtd_hip_desc {
     uint32_t flags;     /* ACFDTD_HIP_COMPENSATED */
     int32_t tile_j;     /* rows per workgroup (1, 2, 4 or 8); 0 = default */
     int32_t chunk_x;    /* planes marched per workgroup; 0 = default */
-    int32_t reserved;
+    int32_t time_block; /* steps per sweep of acfdtd_hip_run: 0 or 1 = one, 2 = two (see psi_swap) */
 
     double vel_coef;    /* dt / (rho dx) */
     double pres_coef;   /* rho c^2 dt / dx */
@@ -113,6 +113,15 @@ typedef struct acfdtd_hip_desc {
     int64_t wall_tape_stride;
     int64_t wall_tape_step0;
     int64_t wall_tape_rows;
+
+    /* Second set of relaxation states (device, same shape as psi), for time_block = 2: a sweep
+     * that advances two steps reads p, v, psi from one set and writes them to the other (its
+     * halo cells read their neighbours' step-n states), in pairs A -> B -> A, so four steps
+     * move (8 + 2 n_relax) elements per cell twice instead of four times. Used when time_block
+     * is 2, the four swap buffers are given, psi_swap is given (or n_relax is 0), no boundary
+     * tape is attached and nz is a multiple of the lane width (4 cells fp32, 2 fp64); the
+     * remaining steps run one per sweep. Results are bit-identical either way. */
+    void *psi_swap;
 } acfdtd_hip_desc;
 
 /* sizeof(acfdtd_hip_desc), for the caller's mirror check. Replaces acfdtd_struct_size

@@ -1,0 +1,110 @@
+"""Two steps per sweep (time_block=2) against one step per sweep, bit for bit.
+
+The two-step sweep recomputes each cell's first step in halo lanes and halo planes and keeps
+the fields of the intermediate step in registers and LDS; every cell's arithmetic is the
+single-step sweep's, so fields, relaxation states and recordings must be identical. The cases
+put workgroup edges inside the grid (rows of lane groups split across workgroups, several x
+chunks), run step counts with every remainder modulo four (the remainder runs as single
+steps), and cover every combination of walls, sponge, air and compensation in fp32 and fp64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from cases import PARITY
+
+from ac_fdtd_amd import AbsorbingLayer, AirAbsorption, Grid, WallAdmittances
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("require_gpu")]
+
+# (shape, tile_j, chunk_x): lane groups per row nz / V, rows split across workgroups when
+# ny * nz / V exceeds 64 * tile_j.
+CASES = [
+    ((9, 11, 12), 4, 8),
+    ((13, 40, 64), 1, 3),
+    ((13, 40, 64), 2, 1),
+    ((21, 17, 32), 8, 5),
+    ((1, 6, 8), 1, 1),
+    ((34, 30, 128), 2, 4),
+]
+
+
+def _solver(shape, dtype, config, time_block, tile_j, chunk_x, compensated=False):
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    rng = np.random.default_rng(11)
+    grid = Grid(dx=0.03, nx=shape[0], ny=shape[1], nz=shape[2])
+    solver = AcousticFDTD(grid, dtype=dtype, time_block=time_block, tile_j=tile_j,
+                          chunk_x=chunk_x, compensated=compensated, **config)
+    solver.p.copy_(torch.as_tensor(rng.standard_normal(grid.shape)))
+    for a in range(3):
+        solver.velocity[a].copy_(torch.as_tensor(1e-3 * rng.standard_normal(grid.velocity_shape(a))))
+    lengths = np.array(grid.lengths)
+    solver.add_volume_source(tuple(0.37 * lengths), rng.standard_normal(23))
+    solver.add_volume_source(tuple(0.71 * lengths), rng.standard_normal(9))
+    for f in (0.13, 0.5, 0.86):
+        solver.add_pressure_receiver(tuple(f * lengths))
+    return solver
+
+
+def _assert_same(a, b):
+    assert torch.equal(a.p, b.p)
+    for axis in range(3):
+        assert torch.equal(a.velocity[axis], b.velocity[axis])
+    assert torch.equal(a.relaxation_states, b.relaxation_states)
+    np.testing.assert_array_equal(a.recorded_pressure, b.recorded_pressure)
+
+
+CONFIGS = {
+    "rigid": {},
+    "walls": {"walls": WallAdmittances(0.1, 0.3, 0.0, 0.2, 0.05, 0.4)},
+    "layer": {"absorbing_layer": AbsorbingLayer(thickness=3)},
+    "air": {"air_absorption": AirAbsorption(18.0, 40.0)},
+    "all": PARITY["all"],
+}
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("config", list(CONFIGS))
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_two_step_sweep_is_bitwise_single_steps(dtype, config, case):
+    shape, tile_j, chunk_x = CASES[case]
+    one = _solver(shape, dtype, CONFIGS[config], 1, tile_j, chunk_x)
+    two = _solver(shape, dtype, CONFIGS[config], 2, tile_j, chunk_x)
+    for n_steps in (8, 5, 4, 7, 6):
+        one.run(n_steps)
+        two.run(n_steps)
+    _assert_same(one, two)
+    assert one.step_index == two.step_index == 30
+
+
+def test_two_step_compensated_fp32():
+    shape, tile_j, chunk_x = CASES[3]
+    one = _solver(shape, np.float32, CONFIGS["all"], 1, tile_j, chunk_x, compensated=True)
+    two = _solver(shape, np.float32, CONFIGS["all"], 2, tile_j, chunk_x, compensated=True)
+    one.run(13)
+    two.run(13)
+    _assert_same(one, two)
+
+
+def test_two_step_falls_back_where_it_does_not_apply():
+    """nz not a multiple of the lane width, or a boundary tape attached: single steps."""
+    shape = (10, 9, 10)  # fp32 lanes of 4 do not divide nz = 10
+    one = _solver(shape, np.float32, CONFIGS["all"], 1, 0, 0)
+    two = _solver(shape, np.float32, CONFIGS["all"], 2, 0, 0)
+    one.run(9)
+    two.run(9)
+    _assert_same(one, two)
+    shape = (10, 9, 12)
+    one = _solver(shape, np.float32, CONFIGS["all"], 1, 0, 0)
+    two = _solver(shape, np.float32, CONFIGS["all"], 2, 0, 0)
+    tapes = [torch.zeros((9, s.wall_tape_stride()), dtype=s.dtype, device=s.device) for s in (one, two)]
+    one.attach_wall_tape(tapes[0], 0)
+    two.attach_wall_tape(tapes[1], 0)
+    one.run(9)
+    two.run(9)
+    _assert_same(one, two)
+    assert torch.equal(tapes[0], tapes[1])
