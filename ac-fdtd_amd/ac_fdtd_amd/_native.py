@@ -160,6 +160,8 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_pressure_sweep.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr]
     lib.acfdtd_hip_fused_step.restype = _c.c_int
     lib.acfdtd_hip_fused_step.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _c.c_int, _ptr]
+    lib.acfdtd_hip_fused2_step.restype = _c.c_int
+    lib.acfdtd_hip_fused2_step.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _c.c_int, _ptr]
     lib.acfdtd_hip_wall_tape_stride.restype = _c.c_int64
     lib.acfdtd_hip_wall_tape_stride.argtypes = [_c.c_int32, _c.c_int32, _c.c_int32]
     lib.acfdtd_hip_adjoint_partials_count.restype = _c.c_size_t

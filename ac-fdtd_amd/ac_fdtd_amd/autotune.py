@@ -25,10 +25,13 @@ import torch
 
 from . import _native
 
-__all__ = ["CANDIDATES", "adjoint_tiling", "forward_tiling"]
+__all__ = ["CANDIDATES", "CANDIDATES_TWO_STEP", "adjoint_tiling", "forward_plan", "forward_tiling"]
 
-#: (tile_j, chunk_x) shapes tried.
+#: (tile_j, chunk_x) shapes tried for one step per sweep.
 CANDIDATES = tuple((tile_j, chunk) for tile_j in (1, 2, 4, 8) for chunk in (1, 2, 3, 4, 6, 8))
+#: (tile_j, chunk_x) shapes tried for two steps per sweep: more wavefronts per workgroup
+#: spread the halo rows thinner, longer marches the halo planes.
+CANDIDATES_TWO_STEP = tuple((tile_j, chunk) for tile_j in (2, 4, 6, 8) for chunk in (4, 8, 16, 32))
 
 _cache: dict[str, tuple[int, int]] = {}
 _loaded = False
@@ -76,11 +79,12 @@ def _key(kind: str, solver) -> str:
 
 
 def _steps_per_sample(solver) -> int:
-    """Enough steps for about 2 ms per timing, at least 4, at most 60, even."""
+    """Enough steps for about 2 ms per timing, at least 4, at most 60, a multiple of 4 (so a
+    two-step timing runs two-step sweeps only)."""
     bytes_per_cell = solver.np_dtype.itemsize * (8 + 2 * solver.coefficients.n_relax)
     seconds = solver.grid.n_cells * bytes_per_cell / 4e12
     steps = int(min(60, max(4, 2e-3 / max(seconds, 1e-9))))
-    return steps + steps % 2
+    return steps + (-steps) % 4
 
 
 class _Scratch:
@@ -98,6 +102,7 @@ class _Scratch:
         self.sets = [[field(g.shape)] + [field(g.velocity_shape(a)) for a in range(3)]
                      for _ in range(2)]
         self.psi = torch.zeros((k.n_relax, *g.shape), **opts)
+        self.psi_swap = torch.zeros((k.n_relax, *g.shape), **opts) if k.n_relax else None
         d = solver._descriptor(None, tape=False)
         d.n_sources = d.n_receivers = 0
         d.source_cell = d.source_length = d.source_signal = None
@@ -105,6 +110,7 @@ class _Scratch:
         d.p, d.vx, d.vy, d.vz = (t.data_ptr() for t in self.sets[0])
         d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.sets[1])
         d.psi = self.psi.data_ptr() if k.n_relax else None
+        d.psi_swap = self.psi_swap.data_ptr() if k.n_relax else None
         self.desc = d
 
 
@@ -137,11 +143,15 @@ def _pick(solver, kind, launch, candidates):
     return best
 
 
-def forward_tiling(solver, candidates=CANDIDATES) -> tuple[int, int]:
-    """Fastest (tile_j, chunk_x) of the fused forward step for ``solver``'s configuration."""
+def forward_tiling(solver, time_block: int = 1, candidates=None) -> tuple[int, int]:
+    """Fastest (tile_j, chunk_x) of the fused forward sweep advancing ``time_block`` steps, for
+    ``solver``'s configuration."""
     if not solver.fused:
         return solver.tile_j, solver.chunk_x
+    if candidates is None:
+        candidates = CANDIDATES_TWO_STEP if time_block == 2 else CANDIDATES
     scratch = _Scratch(solver)
+    scratch.desc.time_block = time_block
     lib = solver._lib
     handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
 
@@ -150,7 +160,34 @@ def forward_tiling(solver, candidates=CANDIDATES) -> tuple[int, int]:
         _native.check(lib.acfdtd_hip_run(ctypes.byref(scratch.desc), 0, steps, 0, handle),
                       "acfdtd_hip_run")
 
-    return _pick(solver, "forward", launch, candidates)
+    kind = "forward2" if time_block == 2 else "forward"
+    return _pick(solver, kind, launch, candidates)
+
+
+def forward_plan(solver) -> tuple[int, int, int]:
+    """(time_block, tile_j, chunk_x) for ``solver``: the fastest one-step tiling and, when the
+    solver may advance two steps per sweep, the fastest two-step tiling, whichever runs more
+    steps per second."""
+    single = forward_tiling(solver, 1)
+    if not solver.two_step_applies():
+        return 1, single[0], single[1]
+    double = forward_tiling(solver, 2)
+    per_step = {}
+    for time_block, shape in ((1, single), (2, double)):
+        scratch = _Scratch(solver)
+        scratch.desc.time_block = time_block
+        scratch.desc.tile_j, scratch.desc.chunk_x = shape
+        handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
+
+        def launch(steps, _shape, desc=scratch.desc):
+            _native.check(solver._lib.acfdtd_hip_run(ctypes.byref(desc), 0, steps, 0, handle),
+                          "acfdtd_hip_run")
+
+        with torch.cuda.device(solver.device):
+            per_step[time_block] = _time(solver, launch, shape, _steps_per_sample(solver), 3)
+    if per_step[2] < per_step[1]:
+        return 2, double[0], double[1]
+    return 1, single[0], single[1]
 
 
 def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tuple[int, int]:

@@ -72,9 +72,10 @@ class AcousticFDTD:
         autotune: time the candidate tilings of the fused step on scratch fields of this
             configuration and keep the fastest (overrides ``tile_j`` / ``chunk_x``; see
             :mod:`ac_fdtd_amd.autotune`).
-        time_block: steps per sweep over memory, 1 or 2. With 2 (fused path only) the run
-            advances two steps per sweep wherever it can (a second set of relaxation states
-            is allocated for it); results are bit-identical to single steps.
+        time_block: steps per sweep over memory, 2 (default) or 1. With 2 (fused path only)
+            the run advances two steps per sweep wherever it can (a second set of relaxation
+            states is allocated for it); results are bit-identical to single steps. With
+            ``autotune`` the faster of the two is kept.
     """
 
     def __init__(
@@ -95,7 +96,7 @@ class AcousticFDTD:
         relaxation_strengths=None,
         wall_table_always: bool = False,
         autotune: bool = False,
-        time_block: int = 1,
+        time_block: int = 2,
     ) -> None:
         if time_block not in (1, 2):
             raise ValueError(f"time_block must be 1 or 2, got {time_block}")
@@ -164,9 +165,9 @@ class AcousticFDTD:
         self._runs: list[torch.Tensor] = []  # per run(): (receivers at that time, n_steps)
         self._energy_workspace = None
         if autotune:
-            from .autotune import forward_tiling
+            from .autotune import forward_plan
 
-            self.tile_j, self.chunk_x = forward_tiling(self)
+            self.time_block, self.tile_j, self.chunk_x = forward_plan(self)
 
     # -- reference API ----------------------------------------------------------------------
     @property
@@ -265,8 +266,9 @@ class AcousticFDTD:
     def timed_steps(self, n_steps: int) -> dict[str, float]:
         """Advance ``n_steps`` like :meth:`run`, one kernel per library call, with HIP events
         around every kernel on the launch stream; returns the mean duration in ms of each
-        kernel kind used (``"fused"`` for the fused step, or ``"velocity"`` and ``"pressure"``
-        for the two sweeps). The state advances exactly as :meth:`run` would."""
+        kernel kind used (``"fused2"`` for a two-step sweep, ``"fused"`` for the fused step,
+        ``"velocity"`` and ``"pressure"`` for the two sweeps). The state advances exactly as
+        :meth:`run` would."""
         n_steps = int(n_steps)
         if n_steps <= 0:
             return {}
@@ -279,13 +281,22 @@ class AcousticFDTD:
         ref = ctypes.byref(desc)
         launches = []  # (kind, call)
         n = 0
-        if self.fused:
-            for n in range(0, n_steps - 1, 2):
+        if self.two_step_applies():
+            quads = n_steps - n_steps % 4
+            for q in range(0, quads, 4):
                 for half in (0, 1):
-                    launches.append(("fused", lambda s=n + half, r=half: _native.check(
+                    launches.append(("fused2", lambda s=q + 2 * half, r=half: _native.check(
+                        self._lib.acfdtd_hip_fused2_step(ref, self.step_index + s, s, r, handle),
+                        "acfdtd_hip_fused2_step")))
+            n = quads
+        if self.fused:
+            pairs = n + (n_steps - n) - (n_steps - n) % 2
+            for q in range(n, pairs, 2):
+                for half in (0, 1):
+                    launches.append(("fused", lambda s=q + half, r=half: _native.check(
                         self._lib.acfdtd_hip_fused_step(ref, self.step_index + s, s, r, handle),
                         "acfdtd_hip_fused_step")))
-            n = n_steps - n_steps % 2
+            n = pairs
         for s in range(n, n_steps):
             launches.append(("velocity", lambda: _native.check(
                 self._lib.acfdtd_hip_velocity_sweep(ref, handle), "acfdtd_hip_velocity_sweep")))
@@ -306,6 +317,18 @@ class AcousticFDTD:
         for e, (kind, _) in enumerate(launches):
             totals.setdefault(kind, []).append(events[e].elapsed_time(events[e + 1]))
         return {kind: sum(v) / len(v) for kind, v in totals.items()}
+
+    def two_step_applies(self) -> bool:
+        """Whether :meth:`run` advances two steps per sweep: ``time_block=2``, no boundary tape,
+        nz a multiple of the lane width (4 cells fp32, 2 fp64) and one halo row on each side of
+        a 64-lane workgroup within 512 lanes (the library's own test, restated)."""
+        if self.time_block != 2 or not self.fused or self._wall_tape is not None:
+            return False
+        if self.coefficients.n_relax and self._psi_swap is None:
+            return False
+        vec = 4 if self.np_dtype.itemsize == 4 else 2
+        nz = self.grid.nz
+        return nz % vec == 0 and 64 + 2 * (nz // vec) <= 512
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)

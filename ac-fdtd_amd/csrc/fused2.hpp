@@ -15,8 +15,14 @@
 
 namespace acfdtd {
 
+// Waves per SIMD the compiler must leave room for (caps the registers per lane: 2 -> 256,
+// 3 -> 168, 4 -> 128).
+#ifndef ACFDTD_FUSED2_MIN_WAVES
+#define ACFDTD_FUSED2_MIN_WAVES 2
+#endif
+
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
-__global__ __launch_bounds__(kFused2Threads) void fused2_step(const SweepArgs<real> a,
+__global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused2_step(const SweepArgs<real> a,
                                                              const Fused2IO<real> io,
                                                              const Tiling2 t2, const int64_t step,
                                                              const int64_t column) {
@@ -79,16 +85,12 @@ Fused2Kernel<real> pick_fused2(int nr, bool walls, bool layer, bool comp) {
     }
 }
 
+// The two directions of a two-step sweep: forward reads the descriptor's fields and writes the
+// swap set, backward the other way round.
 template <typename real>
-int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
-                            int64_t column, hipStream_t stream) {
-    Tiling2 t2;
-    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2)) return 0;
-    SweepArgs<real> a;
-    fill_args(d, a);
-    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
-    const Fused2Kernel<real> k = pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
-    Fused2IO<real> forward{}, backward{};
+void fused2_io(const acfdtd_hip_desc &d, Fused2IO<real> &forward, Fused2IO<real> &backward) {
+    forward = Fused2IO<real>{};
+    backward = Fused2IO<real>{};
     forward.p_in = backward.p_out = static_cast<real *>(d.p);
     forward.vx_in = backward.vx_out = static_cast<real *>(d.vx);
     forward.vy_in = backward.vy_out = static_cast<real *>(d.vy);
@@ -104,6 +106,19 @@ int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64
     backward.vy_in = forward.vy_out;
     backward.vz_in = forward.vz_out;
     backward.psi_in = forward.psi_out;
+}
+
+template <typename real>
+int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
+                            int64_t column, hipStream_t stream) {
+    Tiling2 t2;
+    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2)) return 0;
+    SweepArgs<real> a;
+    fill_args(d, a);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    const Fused2Kernel<real> k = pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    Fused2IO<real> forward, backward;
+    fused2_io<real>(d, forward, backward);
     const dim3 grid(t2.launched), block(t2.threads);
     int64_t n = 0;
     for (; n + 3 < n_steps; n += 4) {
@@ -112,6 +127,25 @@ int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64
                            column + n + 2);
     }
     return n;
+}
+
+// One two-step sweep for steps `step` and `step` + 1 (reverse: from the swap set back);
+// ACFDTD_HIP_EINVAL when the descriptor does not allow it.
+template <typename real>
+int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, bool reverse,
+                       hipStream_t stream) {
+    Tiling2 t2;
+    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2))
+        return fail(ACFDTD_HIP_EINVAL, "the two-step sweep does not apply to this descriptor");
+    SweepArgs<real> a;
+    fill_args(d, a);
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    Fused2IO<real> forward, backward;
+    fused2_io<real>(d, forward, backward);
+    hipLaunchKernelGGL(pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp),
+                       dim3(t2.launched), dim3(t2.threads), 0, stream, a,
+                       reverse ? backward : forward, t2, step, column);
+    return check_hip(hipGetLastError(), "two-step sweep launch");
 }
 
 }  // namespace acfdtd

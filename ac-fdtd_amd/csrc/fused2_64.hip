@@ -7,4 +7,7 @@ namespace acfdtd {
 template int64_t launch_fused2_quads<double>(const acfdtd_hip_desc &, int64_t, int64_t,
                                              int64_t, hipStream_t);
 
+template int launch_fused2_once<double>(const acfdtd_hip_desc &, int64_t, int64_t, bool,
+                                        hipStream_t);
+
 }  // namespace acfdtd

@@ -691,4 +691,10 @@ template <typename real>
 int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
                             int64_t column, hipStream_t stream);
 
+// One two-step sweep (steps `step`, `step` + 1), for per-launch timing; defined beside
+// launch_fused2_quads.
+template <typename real>
+int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, bool reverse,
+                       hipStream_t stream);
+
 }  // namespace acfdtd

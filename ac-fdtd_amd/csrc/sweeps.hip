@@ -354,3 +354,18 @@ extern "C" int acfdtd_hip_stream_copy(const void *src, void *dst, size_t bytes, 
                        static_cast<float4 *>(dst), n);
     return check_hip(hipGetLastError(), "stream copy launch");
 }
+
+extern "C" int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step,
+                                      int64_t recording_column, int reverse, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (step < 0 || recording_column < 0)
+        return fail(ACFDTD_HIP_EINVAL, "step and recording_column must be >= 0");
+    if (desc->n_receivers > 0 && recording_column + 1 >= desc->recording_stride)
+        return fail(ACFDTD_HIP_EINVAL, "recording columns out of range");
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8)
+        return launch_fused2_once<double>(*desc, step, recording_column, reverse != 0, s);
+    return launch_fused2_once<float>(*desc, step, recording_column, reverse != 0, s);
+}

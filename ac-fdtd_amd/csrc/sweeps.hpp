@@ -241,8 +241,7 @@ inline int validate(const acfdtd_hip_desc *d) {
     if ((d->flags & ~ACFDTD_HIP_COMPENSATED) != 0) return fail(ACFDTD_HIP_EINVAL, "unknown flags");
     if ((d->flags & ACFDTD_HIP_COMPENSATED) && d->real_bytes != 4)
         return fail(ACFDTD_HIP_EINVAL, "compensated summation is an fp32 mode");
-    if (!(d->tile_j == 0 || d->tile_j == 1 || d->tile_j == 2 || d->tile_j == 4 || d->tile_j == 8))
-        return fail(ACFDTD_HIP_EINVAL, "tile_j must be 0, 1, 2, 4 or 8");
+    if (d->tile_j < 0 || d->tile_j > 8) return fail(ACFDTD_HIP_EINVAL, "tile_j must be 0 to 8");
     if (d->chunk_x < 0) return fail(ACFDTD_HIP_EINVAL, "chunk_x must be >= 0");
     if (d->time_block < 0 || d->time_block > 2)
         return fail(ACFDTD_HIP_EINVAL, "time_block must be 0, 1 or 2");

@@ -7,14 +7,20 @@ width 4-8 cells at a seeded interior cell, 8 receivers at seeded interior cells,
 steps. All state is resident in HBM before timing; the timed region is one call of the library
 advancing ``--steps`` steps, bracketed by HIP events and synchronisations.
 
-Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3. Under ``extra``:
-the fp64 plain kernel and the compensated fp32 kernel on the same workload, the error of every
-fp32 run against the fp64 run (receiver relative L2, T20 per octave band), rigid-box energy
-drift in fp64 and fp32, a 384^3 fp32 run, 64 uneven rooms advanced as one batch (each checked
-against its own fp64 single-room run), and the cost of a wall-admittance gradient against a
-forward pass at 128^3. ``roofline`` is for the dominant kernel (the fused step), from HIP events
-around each of its launches; ``cpu_baseline`` is the reference's own compiled C loop
-(``oracle/_ref``, all host cores this process may use) on a bounded sample of the same workload.
+Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3 (two steps per
+sweep where the autotuner finds that faster, tile shapes tuned). ``roofline`` is for the
+dominant kernel: ``achieved`` counts the algorithmic bytes of the reference's two-sweep step,
+(12 + 2 n_relax) elements per cell-update (72 B in fp32 with air), times the cell-updates one
+launch performs, over the launch's mean duration from HIP events on the launch stream;
+``traffic`` is the HBM bytes per launch rocprofv3 measured for the same kernel and workload
+(committed profile), when there is one. Under ``extra``: the same run at one step per sweep,
+the fp64 plain kernel and the compensated fp32 kernel, the error of every fp32 run against the
+fp64 run (receiver relative L2, T20 per octave band), rigid-box energy drift in fp64 and fp32,
+a 384^3 fp32 run, 64 uneven rooms advanced as one batch (each checked against its own fp64
+single-room run), the cost of a wall-admittance gradient against a forward pass at 128^3, a
+512^3 room with its T20 per octave band against fp64 and its wall gradient, and the stream
+ceiling. ``cpu_baseline`` is the reference's own compiled C loop (``oracle/_ref``, all host
+cores this process may use) on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -79,43 +85,53 @@ def make_solver(cls, size, steps, dtype, **kwargs):
     return solver
 
 
-def bytes_per_cell(n_relax: int, itemsize: int) -> dict[str, int]:
-    """Algorithmic bytes per cell-update of each kernel, and of a two-sweep step.
+def update_bytes(n_relax: int, itemsize: int) -> int:
+    """Algorithmic bytes per cell-update, the roofline basis: each field element of the
+    reference's two-sweep step counted once (velocity sweep: read p, read and write vx, vy, vz;
+    pressure sweep: read vx, vy, vz, read and write p and each relaxation state), 12 + 2 n_relax
+    elements."""
+    return (12 + 2 * n_relax) * itemsize
 
-    Two sweeps: the velocity sweep reads p and reads/writes vx, vy, vz (7 elements), the
-    pressure sweep reads vx, vy, vz and reads/writes p and each relaxation state
-    (5 + 2 n_relax): 12 + 2 n_relax per step. The fused step reads p, vx, vy, vz from one set,
-    writes them to the other and reads/writes each relaxation state: 8 + 2 n_relax.
-    """
+
+def launch_bytes(kind: str, n_relax: int, itemsize: int) -> int:
+    """Algorithmic bytes per cell of one launch of ``kind`` on that basis: the two-step sweep
+    does two cell-updates, the fused step one, the velocity and pressure sweeps their parts."""
     return {
+        "fused2": 2 * update_bytes(n_relax, itemsize),
+        "fused": update_bytes(n_relax, itemsize),
         "velocity": 7 * itemsize,
         "pressure": (5 + 2 * n_relax) * itemsize,
-        "fused": (8 + 2 * n_relax) * itemsize,
-        "two_sweep_step": (12 + 2 * n_relax) * itemsize,
-    }
+    }[kind]
 
 
-PROFILE = ROOT / "profiles" / "fused_step_256cubed_fp32.json"
+def minimum_bytes(kind: str, n_relax: int, itemsize: int) -> float:
+    """Bytes per cell-update a launch of ``kind`` must move itself: the fused step reads p, v
+    from one set, writes them to the other and reads and writes the relaxation states
+    (8 + 2 n_relax elements); the two-step sweep does so once for two updates."""
+    fused = (8 + 2 * n_relax) * itemsize
+    return {"fused2": fused / 2, "fused": fused, "velocity": 7 * itemsize,
+            "pressure": (5 + 2 * n_relax) * itemsize}[kind]
+
+PROFILES = ROOT / "profiles"
 
 
 def measured_traffic(size: int, kernel: str):
-    """HBM bytes per launch of the fused step measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE, calibrated on a copy of known size) for this workload, from the committed
-    profile; None for another grid or kernel."""
-    if kernel != "fused" or not PROFILE.exists():
-        return None
-    record = json.loads(PROFILE.read_text())
-    if record.get("grid") != [size] * 3:
-        return None
-    return record["traffic_bytes_per_launch"]
+    """HBM bytes per launch of ``kernel`` measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE,
+    calibrated on a copy of known size) for this workload, from the committed profiles; None
+    when no profile covers this kernel and grid."""
+    for path in sorted(PROFILES.glob("*.json")):
+        record = json.loads(path.read_text())
+        if record.get("kind") == kernel and record.get("grid") == [size] * 3:
+            return record["traffic_bytes_per_launch"]
+    return None
 
-def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True):
+def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True, time_block=2):
     import torch
 
     from ac_fdtd_amd.scheme import AcousticFDTD
 
     solver = make_solver(AcousticFDTD, size, steps + warmup + 40, dtype, compensated=compensated,
-                         autotune=autotune)
+                         autotune=autotune, time_block=time_block)
     solver.run(warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -141,6 +157,7 @@ def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True)
         "recorded": recorded.astype(np.float64),
         "sample_rate": solver.sample_rate,
         "tiling": [solver.tile_j, solver.chunk_x],
+        "time_block": solver.time_block,
     }
 
 
@@ -584,24 +601,35 @@ def main() -> int:
     log(f"fp32 {args.size}^3, {args.steps} steps")
     f32 = gpu_throughput(args.size, args.steps, args.warmup, np.float32)
     log(f"fp32: {f32['gcells']:.2f} Gcell/s, kernels {f32['kernels']}")
-    per_cell = bytes_per_cell(f32["n_relax"], 4)
-    # The dominant kernel: the one with the largest total time per step.
+    # The dominant kernel: the one with the largest mean launch time.
     dominant = max(f32["kernels"], key=f32["kernels"].get)
-    sweep_bytes = per_cell[dominant]
+    sweep_bytes = launch_bytes(dominant, f32["n_relax"], 4)
     achieved = sweep_bytes * f32["cells"] / (f32["kernels"][dominant] * 1e-3) / 1e9
     traffic = measured_traffic(args.size, dominant)
 
     extra = {
+        "time_block": f32["time_block"],
         "tiling_tile_j_chunk_x": f32["tiling"],
         "kernel_ms": {k: round(v, 4) for k, v in f32["kernels"].items()},
-        "bytes_per_cell_update": {k: per_cell[k] for k in (*f32["kernels"], "two_sweep_step")},
-        # Gcell/s against the two-sweep byte count (12 + 2 n_relax elements per cell-update),
-        # the ceiling 8 TB/s / 72 B = 111 Gcell/s that the throughput target is stated on.
-        "two_sweep_equivalent_hbm_fraction": round(
-            per_cell["two_sweep_step"] * f32["gcells"] / HBM_PEAK_GBS, 4
-        ),
+        "roofline_basis_bytes_per_cell_update": update_bytes(f32["n_relax"], 4),
+        "kernel_minimum_bytes_per_cell_update": minimum_bytes(dominant, f32["n_relax"], 4),
+        # Bandwidth the kernel itself needs at this rate: its own minimum bytes per update.
+        "kernel_minimum_bytes_hbm_fraction": round(
+            minimum_bytes(dominant, f32["n_relax"], 4) * f32["gcells"] / HBM_PEAK_GBS, 4),
     }
     if not args.skip_extra:
+        log("fp32 one step per sweep")
+        one = gpu_throughput(args.size, args.steps, args.warmup, np.float32, time_block=1)
+        kind = max(one["kernels"], key=one["kernels"].get)
+        extra["fp32_one_step_per_sweep"] = {
+            "gcells": round(one["gcells"], 3),
+            "tiling_tile_j_chunk_x": one["tiling"],
+            "kernel_ms": {k: round(v, 4) for k, v in one["kernels"].items()},
+            "roofline_frac": round(launch_bytes(kind, one["n_relax"], 4) * one["cells"]
+                                   / (one["kernels"][kind] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bitwise_equal_recordings": bool(np.array_equal(one["recorded"], f32["recorded"])),
+        }
+        del one
         log("fp32 compensated")
         comp = gpu_throughput(args.size, args.steps, args.warmup, np.float32, compensated=True)
         extra["fp32_compensated_gcells"] = round(comp["gcells"], 3)
@@ -609,9 +637,9 @@ def main() -> int:
         f64 = gpu_throughput(args.size, args.steps, args.warmup, np.float64)
         extra["fp64_plain_gcells"] = round(f64["gcells"], 3)
         extra["fp32_over_fp64_speed"] = round(f32["gcells"] / f64["gcells"], 3)
-        extra["fp64_two_sweep_equivalent_hbm_fraction"] = round(
-            bytes_per_cell(f64["n_relax"], 8)["two_sweep_step"] * f64["gcells"] / HBM_PEAK_GBS, 4
-        )
+        extra["fp64_time_block"] = f64["time_block"]
+        extra["fp64_hbm_fraction"] = round(
+            update_bytes(f64["n_relax"], 8) * f64["gcells"] / HBM_PEAK_GBS, 4)
         extra["fp32_error_vs_fp64"] = error_against(f32, f64)
         extra["fp32_compensated_error_vs_fp64"] = error_against(comp, f64)
         del comp, f64
@@ -641,7 +669,7 @@ def main() -> int:
     log("stream ceiling")
     ceiling = stream_ceiling()
     extra["stream_copy_ceiling_gbs"] = round(ceiling, 1)
-    extra["fused_fraction_of_stream_ceiling"] = round(achieved / ceiling, 4)
+    extra["dominant_kernel_fraction_of_stream_ceiling"] = round(achieved / ceiling, 4)
 
     baseline = None
     if not args.skip_cpu:

@@ -55,7 +55,7 @@ typedef struct acfdtd_hip_desc {
     int32_t n_sources;
     int32_t n_receivers;
     uint32_t flags;     /* ACFDTD_HIP_COMPENSATED */
-    int32_t tile_j;     /* rows per workgroup (1, 2, 4 or 8); 0 = default */
+    int32_t tile_j;     /* wavefronts of lanes per workgroup, 1 to 8; 0 = default */
     int32_t chunk_x;    /* planes marched per workgroup; 0 = default */
     int32_t time_block; /* steps per sweep of acfdtd_hip_run: 0 or 1 = one, 2 = two (see psi_swap) */
 
@@ -155,6 +155,13 @@ int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t step, int64_t
  * counterpart. */
 int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step, int64_t recording_column,
                           int reverse, void *stream);
+
+/* One two-step sweep (steps `step` and `step` + 1, recording into `recording_column` and the
+ * next), from p / vx / vy / vz / psi to the swap set, or back with `reverse` nonzero; needs
+ * what time_block = 2 needs (see psi_swap) and fails with ACFDTD_HIP_EINVAL otherwise. For
+ * per-kernel timing and tuning; no reference counterpart. */
+int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step, int64_t recording_column,
+                           int reverse, void *stream);
 
 /* ----------------------------------------------------------------------- batched rooms --
  * Many rooms advance together, one launch per step (one fused launch per step with swap

@@ -50,7 +50,8 @@ def _solver(autotuned: bool, dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_tuned_solver_is_bitwise_the_default(dtype):
     tuned, plain = _solver(True, dtype), _solver(False, dtype)
-    assert (tuned.tile_j, tuned.chunk_x) in autotune.CANDIDATES
+    assert (tuned.tile_j, tuned.chunk_x) in autotune.CANDIDATES + autotune.CANDIDATES_TWO_STEP
+    assert tuned.time_block in (1, 2)
     tuned.run(41)
     plain.run(41)
     assert torch.equal(tuned.p, plain.p)

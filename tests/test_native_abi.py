@@ -72,7 +72,9 @@ def _desc(**overrides):
         ({"n_relax": 2}, EINVAL, "relaxation states"),
         ({"flags": 1}, EINVAL, "fp32 mode"),
         ({"flags": 8}, EINVAL, "unknown flags"),
-        ({"tile_j": 3}, EINVAL, "tile_j"),
+        ({"tile_j": 9}, EINVAL, "tile_j"),
+        ({"tile_j": -1}, EINVAL, "tile_j"),
+        ({"time_block": 3}, EINVAL, "time_block"),
         ({"p": None}, EINVAL, "field pointer"),
         ({"wall_mask": 5}, EINVAL, "wall table"),
         ({"has_layer": 1}, EINVAL, "sponge profile"),
