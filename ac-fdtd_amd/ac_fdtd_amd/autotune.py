@@ -6,7 +6,9 @@ workgroup ``chunk_x`` consecutive planes to march. Which shape is fastest depend
 on the precision and on the terms the step carries, and it differs between the forward step
 and the adjoint step, so it is measured: every candidate shape runs a few steps on scratch
 fields that share the solver's descriptor (coefficients, walls, sponge, relaxation) and is
-timed with HIP events on the launch stream; the fastest wins.
+timed with HIP events on the launch stream; the fastest wins. Where the solver may advance
+two steps per sweep (``time_block=2``), the best two-step shape and the best one-step shape are
+timed against each other too, and the faster plan is kept.
 
 Results are kept per configuration for the life of the process and, when the environment
 variable ``ACFDTD_TUNE_CACHE`` names a JSON file, across processes. A tiling never changes the
@@ -33,7 +35,7 @@ CANDIDATES = tuple((tile_j, chunk) for tile_j in (1, 2, 4, 8) for chunk in (1, 2
 #: spread the halo rows thinner, longer marches the halo planes.
 CANDIDATES_TWO_STEP = tuple((tile_j, chunk) for tile_j in (2, 4, 6, 8) for chunk in (4, 8, 16, 32))
 
-_cache: dict[str, tuple[int, int]] = {}
+_cache: dict[str, tuple[int, ...]] = {}
 _loaded = False
 
 
@@ -56,7 +58,7 @@ def _lookup(key: str):
     return _cache.get(key)
 
 
-def _store(key: str, value: tuple[int, int]) -> None:
+def _store(key: str, value: tuple[int, ...]) -> None:
     _cache[key] = value
     path = _cache_file()
     if path is not None:
@@ -172,6 +174,10 @@ def forward_plan(solver) -> tuple[int, int, int]:
     if not solver.two_step_applies():
         return 1, single[0], single[1]
     double = forward_tiling(solver, 2)
+    key = _key("plan", solver)
+    found = _lookup(key)
+    if found is not None:
+        return tuple(found)
     per_step = {}
     for time_block, shape in ((1, single), (2, double)):
         scratch = _Scratch(solver)
@@ -185,9 +191,9 @@ def forward_plan(solver) -> tuple[int, int, int]:
 
         with torch.cuda.device(solver.device):
             per_step[time_block] = _time(solver, launch, shape, _steps_per_sample(solver), 3)
-    if per_step[2] < per_step[1]:
-        return 2, double[0], double[1]
-    return 1, single[0], single[1]
+    plan = (2, *double) if per_step[2] < per_step[1] else (1, *single)
+    _store(key, plan)
+    return plan
 
 
 def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tuple[int, int]:

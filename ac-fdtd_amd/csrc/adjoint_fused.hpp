@@ -252,25 +252,21 @@ __device__ __forceinline__ void adjoint_march_vec(
                 for (int r = 0; r < a.n_receivers; ++r) {
                     const int e = a.receiver_cell[2 * r + 1] - jk0;
                     if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
-                        const real gr = g.grad_recording[r * g.grad_recording_stride +
-                                                         (step - g.grad_recording_step0)];
-#pragma unroll
-                        for (int q = 0; q < V; ++q)
-                            if (q == e) lpc[q] = lpc[q] + gr;
+                        lane_add<real, V>(lpc, e,
+                                          g.grad_recording[r * g.grad_recording_stride +
+                                                           (step - g.grad_recording_step0)]);
                     }
                 }
                 for (int s = 0; s < a.n_sources; ++s) {
                     const int e = a.source_cell[2 * s + 1] - jk0;
                     if (a.source_cell[2 * s] == i && e >= 0 && e < V &&
                         step < a.source_length[s]) {
-#pragma unroll
-                        for (int q = 0; q < V; ++q) {
-                            if (q != e) continue;
-                            if (g.grad_signal)
-                                g.grad_signal[s * a.source_stride + step] = a.src * lpc[q];
-                            if (sponge_cell)
-                                injected[q] += a.src * a.source_signal[s * a.source_stride + step];
-                        }
+                        if (g.grad_signal)
+                            g.grad_signal[s * a.source_stride + step] =
+                                a.src * lane_get<real, V>(lpc, e);
+                        if (sponge_cell)
+                            lane_add<real, V>(injected, e,
+                                              a.src * a.source_signal[s * a.source_stride + step]);
                     }
                 }
             }

@@ -392,6 +392,37 @@ __device__ __forceinline__ void vstore(real *__restrict__ p, const real (&in)[V]
     }
 }
 
+// x[e] and x[e] += add for a run-time e < V, through a vector value: indexing the array itself
+// with a run-time index would place it (and what the optimiser merges with it) in scratch
+// memory.
+template <typename real, int V>
+__device__ __forceinline__ real lane_get(const real (&x)[V], const int e) {
+    if constexpr (V == 1) {
+        (void)e;
+        return x[0];
+    } else {
+        typename VecOf<real, V>::type v;
+#pragma unroll
+        for (int q = 0; q < V; ++q) v[q] = x[q];
+        return v[e];
+    }
+}
+
+template <typename real, int V>
+__device__ __forceinline__ void lane_add(real (&x)[V], const int e, const real add) {
+    if constexpr (V == 1) {
+        (void)e;
+        x[0] = x[0] + add;
+    } else {
+        typename VecOf<real, V>::type v;
+#pragma unroll
+        for (int q = 0; q < V; ++q) v[q] = x[q];
+        v[e] = v[e] + add;
+#pragma unroll
+        for (int q = 0; q < V; ++q) x[q] = v[q];
+    }
+}
+
 // Lane position in the flattened (y, z / V) plane used by the V-wide fused kernels: consecutive
 // lanes take consecutive V-cell groups of a row and run on into the next row, so no lane idles
 // whatever nz is. Returns false for lanes past the plane.

@@ -158,6 +158,26 @@ struct PlaneState {
     real psi[NR > 0 ? NR : 1][V];
 };
 
+// Element-wise copy (a struct assignment becomes a memcpy that keeps both states in scratch
+// memory instead of registers).
+template <typename real, int V, int NR>
+__device__ __forceinline__ void copy_state(PlaneState<real, V, NR> &to,
+                                           const PlaneState<real, V, NR> &from) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        to.p[e] = from.p[e];
+        to.vx_hi[e] = from.vx_hi[e];
+        to.vy_lo[e] = from.vy_lo[e];
+        to.vy_hi[e] = from.vy_hi[e];
+        to.vz[e] = from.vz[e];
+    }
+    to.vz[V] = from.vz[V];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) to.psi[r][e] = from.psi[r][e];
+}
+
 // One step of the lane's V cells in plane i, the arithmetic of fused_march_vec's plane body:
 // from the pressures p_c (plane i), p_up (plane i + 1), q_lo / q_hi (rows j -/+ 1), pz_lo /
 // pz_hi (cells k0 - 1 and k0 + V), the new velocity on the lower x face, the old velocities on
@@ -248,10 +268,7 @@ __device__ __forceinline__ void advance_cells(
         for (int s = 0; s < a.n_sources; ++s) {
             const int e = a.source_cell[2 * s + 1] - c.jk0;
             if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
-                const real add = a.src * a.source_signal[s * a.source_stride + step];
-#pragma unroll
-                for (int q = 0; q < V; ++q)
-                    if (q == e) o.p[q] = o.p[q] + add;
+                lane_add<real, V>(o.p, e, a.src * a.source_signal[s * a.source_stride + step]);
             }
         }
     }
@@ -268,9 +285,7 @@ __device__ __forceinline__ void record_cells(const SweepArgs<real> &a, const Lan
     for (int r = 0; r < a.n_receivers; ++r) {
         const int e = a.receiver_cell[2 * r + 1] - c.jk0;
         if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
-#pragma unroll
-            for (int q = 0; q < V; ++q)
-                if (q == e) a.recording[r * a.recording_stride + column] = p[q];
+            a.recording[r * a.recording_stride + column] = lane_get<real, V>(p, e);
         }
     }
 }
@@ -297,15 +312,17 @@ __device__ __forceinline__ void fused2_march(
     LaneConsts<real, V> c;
     if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, c);
 
-    auto slot = [&](const int plane, const int lane) { return lds + ((plane % 3) * T + lane) * V; };
-    auto share = [&](const int plane, const real (&p)[V]) {
+    auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
+        return lds + ((plane % 3) * T + lane) * V;
+    };
+    auto share = [&](const int plane, const real (&p)[V]) __attribute__((always_inline)) {
         real *d = slot(plane, t);
         vstore<real, V>(d, p);
     };
 
     // First step: p^n of the plane being advanced, and the new lower x face velocity.
     real pn_c[V], s1_vx_lo[V];
-    auto first_face = [&](const int s) {
+    auto first_face = [&](const int s) __attribute__((always_inline)) {
         real v[V], prev[V];
         vload<real, V>(p_in + L.cell(s), pn_c);
         vload<real, V>(vx_in + L.cell(s), v);
@@ -319,7 +336,7 @@ __device__ __forceinline__ void fused2_march(
     };
     // Step 1 of plane s from global memory into o (p^n of plane s in pn_c, new lower face in
     // s1_vx_lo); then carry plane s + 1's p^n and lower face.
-    auto step1 = [&](const int s, PlaneState<real, V, NR> &o, auto xwall_tag) {
+    auto step1 = [&](const int s, PlaneState<real, V, NR> &o, auto xwall_tag) __attribute__((always_inline)) {
         constexpr bool XWALL = decltype(xwall_tag)::value;
         const int64_t cc = L.cell(s);
         const bool up_interior = !XWALL || s + 1 < nx;
@@ -348,7 +365,7 @@ __device__ __forceinline__ void fused2_march(
             s1_vx_lo[e] = o.vx_hi[e];
         }
     };
-    auto step1_at = [&](const int s, PlaneState<real, V, NR> &o) {
+    auto step1_at = [&](const int s, PlaneState<real, V, NR> &o) __attribute__((always_inline)) {
         if (s == 0 || s == nx - 1) step1(s, o, Wall{});
         else step1(s, o, Interior{});
     };
@@ -357,7 +374,7 @@ __device__ __forceinline__ void fused2_march(
     // plane i + 1), the lower face's new velocity s2_vx_lo and the neighbours in LDS.
     real s2_vx_lo[V];
     auto step2 = [&](const int i, PlaneState<real, V, NR> &A, const real (&p_next)[V],
-                     auto xwall_tag) {
+                     auto xwall_tag) __attribute__((always_inline)) {
         constexpr bool XWALL = decltype(xwall_tag)::value;
         const int64_t cc = L.cell(i);
         real q_lo[V], q_hi[V];
@@ -431,10 +448,15 @@ __device__ __forceinline__ void fused2_march(
         }
         __syncthreads();
         if (owner) {
-            if (i == 0 || i == nx - 1) step2(i, A, more ? B.p : A.p, Wall{});
-            else step2(i, A, more ? B.p : A.p, Interior{});
+            // Selected by value: a select between the two arrays' addresses would keep both
+            // plane states in scratch memory.
+            real p_next[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) p_next[e] = more ? B.p[e] : A.p[e];
+            if (i == 0 || i == nx - 1) step2(i, A, p_next, Wall{});
+            else step2(i, A, p_next, Interior{});
         }
-        if (valid && more) A = B;
+        if (valid && more) copy_state(A, B);
     }
 }
 
