@@ -167,8 +167,6 @@ class RoomResponse:
             relaxation_strengths=delta,
             wall_table_always=wall_table_always,
             autotune=self.autotune,
-            # A forward pass that writes the boundary tape runs one step per sweep.
-            time_block=1 if wall_table_always else 2,
         )
         signals = self.signals if signal is None else list(
             torch.as_tensor(signal).detach().cpu().double().numpy()

@@ -319,10 +319,10 @@ class AcousticFDTD:
         return {kind: sum(v) / len(v) for kind, v in totals.items()}
 
     def two_step_applies(self) -> bool:
-        """Whether :meth:`run` advances two steps per sweep: ``time_block=2``, no boundary tape,
-        nz a multiple of the lane width (4 cells fp32, 2 fp64) and one halo row on each side of
-        a 64-lane workgroup within 512 lanes (the library's own test, restated)."""
-        if self.time_block != 2 or not self.fused or self._wall_tape is not None:
+        """Whether :meth:`run` advances two steps per sweep: ``time_block=2``, nz a multiple of
+        the lane width (4 cells fp32, 2 fp64) and one halo row on each side of a 64-lane
+        workgroup within 512 lanes (the library's own test, restated)."""
+        if self.time_block != 2 or not self.fused:
             return False
         if self.coefficients.n_relax and self._psi_swap is None:
             return False

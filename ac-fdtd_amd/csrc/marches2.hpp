@@ -10,6 +10,8 @@
 // row (nz / V lane groups) away, so the workgroup also runs step 1, without storing anything,
 // on one row of halo lanes on each side, and on one halo plane at each end of its chunk.
 //
+// Owner lanes also write the boundary tape of both steps when one is attached.
+//
 // Every cell's arithmetic is that of fused_march_vec term by term, and a halo cell's first
 // step reads the same unmodified inputs as its owner's, so p^(n+1), v^(n+1/2) and psi^(n+1)
 // are the same bits in every lane that computes them, and the sweep is bit-identical to two
@@ -53,8 +55,7 @@ struct Tiling2 {
 
 // Whether acfdtd_hip_run may use the two-step sweep for this descriptor.
 inline bool fused2_applies(const acfdtd_hip_desc &d) {
-    return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr) &&
-           d.wall_tape == nullptr;
+    return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr);
 }
 
 // Shape of the two-step sweep: tile_j wavefronts of own lanes (fewer if the halo rows would
@@ -182,14 +183,16 @@ __device__ __forceinline__ void copy_state(PlaneState<real, V, NR> &to,
 // from the pressures p_c (plane i), p_up (plane i + 1), q_lo / q_hi (rows j -/+ 1), pz_lo /
 // pz_hi (cells k0 - 1 and k0 + V), the new velocity on the lower x face, the old velocities on
 // the other faces and the relaxation states (advanced in place), produce the new velocities,
-// the new pressure after walls, sponge and sources of `step`.
+// the new pressure after walls, sponge and sources of `step`, and the boundary tape's value
+// p_tmp + p_old (the pressure before the wall closed form plus the old pressure).
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool XWALL>
 __device__ __forceinline__ void advance_cells(
     const SweepArgs<real> &a, const LaneConsts<real, V> &c, const int i, const int64_t step,
     const real (&p_c)[V], const real (&p_up)[V], const real (&q_lo)[V], const real (&q_hi)[V],
     const real pz_lo, const real pz_hi, const real (&vx_lo)[V], const real (&vx_v)[V],
     const real (&vy_lo_in)[V], const real (&vy_hi_in)[V], const real (&vz_v)[V],
-    const real vz_top, real (&states)[NR > 0 ? NR : 1][V], PlaneState<real, V, NR> &o) {
+    const real vz_top, real (&states)[NR > 0 ? NR : 1][V], PlaneState<real, V, NR> &o,
+    real (&tape_value)[V]) {
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
@@ -247,6 +250,7 @@ __device__ __forceinline__ void advance_cells(
             if (NR > 0) pn = pn + div * a.relax_scale;
             pn = pn - div * a.pres;
         }
+        tape_value[e] = pn + p_old;
         if (WALLS) {
             bool is_wall = c.wall_in[e];
             real fu = c.fu_in[e], fp = c.fp_in[e];
@@ -276,6 +280,23 @@ __device__ __forceinline__ void advance_cells(
     for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int e = 0; e < V; ++e) o.psi[r][e] = states[r][e];
+}
+
+// Boundary tape of `step` for the lane's wall cells in plane i (as fused_march_vec writes it).
+template <typename real, int V, bool XWALL>
+__device__ __forceinline__ void tape_cells(const SweepArgs<real> &a, const LaneConsts<real, V> &c,
+                                           const int i, const int64_t step,
+                                           const real (&tape_value)[V]) {
+    if (!(XWALL || c.yz_boundary)) return;
+    real *row = a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride;
+    const int cx = XWALL ? wall_class(i, a.nx) : 0;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int code = c.code_yz[e] | cx;
+        if (code)
+            tape_store(row, TapePlanes(a.nx, a.ny, a.nz), a.nx, a.ny, a.nz, i, c.j, c.k0 + e, code,
+                       tape_value[e]);
+    }
 }
 
 template <typename real, int V>
@@ -356,9 +377,12 @@ __device__ __forceinline__ void fused2_march(
         real states[NR > 0 ? NR : 1][V];
 #pragma unroll
         for (int r = 0; r < NR; ++r) vload<real, V>(psi_in + (int64_t)r * n_cells + cc, states[r]);
+        real tape_value[V];
         advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
             a, c, s, step, pn_c, p_up, q_lo, q_hi, pz_lo, pz_hi, s1_vx_lo, vx_v, vy_lo_in,
-            vy_hi_in, vz_v, vz_top, states, o);
+            vy_hi_in, vz_v, vz_top, states, o, tape_value);
+        if (a.wall_tape && owner && s >= i0 && s < i1)
+            tape_cells<real, V, XWALL>(a, c, s, step, tape_value);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             pn_c[e] = p_up[e];
@@ -386,9 +410,11 @@ __device__ __forceinline__ void fused2_march(
 #pragma unroll
         for (int e = 0; e < V; ++e) vz_v[e] = A.vz[e];
         PlaneState<real, V, NR> o;
+        real tape_value[V];
         advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
             a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi, A.vy_lo,
-            A.vy_hi, vz_v, A.vz[V], A.psi, o);
+            A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
+        if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, tape_value);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
         vstore<real, V>(vy_out + cy, o.vy_lo);

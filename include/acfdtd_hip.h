@@ -118,9 +118,9 @@ typedef struct acfdtd_hip_desc {
      * that advances two steps reads p, v, psi from one set and writes them to the other (its
      * halo cells read their neighbours' step-n states), in pairs A -> B -> A, so four steps
      * move (8 + 2 n_relax) elements per cell twice instead of four times. Used when time_block
-     * is 2, the four swap buffers are given, psi_swap is given (or n_relax is 0), no boundary
-     * tape is attached and nz is a multiple of the lane width (4 cells fp32, 2 fp64); the
-     * remaining steps run one per sweep. Results are bit-identical either way. */
+     * is 2, the four swap buffers are given, psi_swap is given (or n_relax is 0) and nz is a
+     * multiple of the lane width (4 cells fp32, 2 fp64); the remaining steps run one per
+     * sweep. Results (boundary tape included) are bit-identical either way. */
     void *psi_swap;
 } acfdtd_hip_desc;
 

@@ -90,8 +90,9 @@ def test_two_step_compensated_fp32():
     _assert_same(one, two)
 
 
-def test_two_step_falls_back_where_it_does_not_apply():
-    """nz not a multiple of the lane width, or a boundary tape attached: single steps."""
+def test_two_step_falls_back_and_writes_the_tape():
+    """nz not a multiple of the lane width runs single steps; with a boundary tape attached the
+    two-step sweep writes the same tape rows as single steps."""
     shape = (10, 9, 10)  # fp32 lanes of 4 do not divide nz = 10
     one = _solver(shape, np.float32, CONFIGS["all"], 1, 0, 0)
     two = _solver(shape, np.float32, CONFIGS["all"], 2, 0, 0)
@@ -108,3 +109,23 @@ def test_two_step_falls_back_where_it_does_not_apply():
     two.run(9)
     _assert_same(one, two)
     assert torch.equal(tapes[0], tapes[1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_two_step_boundary_tape_is_bitwise_single_steps(dtype, case):
+    """Owner lanes write the tape rows of both steps of a sweep; halo lanes and halo planes
+    write nothing."""
+    shape, tile_j, chunk_x = CASES[case]
+    one = _solver(shape, dtype, CONFIGS["all"], 1, tile_j, chunk_x)
+    two = _solver(shape, dtype, CONFIGS["all"], 2, tile_j, chunk_x)
+    tapes = [torch.full((17, s.wall_tape_stride()), -7.0, dtype=s.dtype, device=s.device)
+             for s in (one, two)]
+    one.attach_wall_tape(tapes[0], 0)
+    two.attach_wall_tape(tapes[1], 0)
+    one.run(17)
+    two.run(17)
+    _assert_same(one, two)
+    assert torch.equal(tapes[0], tapes[1])
+    stride = one.wall_tape_stride()
+    assert not torch.any(tapes[1][:, :stride] == -7.0)
