@@ -319,16 +319,15 @@ class AcousticFDTD:
         return {kind: sum(v) / len(v) for kind, v in totals.items()}
 
     def two_step_applies(self) -> bool:
-        """Whether :meth:`run` advances two steps per sweep: ``time_block=2``, nz a multiple of
-        the lane width (4 cells fp32, 2 fp64) and one halo row on each side of a 64-lane
-        workgroup within 512 lanes (the library's own test, restated)."""
+        """Whether :meth:`run` advances two steps per sweep: ``time_block=2`` on the fused path
+        and nz a multiple of the lane width, 4 cells fp32 and 2 fp64 (the library's own test,
+        restated)."""
         if self.time_block != 2 or not self.fused:
             return False
         if self.coefficients.n_relax and self._psi_swap is None:
             return False
         vec = 4 if self.np_dtype.itemsize == 4 else 2
-        nz = self.grid.nz
-        return nz % vec == 0 and 64 + 2 * (nz // vec) <= 512
+        return self.grid.nz % vec == 0
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)

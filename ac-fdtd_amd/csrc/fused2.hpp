@@ -31,14 +31,19 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     unsigned b = blockIdx.x;
     b = (b & 7u) * (n >> 3) + (b >> 3);  // contiguous tiles per XCD, as decode_tile
     if (b >= (unsigned)t2.tiles) return;  // whole workgroup, before any barrier
-    const int jb = (int)(b % (unsigned)t2.nblk);
+    const int blk = (int)(b % (unsigned)t2.nblk);
     const int xc = (int)(b / (unsigned)t2.nblk);
+    const int bg = blk % t2.nblk_g, bj = blk / t2.nblk_g;
     const int t = (int)threadIdx.x;
-    const int64_t f = (int64_t)jb * t2.main - t2.groups + t;
-    const bool valid = t < t2.main + 2 * t2.groups && f >= 0 && f < (int64_t)a.ny * t2.groups;
-    const bool owner = valid && t >= t2.groups && t < t2.groups + t2.main;
-    const int j = valid ? (int)(f / t2.groups) : 0;
-    const int k0 = valid ? (int)(f - (int64_t)j * t2.groups) * V : 0;
+    const int r = t / t2.pitch - 1;             // block row, -1 and rows: halo
+    const int sl = t % t2.pitch - t2.zhalo;     // segment position, -1 and seg: halo
+    const int jj = bj * t2.rows + r;
+    const int gg = bg * t2.seg + sl;
+    const bool valid = t < (t2.rows + 2) * t2.pitch && jj >= 0 && jj < a.ny && gg >= 0 &&
+                       gg < t2.groups;
+    const bool owner = valid && r >= 0 && r < t2.rows && sl >= 0 && sl < t2.seg;
+    const int j = valid ? jj : 0;
+    const int k0 = valid ? gg * V : 0;
     const int i0 = xc * t2.chunk;
     const int i1 = min(i0 + t2.chunk, a.nx);
     fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED>(

@@ -2,13 +2,13 @@
 //
 // One sweep reads (p^n, v^(n-1/2), psi^n) from the `in` set and writes
 // (p^(n+2), v^(n+3/2), psi^(n+2)) to the `out` set, so each field crosses HBM once per two
-// steps. A workgroup owns `main` consecutive lane groups of the flattened (y, z / V) plane (V
-// z cells each, as in fused_march_vec) and marches a chunk of x planes. Along x it runs the
-// first step one plane ahead of the second: step 1 of plane i + 1, then step 2 of plane i,
-// which needs p^(n+1) of planes i - 1, i, i + 1 (registers of the same lane) and of its four
-// y and z neighbours in plane i (other lanes: exchanged through LDS). Its y neighbours lie a
-// row (nz / V lane groups) away, so the workgroup also runs step 1, without storing anything,
-// on one row of halo lanes on each side, and on one halo plane at each end of its chunk.
+// steps. A workgroup owns a block of rows (y) by lane groups (z, V cells per lane, as in
+// fused_march_vec) and marches a chunk of x planes. Along x it runs the first step one plane
+// ahead of the second: step 1 of plane i + 1, then step 2 of plane i, which needs p^(n+1) of
+// planes i - 1, i, i + 1 (registers of the same lane) and of its four y and z neighbours in
+// plane i (other lanes: exchanged through LDS). So the workgroup also runs step 1, without
+// storing anything, on a halo row above and below the block, on a halo lane group at each end
+// of a row segment shorter than the row, and on one halo plane at each end of its chunk.
 //
 // Owner lanes also write the boundary tape of both steps when one is attached.
 //
@@ -43,14 +43,21 @@ struct Fused2IO {
     real *p_out, *vx_out, *vy_out, *vz_out, *psi_out;
 };
 
-// Workgroup shape of the two-step sweep.
+// Workgroup shape of the two-step sweep: a block of `rows` rows (y) by `seg` lane groups of a
+// row (z, `vec` cells each), plus one halo row above and below and, when a segment is shorter
+// than the row, one halo lane group at each end. Thread t sits at block row t / pitch - 1 and
+// segment position t % pitch - zhalo, so its y neighbours are t -/+ pitch and its z
+// neighbours t -/+ 1.
 struct Tiling2 {
     int vec;      // z cells per lane (V)
-    int groups;   // lane groups per row (nz / V), also the halo on each side
-    int main;     // lane groups a workgroup advances (64 * rows)
-    int threads;  // main + 2 * groups, rounded up to whole wavefronts
+    int groups;   // lane groups per row (nz / V)
+    int rows;     // own rows per block (R)
+    int seg;      // own lane groups per block row (S)
+    int zhalo;    // 1 when a segment is shorter than the row, else 0
+    int pitch;    // S + 2 zhalo: threads per block row
+    int threads;  // (R + 2) * pitch, rounded up to whole wavefronts (<= kFused2Threads)
     int chunk;    // x planes per march
-    int nblk, nchunk, tiles, launched;
+    int nblk_g, nblk_j, nblk, nchunk, tiles, launched;
 };
 
 // Whether acfdtd_hip_run may use the two-step sweep for this descriptor.
@@ -58,23 +65,33 @@ inline bool fused2_applies(const acfdtd_hip_desc &d) {
     return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr);
 }
 
-// Shape of the two-step sweep: tile_j wavefronts of own lanes (fewer if the halo rows would
-// push the workgroup past kFused2Threads), chunk_x planes per march (default 8). False when
-// nz is not a multiple of the lane width or one row of halo on each side does not fit.
+// Shape of the two-step sweep: R = 2 tile_j own rows (default 6), whole rows when R + 2 rows of
+// them fit in kFused2Threads lanes, else row segments as long as fit; chunk_x planes per march
+// (default 8). False when nz is not a multiple of the lane width or no segment fits.
 inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
     if (d.nz % vec != 0) return false;
     t.vec = vec;
     t.groups = d.nz / vec;
-    int rows = d.tile_j > 0 ? d.tile_j : 4;
-    while (rows > 1 && 64 * rows + 2 * t.groups > kFused2Threads) --rows;
-    t.main = 64 * rows;
-    if (t.main + 2 * t.groups > kFused2Threads) return false;
-    t.threads = ceil_div(t.main + 2 * t.groups, 64) * 64;
+    t.rows = d.tile_j > 0 ? 2 * d.tile_j : 6;
+    if ((t.rows + 2) * t.groups <= kFused2Threads) {
+        t.seg = t.groups;
+        t.zhalo = 0;
+        t.pitch = t.groups;
+    } else {
+        t.pitch = kFused2Threads / (t.rows + 2);
+        t.seg = t.pitch - 2;
+        t.zhalo = 1;
+        if (t.seg < 1) return false;
+    }
+    t.threads = ceil_div((int64_t)(t.rows + 2) * t.pitch, 64) * 64;
     t.chunk = d.chunk_x > 0 ? d.chunk_x : 8;
-    t.nblk = ceil_div((int64_t)d.ny * t.groups, t.main);
+    t.nblk_g = ceil_div(t.groups, t.seg);
+    t.nblk_j = ceil_div(d.ny, t.rows);
+    const int64_t nblk = (int64_t)t.nblk_g * t.nblk_j;
     t.nchunk = ceil_div(d.nx, t.chunk);
-    const int64_t tiles = (int64_t)t.nblk * t.nchunk;
+    const int64_t tiles = nblk * t.nchunk;
     if (tiles > (int64_t)INT32_MAX - 8) return false;
+    t.nblk = (int)nblk;
     t.tiles = (int)tiles;
     t.launched = ceil_div(tiles, 8) * 8;
     return true;
@@ -329,7 +346,7 @@ __device__ __forceinline__ void fused2_march(
     const Layout L(nx, a.ny, nz, j, k0);
     const int64_t n_cells = L.plane * nx;
     const real vel = a.vel;
-    const int G = t2.groups, T = t2.threads;
+    const int G = t2.pitch, T = t2.threads;  // G: lanes between y neighbours
     LaneConsts<real, V> c;
     if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, c);
 

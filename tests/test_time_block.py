@@ -20,8 +20,9 @@ from ac_fdtd_amd import AbsorbingLayer, AirAbsorption, Grid, WallAdmittances
 
 pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("require_gpu")]
 
-# (shape, tile_j, chunk_x): lane groups per row nz / V, rows split across workgroups when
-# ny * nz / V exceeds 64 * tile_j.
+# (shape, tile_j, chunk_x): blocks of 2 tile_j rows; whole rows of nz / V lane groups when
+# (2 tile_j + 2) rows fit in 512 lanes, else row segments with a halo lane group at each end
+# (the last two cases, and nz = 512 in fp32 / fp64).
 CASES = [
     ((9, 11, 12), 4, 8),
     ((13, 40, 64), 1, 3),
@@ -29,6 +30,8 @@ CASES = [
     ((21, 17, 32), 8, 5),
     ((1, 6, 8), 1, 1),
     ((34, 30, 128), 2, 4),
+    ((9, 13, 512), 2, 4),
+    ((11, 37, 128), 8, 4),
 ]
 
 
