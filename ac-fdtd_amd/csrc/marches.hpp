@@ -1,8 +1,8 @@
 // Device bodies of the time step, shared by the single-room, fused and batched kernels.
 //
 // Each body marches one lane's column of cells along x over [i0, i1): velocity_march and
-// pressure_march are the two in-place sweeps, fused_march the one-sweep step from an input set
-// of fields to an output set, fused_march_vec the same with V z-cells per lane. Pointers are
+// pressure_march are the two in-place sweeps, fused_march_vec the one-sweep step from an input
+// set of fields to an output set with V z-cells per lane. Pointers are
 // passed as restrict-qualified arguments of inlined functions, so the compiler may issue the
 // loads of the next planes before the stores of the current one retire.
 #pragma once
@@ -206,154 +206,11 @@ __device__ __forceinline__ real damp3(real v, real f1, real f2, real f3) {
     return v;
 }
 
-template <typename real, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
-__device__ __forceinline__ void fused_march(
-    const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k,
-    const int64_t step, const int64_t column, const real *__restrict__ p_in,
-    const real *__restrict__ vx_in, const real *__restrict__ vy_in,
-    const real *__restrict__ vz_in, real *__restrict__ p_out, real *__restrict__ vx_out,
-    real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi,
-    const real *__restrict__ wall_table, const real *__restrict__ cell_x,
-    const real *__restrict__ face_x, const real *__restrict__ cell_y,
-    const real *__restrict__ face_y, const real *__restrict__ cell_z,
-    const real *__restrict__ face_z, real *__restrict__ recording) {
-    const Layout L(a.nx, a.ny, a.nz, j, k);
-    const int64_t n_cells = L.plane * a.nx;
-    const int jk = j * a.nz + k;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const real vel = a.vel;
-    const int code_yz = WALLS ? (wall_class(j, ny) << 2) | (wall_class(k, nz) << 4) : 0;
-    real fyc = 1, fzc = 1, fyf_lo = 1, fyf_hi = 1, fzf_lo = 1, fzf_hi = 1;
-    if (LAYER) {
-        fyc = cell_y[j];
-        fzc = cell_z[k];
-        fyf_lo = face_y[j];
-        fyf_hi = face_y[j + 1];
-        fzf_lo = face_z[k];
-        fzf_hi = face_z[k + 1];
-    }
-
-    // New vx on face i0, the lower face of the first cell of the march.
-    real p_c = p_in[L.cell(i0)];
-    real vx_lo;
-    {
-        const real fxf = LAYER ? face_x[i0] : real(1);
-        real v = vx_in[L.cell(i0)];
-        if (i0 > 0) v = v - vel * (p_c - p_in[L.cell(i0 - 1)]);
-        vx_lo = damp3<real, LAYER>(v, fxf, fyc, fzc);
-        vx_out[L.cell(i0)] = vx_lo;
-    }
-
-#pragma unroll kXUnroll
-    for (int i = i0; i < i1; ++i) {
-        const int64_t c = L.cell(i);
-        const real fxc = LAYER ? cell_x[i] : real(1);
-
-        // x: upper face i + 1 (interior, or the far wall when i is the last cell).
-        real vx_hi;
-        real p_up = 0;
-        {
-            const real fxf = LAYER ? face_x[i + 1] : real(1);
-            real v = vx_in[c + L.plane];
-            if (i + 1 < nx) {
-                p_up = p_in[c + L.plane];
-                v = v - vel * (p_up - p_c);
-            }
-            vx_hi = damp3<real, LAYER>(v, fxf, fyc, fzc);
-            if (i + 1 < i1 || i + 1 == nx) vx_out[c + L.plane] = vx_hi;
-        }
-        // y: lower face j (owned), upper face j + 1 (written only for the far wall).
-        const int64_t cy = L.face_y(i);
-        real vy_lo, vy_hi;
-        {
-            real v = vy_in[cy];
-            if (j > 0) v = v - vel * (p_c - p_in[c - nz]);
-            vy_lo = damp3<real, LAYER>(v, fxc, fyf_lo, fzc);
-            vy_out[cy] = vy_lo;
-            real w = vy_in[cy + nz];
-            if (j + 1 < ny) w = w - vel * (p_in[c + nz] - p_c);
-            vy_hi = damp3<real, LAYER>(w, fxc, fyf_hi, fzc);
-            if (j + 1 == ny) vy_out[cy + nz] = vy_hi;
-        }
-        // z: lower face k (owned), upper face k + 1 (written only for the far wall).
-        const int64_t cz = L.face_z(i);
-        real vz_lo, vz_hi;
-        {
-            real v = vz_in[cz];
-            if (k > 0) v = v - vel * (p_c - p_in[c - 1]);
-            vz_lo = damp3<real, LAYER>(v, fxc, fyc, fzf_lo);
-            vz_out[cz] = vz_lo;
-            real w = vz_in[cz + 1];
-            if (k + 1 < nz) w = w - vel * (p_in[c + 1] - p_c);
-            vz_hi = damp3<real, LAYER>(w, fxc, fyc, fzf_hi);
-            if (k + 1 == nz) vz_out[cz + 1] = vz_hi;
-        }
-
-        const real div = ((vx_hi - vx_lo) + (vy_hi - vy_lo)) + (vz_hi - vz_lo);
-        const real p_old = p_c;
-        real pn;
-        if (COMPENSATED) {
-            real s = p_old, e = 0;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                real *state = psi + (int64_t)r * n_cells + c;
-                const real target = div * a.gain[r];
-                const real offset = *state - target;
-                *state = offset * a.decay[r] + target;
-                two_sum_acc(s, offset * a.mean[r], e);
-            }
-            if (NR > 0) two_sum_acc(s, div * a.relax_scale, e);
-            const real prod = div * a.pres;
-            const real prod_err = fma_real(div, a.pres, -prod);
-            two_sum_acc(s, -prod, e);
-            pn = s + (e - prod_err);
-        } else {
-            pn = p_old;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                real *state = psi + (int64_t)r * n_cells + c;
-                const real target = div * a.gain[r];
-                const real offset = *state - target;
-                *state = offset * a.decay[r] + target;
-                pn = pn + offset * a.mean[r];
-            }
-            if (NR > 0) pn = pn + div * a.relax_scale;
-            pn = pn - div * a.pres;
-        }
-        if (a.wall_tape) {
-            const int code = (wall_class(j, ny) << 2) | (wall_class(k, nz) << 4) |
-                             wall_class(i, nx);
-            if (code)
-                tape_store(a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride,
-                           TapePlanes(nx, ny, nz), nx, ny, nz, i, j, k, code, pn + p_old);
-        }
-        if (WALLS) {
-            const int code = code_yz | wall_class(i, nx);
-            if ((a.wall_mask >> code) & 1ull)
-                pn = pn * wall_table[code] - p_old * wall_table[ACFDTD_HIP_WALL_CODES + code];
-        }
-        if (LAYER) pn = ((pn * fxc) * fyc) * fzc;
-        for (int s = 0; s < a.n_sources; ++s) {
-            if (a.source_cell[2 * s] == i && a.source_cell[2 * s + 1] == jk &&
-                step < a.source_length[s])
-                pn = pn + a.src * a.source_signal[s * a.source_stride + step];
-        }
-        p_out[c] = pn;
-        for (int r = 0; r < a.n_receivers; ++r) {
-            if (a.receiver_cell[2 * r] == i && a.receiver_cell[2 * r + 1] == jk)
-                recording[r * a.recording_stride + column] = pn;
-        }
-        vx_lo = vx_hi;
-        p_c = p_up;
-    }
-}
-
 // ----------------------------------------------------------- fused sweep, V cells per lane --
 //
-// The same step as fused_march with each lane owning V consecutive z cells (V = 1, 2 or 4), so
-// every field access is a V-wide vector access (dwordx2 / dwordx4): fp32 with one cell per
-// lane issues too few bytes per memory instruction to keep HBM busy. The arithmetic per cell
-// is exactly the scalar kernel's, so the bits are the same. Used when nz is a multiple of V:
+// Each lane owns V consecutive z cells (V = 1, 2 or 4), so every field access is a V-wide
+// vector access (dwordx2 / dwordx4): fp32 with one cell per lane issues too few bytes per
+// memory instruction to keep HBM busy. Used when nz is a multiple of V:
 // then each lane's cells lie in one row and every vector is V-element aligned in all fields
 // except vz (rows of nz + 1), whose accesses only need the 4-byte alignment gfx950 requires.
 //
@@ -494,15 +351,22 @@ __device__ __forceinline__ void fused_march_vec(
             }
     }
 
-    // Does any source or receiver lie in this lane's cells (in some plane)?
-    bool special = false;
+    // The planes [ev_lo, ev_hi] of the sources and receivers in this lane's cells (none when
+    // ev_lo > ev_hi): the per-plane scans of the tables run only there.
+    int ev_lo = nx, ev_hi = -1;
     for (int s = 0; s < a.n_sources; ++s) {
         const int e = a.source_cell[2 * s + 1] - jk0;
-        special = special || (e >= 0 && e < V);
+        if (e >= 0 && e < V) {
+            ev_lo = min(ev_lo, a.source_cell[2 * s]);
+            ev_hi = max(ev_hi, a.source_cell[2 * s]);
+        }
     }
     for (int r = 0; r < a.n_receivers; ++r) {
         const int e = a.receiver_cell[2 * r + 1] - jk0;
-        special = special || (e >= 0 && e < V);
+        if (e >= 0 && e < V) {
+            ev_lo = min(ev_lo, a.receiver_cell[2 * r]);
+            ev_hi = max(ev_hi, a.receiver_cell[2 * r]);
+        }
     }
 
     // New vx on face i0, the lower face of the first cell of the march.
@@ -642,7 +506,7 @@ __device__ __forceinline__ void fused_march_vec(
                                tape_value[e]);
             }
         }
-        if (special) {
+        if (i >= ev_lo && i <= ev_hi) {
             // Sources in insertion order, after the sponge (scheme.py:360-363).
             for (int s = 0; s < a.n_sources; ++s) {
                 const int e = a.source_cell[2 * s + 1] - jk0;
@@ -655,7 +519,7 @@ __device__ __forceinline__ void fused_march_vec(
             }
         }
         vstore<real, V>(p_out + c, p_new);
-        if (special) {
+        if (i >= ev_lo && i <= ev_hi) {
             for (int r = 0; r < a.n_receivers; ++r) {
                 const int e = a.receiver_cell[2 * r + 1] - jk0;
                 if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {

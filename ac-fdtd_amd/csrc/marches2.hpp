@@ -36,6 +36,10 @@ namespace acfdtd {
 
 // Largest workgroup of the two-step sweep (launch bound and LDS size).
 constexpr int kFused2Threads = 512;
+// Source and receiver tables up to these sizes are copied to LDS (larger ones are read from
+// global memory).
+constexpr int kFused2Sources = 64;
+constexpr int kFused2Receivers = 256;
 
 template <typename real>
 struct Fused2IO {
@@ -100,6 +104,11 @@ inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
 // Per-lane constants of the step: position, neighbour existence, sponge factors, wall classes.
 template <typename real, int V>
 struct LaneConsts {
+    // Source and receiver tables (LDS copies when they fit, see fused2_step).
+    const int32_t *src_cell, *rx_cell;
+    const int64_t *src_len;
+    const real *src_now;  // the sources' samples of the sweep's two steps, two per source
+    int ev_lo, ev_hi;     // planes of the lane's sources and receivers (empty: lo > hi)
     int j, k0, jk0;
     bool has_ylo, has_yhi, has_zlo, has_zhi, yz_boundary, special;
     int64_t dy_lo, dy_hi, dz_lo, dz_hi;
@@ -111,8 +120,14 @@ struct LaneConsts {
 
 template <typename real, int V, bool WALLS, bool LAYER>
 __device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int j, const int k0,
+                                            const int32_t *src_cell, const int64_t *src_len,
+                                            const real *src_now, const int32_t *rx_cell,
                                             LaneConsts<real, V> &c) {
     const int ny = a.ny, nz = a.nz;
+    c.src_cell = src_cell;
+    c.src_len = src_len;
+    c.src_now = src_now;
+    c.rx_cell = rx_cell;
     c.j = j;
     c.k0 = k0;
     c.jk0 = j * nz + k0;
@@ -156,16 +171,30 @@ __device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int 
                 c.fp_in[e] = a.wall_table[ACFDTD_HIP_WALL_CODES + c.code_yz[e]];
             }
     }
-    bool special = false;
+    int lo = a.nx, hi = -1;
     for (int s = 0; s < a.n_sources; ++s) {
-        const int e = a.source_cell[2 * s + 1] - c.jk0;
-        special = special || (e >= 0 && e < V);
+        const int e = c.src_cell[2 * s + 1] - c.jk0;
+        if (e >= 0 && e < V) {
+            lo = min(lo, c.src_cell[2 * s]);
+            hi = max(hi, c.src_cell[2 * s]);
+        }
     }
     for (int r = 0; r < a.n_receivers; ++r) {
-        const int e = a.receiver_cell[2 * r + 1] - c.jk0;
-        special = special || (e >= 0 && e < V);
+        const int e = c.rx_cell[2 * r + 1] - c.jk0;
+        if (e >= 0 && e < V) {
+            lo = min(lo, c.rx_cell[2 * r]);
+            hi = max(hi, c.rx_cell[2 * r]);
+        }
     }
-    c.special = special;
+    c.ev_lo = lo;
+    c.ev_hi = hi;
+    c.special = hi >= 0;
+}
+
+// Whether plane i of the lane holds a source or a receiver.
+template <typename real, int V>
+__device__ __forceinline__ bool has_events(const LaneConsts<real, V> &c, const int i) {
+    return c.special && i >= c.ev_lo && i <= c.ev_hi;
 }
 
 // What one step leaves in a lane for one plane: the new pressure, the new velocity on the
@@ -284,12 +313,14 @@ __device__ __forceinline__ void advance_cells(
         if (LAYER) pn = ((pn * fxc) * c.fyc) * c.fzc[e];
         o.p[e] = pn;
     }
-    if (c.special) {
+    if (has_events(c, i)) {
         // Sources in insertion order, after the sponge (scheme.py:360-363).
         for (int s = 0; s < a.n_sources; ++s) {
-            const int e = a.source_cell[2 * s + 1] - c.jk0;
-            if (a.source_cell[2 * s] == i && e >= 0 && e < V && step < a.source_length[s]) {
-                lane_add<real, V>(o.p, e, a.src * a.source_signal[s * a.source_stride + step]);
+            const int e = c.src_cell[2 * s + 1] - c.jk0;
+            if (c.src_cell[2 * s] == i && e >= 0 && e < V && step < c.src_len[s]) {
+                const real sample = c.src_now ? c.src_now[2 * s + (step & 1)]
+                                              : a.source_signal[s * a.source_stride + step];
+                lane_add<real, V>(o.p, e, a.src * sample);
             }
         }
     }
@@ -321,8 +352,8 @@ __device__ __forceinline__ void record_cells(const SweepArgs<real> &a, const Lan
                                              const int i, const int64_t column,
                                              const real (&p)[V]) {
     for (int r = 0; r < a.n_receivers; ++r) {
-        const int e = a.receiver_cell[2 * r + 1] - c.jk0;
-        if (a.receiver_cell[2 * r] == i && e >= 0 && e < V) {
+        const int e = c.rx_cell[2 * r + 1] - c.jk0;
+        if (c.rx_cell[2 * r] == i && e >= 0 && e < V) {
             a.recording[r * a.recording_stride + column] = lane_get<real, V>(p, e);
         }
     }
@@ -339,7 +370,8 @@ __device__ __forceinline__ void fused2_march(
     const real *__restrict__ vy_in, const real *__restrict__ vz_in,
     const real *__restrict__ psi_in, real *__restrict__ p_out, real *__restrict__ vx_out,
     real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi_out,
-    real *__restrict__ lds) {
+    real *__restrict__ lds, const int32_t *src_cell, const int64_t *src_len,
+    const real *src_now, const int32_t *rx_cell) {
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
     const int nx = a.nx, nz = a.nz;
@@ -348,7 +380,7 @@ __device__ __forceinline__ void fused2_march(
     const real vel = a.vel;
     const int G = t2.pitch, T = t2.threads;  // G: lanes between y neighbours
     LaneConsts<real, V> c;
-    if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, c);
+    if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, src_cell, src_len, src_now, rx_cell, c);
 
     auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
         return lds + ((plane % 3) * T + lane) * V;
@@ -446,7 +478,7 @@ __device__ __forceinline__ void fused2_march(
 #pragma unroll
         for (int r = 0; r < NR; ++r) vstore<real, V>(psi_out + (int64_t)r * n_cells + cc, o.psi[r]);
         vstore<real, V>(p_out + cc, o.p);
-        if (c.special) record_cells<real, V>(a, c, i, column + 1, o.p);
+        if (has_events(c, i)) record_cells<real, V>(a, c, i, column + 1, o.p);
 #pragma unroll
         for (int e = 0; e < V; ++e) s2_vx_lo[e] = o.vx_hi[e];
     };
@@ -466,7 +498,7 @@ __device__ __forceinline__ void fused2_march(
 #pragma unroll
         for (int e = 0; e < V; ++e) v_half_lo[e] = s1_vx_lo[e];
         step1_at(i0, A);
-        if (owner && c.special) record_cells<real, V>(a, c, i0, column, A.p);
+        if (owner && has_events(c, i0)) record_cells<real, V>(a, c, i0, column, A.p);
         share(i0, A.p);
     }
     __syncthreads();
@@ -486,7 +518,8 @@ __device__ __forceinline__ void fused2_march(
         const bool more = i + 1 < nx;
         if (valid && more) {
             step1_at(i + 1, B);
-            if (owner && c.special && i + 1 < i1) record_cells<real, V>(a, c, i + 1, column, B.p);
+            if (owner && i + 1 < i1 && has_events(c, i + 1))
+                record_cells<real, V>(a, c, i + 1, column, B.p);
             share(i + 1, B.p);
         }
         __syncthreads();
