@@ -320,9 +320,12 @@ class AcousticFDTD:
 
     def two_step_applies(self) -> bool:
         """Whether :meth:`run` advances two steps per sweep: ``time_block=2`` on the fused path
-        and nz a multiple of the lane width, 4 cells fp32 and 2 fp64 (the library's own test,
+        and nz a multiple of the lane width, 4 cells fp32 and 2 fp64, with at most 64 sources
+        and 256 receivers, whose tables the sweep keeps in LDS (the library's own test,
         restated)."""
         if self.time_block != 2 or not self.fused:
+            return False
+        if len(self._sources) > 64 or len(self._receivers) > 256:
             return False
         if self.coefficients.n_relax and self._psi_swap is None:
             return False

@@ -29,6 +29,8 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     __shared__ real lds[3 * kFused2Threads * V];
     // Source and receiver tables, copied once: the lanes that hold a source or a receiver scan
     // them every plane, and with a barrier per plane the whole workgroup waits for those lanes.
+    // In LDS their reads wait for nothing else; a read from global (or generic) memory waits
+    // for every store the lane issued before it.
     __shared__ int32_t src_lds[2 * kFused2Sources];
     __shared__ int64_t len_lds[kFused2Sources];
     __shared__ real now_lds[2 * kFused2Sources];  // samples of steps `step` and `step` + 1
@@ -52,27 +54,24 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     const int k0 = valid ? gg * V : 0;
     const int i0 = xc * t2.chunk;
     const int i1 = min(i0 + t2.chunk, a.nx);
-    const bool src_fit = a.n_sources <= kFused2Sources, rx_fit = a.n_receivers <= kFused2Receivers;
-    if (src_fit)
-        for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
-            src_lds[2 * q] = a.source_cell[2 * q];
-            src_lds[2 * q + 1] = a.source_cell[2 * q + 1];
-            const int64_t len = a.source_length[q];
-            len_lds[q] = len;
-            // Indexed by step parity; a sample past the signal's end is never read.
-            for (int h = 0; h < 2; ++h) {
-                const int64_t n = step + h;
-                now_lds[2 * q + (int)(n & 1)] = n < len ? a.source_signal[q * a.source_stride + n] : real(0);
-            }
+    // fused2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
+    for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
+        src_lds[2 * q] = a.source_cell[2 * q];
+        src_lds[2 * q + 1] = a.source_cell[2 * q + 1];
+        const int64_t len = a.source_length[q];
+        len_lds[q] = len;
+        // Indexed by step parity; a sample past the signal's end is never read.
+        for (int h = 0; h < 2; ++h) {
+            const int64_t n = step + h;
+            now_lds[2 * q + (int)(n & 1)] = n < len ? a.source_signal[q * a.source_stride + n] : real(0);
         }
-    if (rx_fit)
-        for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
-    __syncthreads();
+    }
+    for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
+    lds_barrier();
     fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED>(
         a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
-        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds,
-        src_fit ? src_lds : a.source_cell, src_fit ? len_lds : a.source_length,
-        src_fit ? now_lds : nullptr, rx_fit ? rx_lds : a.receiver_cell);
+        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
+        len_lds, now_lds, rx_lds);
 }
 
 template <typename real>

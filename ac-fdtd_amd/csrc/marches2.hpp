@@ -47,6 +47,20 @@ struct Fused2IO {
     real *p_out, *vx_out, *vy_out, *vz_out, *psi_out;
 };
 
+// The workgroup barrier of the plane exchange. It orders LDS only: __syncthreads is also a
+// workgroup release fence on global memory, which makes every wave wait for its outstanding
+// global stores (the fields, the tape, the recording) at each plane, and no wave of a sweep
+// reads what another writes to global memory.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// s_waitcnt vmcnt(0): wait for every outstanding vector memory operation (gfx9 encoding:
+// vmcnt in bits 3:0 and 15:14, expcnt in 6:4, lgkmcnt in 11:8; the others at their maximum).
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Workgroup shape of the two-step sweep: a block of `rows` rows (y) by `seg` lane groups of a
 // row (z, `vec` cells each), plus one halo row above and below and, when a segment is shorter
 // than the row, one halo lane group at each end. Thread t sits at block row t / pitch - 1 and
@@ -62,11 +76,13 @@ struct Tiling2 {
     int threads;  // (R + 2) * pitch, rounded up to whole wavefronts (<= kFused2Threads)
     int chunk;    // x planes per march
     int nblk_g, nblk_j, nblk, nchunk, tiles, launched;
+    int tape_vec;  // boundary tape rows start V-aligned: x and y face rows as vector stores
 };
 
 // Whether acfdtd_hip_run may use the two-step sweep for this descriptor.
 inline bool fused2_applies(const acfdtd_hip_desc &d) {
-    return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr);
+    return d.time_block == 2 && has_swap(d) && (d.n_relax == 0 || d.psi_swap != nullptr) &&
+           d.n_sources <= kFused2Sources && d.n_receivers <= kFused2Receivers;
 }
 
 // Shape of the two-step sweep: R = 2 tile_j own rows (default 6), whole rows when R + 2 rows of
@@ -98,6 +114,9 @@ inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
     t.nblk = (int)nblk;
     t.tiles = (int)tiles;
     t.launched = ceil_div(tiles, 8) * 8;
+    t.tape_vec = d.wall_tape != nullptr &&
+                 reinterpret_cast<uintptr_t>(d.wall_tape) % ((size_t)vec * d.real_bytes) == 0 &&
+                 d.wall_tape_stride % vec == 0;
     return true;
 }
 
@@ -318,9 +337,7 @@ __device__ __forceinline__ void advance_cells(
         for (int s = 0; s < a.n_sources; ++s) {
             const int e = c.src_cell[2 * s + 1] - c.jk0;
             if (c.src_cell[2 * s] == i && e >= 0 && e < V && step < c.src_len[s]) {
-                const real sample = c.src_now ? c.src_now[2 * s + (step & 1)]
-                                              : a.source_signal[s * a.source_stride + step];
-                lane_add<real, V>(o.p, e, a.src * sample);
+                lane_add<real, V>(o.p, e, a.src * c.src_now[2 * s + (step & 1)]);
             }
         }
     }
@@ -330,21 +347,37 @@ __device__ __forceinline__ void advance_cells(
         for (int e = 0; e < V; ++e) o.psi[r][e] = states[r][e];
 }
 
-// Boundary tape of `step` for the lane's wall cells in plane i (as fused_march_vec writes it).
+// Boundary tape of `step` for the lane's wall cells in plane i: the values fused_march_vec
+// writes, cell for cell, with one test per face row instead of one per cell and face. A lane's
+// V cells share their x and y faces, so its x and y face rows take one vector store each when
+// the tape rows are V-aligned (nz, and so every face row offset, is a multiple of V); only
+// the first and last lane of a row hold z-face cells, one each.
 template <typename real, int V, bool XWALL>
 __device__ __forceinline__ void tape_cells(const SweepArgs<real> &a, const LaneConsts<real, V> &c,
-                                           const int i, const int64_t step,
+                                           const int i, const int64_t step, const bool vec_rows,
                                            const real (&tape_value)[V]) {
     if (!(XWALL || c.yz_boundary)) return;
     real *row = a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride;
-    const int cx = XWALL ? wall_class(i, a.nx) : 0;
+    const TapePlanes t(a.nx, a.ny, a.nz);
+    auto face_row = [&](real *dst) __attribute__((always_inline)) {
+        if (vec_rows) {
+            vstore<real, V>(dst, tape_value);
+        } else {
 #pragma unroll
-    for (int e = 0; e < V; ++e) {
-        const int code = c.code_yz[e] | cx;
-        if (code)
-            tape_store(row, TapePlanes(a.nx, a.ny, a.nz), a.nx, a.ny, a.nz, i, c.j, c.k0 + e, code,
-                       tape_value[e]);
+            for (int e = 0; e < V; ++e) dst[e] = tape_value[e];
+        }
+    };
+    if (XWALL) {
+        const int64_t yz = (int64_t)c.j * a.nz + c.k0;
+        if (i == 0) face_row(row + t.x_lo + yz);
+        if (i == a.nx - 1) face_row(row + t.x_hi + yz);
     }
+    const int64_t xz = (int64_t)i * a.nz + c.k0;
+    if (c.j == 0) face_row(row + t.y_lo + xz);
+    if (c.j == a.ny - 1) face_row(row + t.y_hi + xz);
+    const int64_t xy = (int64_t)i * a.ny + c.j;
+    if (c.k0 == 0) row[t.z_lo + xy] = tape_value[0];
+    if (c.k0 + V == a.nz) row[t.z_hi + xy] = tape_value[V - 1];
 }
 
 template <typename real, int V>
@@ -381,6 +414,11 @@ __device__ __forceinline__ void fused2_march(
     const int G = t2.pitch, T = t2.threads;  // G: lanes between y neighbours
     LaneConsts<real, V> c;
     if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, src_cell, src_len, src_now, rx_cell, c);
+    // The lane constants' loads complete here, on every path into the march. Lanes that skip
+    // the prologue would otherwise reach the loop with those loads counted as outstanding,
+    // and the compiler waits for them inside the loop with a memory count of zero, which on
+    // this counter also waits for every store issued before (the boundary tape's).
+    wait_memory();
 
     auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
         return lds + ((plane % 3) * T + lane) * V;
@@ -431,7 +469,7 @@ __device__ __forceinline__ void fused2_march(
             a, c, s, step, pn_c, p_up, q_lo, q_hi, pz_lo, pz_hi, s1_vx_lo, vx_v, vy_lo_in,
             vy_hi_in, vz_v, vz_top, states, o, tape_value);
         if (a.wall_tape && owner && s >= i0 && s < i1)
-            tape_cells<real, V, XWALL>(a, c, s, step, tape_value);
+            tape_cells<real, V, XWALL>(a, c, s, step, t2.tape_vec != 0, tape_value);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             pn_c[e] = p_up[e];
@@ -463,7 +501,7 @@ __device__ __forceinline__ void fused2_march(
         advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
             a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi, A.vy_lo,
             A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
-        if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, tape_value);
+        if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, t2.tape_vec != 0, tape_value);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
         vstore<real, V>(vy_out + cy, o.vy_lo);
@@ -501,7 +539,7 @@ __device__ __forceinline__ void fused2_march(
         if (owner && has_events(c, i0)) record_cells<real, V>(a, c, i0, column, A.p);
         share(i0, A.p);
     }
-    __syncthreads();
+    lds_barrier();
     if (owner) {
         // v^(n+3/2) on face i0, the lower face of the first cell (the prologue of
         // fused_march_vec with the first step's results as inputs).
@@ -522,7 +560,7 @@ __device__ __forceinline__ void fused2_march(
                 record_cells<real, V>(a, c, i + 1, column, B.p);
             share(i + 1, B.p);
         }
-        __syncthreads();
+        lds_barrier();
         if (owner) {
             // Selected by value: a select between the two arrays' addresses would keep both
             // plane states in scratch memory.

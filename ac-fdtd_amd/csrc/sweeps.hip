@@ -308,7 +308,8 @@ extern "C" int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t st
 }
 
 extern "C" int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz) {
-    return TapePlanes(nx, ny, nz).stride;
+    // Rounded up to whole 4-element groups, so that rows of a 16-byte aligned tape stay aligned.
+    return (TapePlanes(nx, ny, nz).stride + 3) / 4 * 4;
 }
 
 

@@ -108,7 +108,10 @@ typedef struct acfdtd_hip_desc {
      * elements each) receives p_tmp + p_old of every boundary cell, where p_tmp is the
      * pressure before the wall closed form, once per face plane the cell lies on. Planes in
      * row order: x-low, x-high (ny*nz, index j*nz+k), y-low, y-high (nx*nz, i*nz+k), z-low,
-     * z-high (nx*ny, i*ny+j); acfdtd_hip_wall_tape_stride() gives the row length. */
+     * z-high (nx*ny, i*ny+j), 2 (ny*nz + nx*nz + nx*ny) elements in all. Any stride of at
+     * least that is accepted; acfdtd_hip_wall_tape_stride() gives it rounded up to a multiple
+     * of 4, which with a 16-byte aligned tape lets the two-step sweep write x and y face rows
+     * as vector stores. */
     void *wall_tape;
     int64_t wall_tape_stride;
     int64_t wall_tape_step0;
@@ -182,7 +185,8 @@ int acfdtd_hip_batch_run(acfdtd_hip_batch *batch, int64_t step_offset, int64_t n
                          int64_t recording_column, void *stream);
 void acfdtd_hip_batch_destroy(acfdtd_hip_batch *batch);
 
-/* Elements per step row of the boundary tape: 2 (ny nz + nx nz + nx ny). */
+/* Stride of a boundary tape row: 2 (ny nz + nx nz + nx ny) elements rounded up to a multiple
+ * of 4. */
 int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
 
 /* Energy terms of the stored state, in fp64 on the device: out[0] = sum p^2,

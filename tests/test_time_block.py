@@ -114,15 +114,36 @@ def test_two_step_falls_back_and_writes_the_tape():
     assert torch.equal(tapes[0], tapes[1])
 
 
+def test_more_sources_than_the_sweep_holds_run_single_steps():
+    """The two-step sweep keeps the source and receiver tables in LDS (64 and 256 entries);
+    beyond them the solver runs single steps, with the same results."""
+    shape, tile_j, chunk_x = CASES[5]
+    one = _solver(shape, np.float32, CONFIGS["all"], 1, tile_j, chunk_x)
+    two = _solver(shape, np.float32, CONFIGS["all"], 2, tile_j, chunk_x)
+    assert two.two_step_applies()
+    rng = np.random.default_rng(5)
+    lengths = np.array(one.grid.lengths)
+    for _ in range(63):
+        where, signal = tuple(rng.uniform(0.05, 0.95, 3) * lengths), rng.standard_normal(12)
+        one.add_volume_source(where, signal)
+        two.add_volume_source(where, signal)
+    assert not two.two_step_applies()
+    one.run(11)
+    two.run(11)
+    _assert_same(one, two)
+
+
+@pytest.mark.parametrize("pad", [0, 1], ids=["aligned", "unaligned"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_two_step_boundary_tape_is_bitwise_single_steps(dtype, case):
+def test_two_step_boundary_tape_is_bitwise_single_steps(dtype, case, pad):
     """Owner lanes write the tape rows of both steps of a sweep; halo lanes and halo planes
-    write nothing."""
+    write nothing. With the library's stride the x and y face rows go out as vector stores;
+    one element more makes rows unaligned and takes the per-cell stores."""
     shape, tile_j, chunk_x = CASES[case]
     one = _solver(shape, dtype, CONFIGS["all"], 1, tile_j, chunk_x)
     two = _solver(shape, dtype, CONFIGS["all"], 2, tile_j, chunk_x)
-    tapes = [torch.full((17, s.wall_tape_stride()), -7.0, dtype=s.dtype, device=s.device)
+    tapes = [torch.full((17, s.wall_tape_stride() + pad), -7.0, dtype=s.dtype, device=s.device)
              for s in (one, two)]
     one.attach_wall_tape(tapes[0], 0)
     two.attach_wall_tape(tapes[1], 0)
@@ -130,5 +151,7 @@ def test_two_step_boundary_tape_is_bitwise_single_steps(dtype, case):
     two.run(17)
     _assert_same(one, two)
     assert torch.equal(tapes[0], tapes[1])
-    stride = one.wall_tape_stride()
+    nx, ny, nz = shape
+    stride = 2 * (ny * nz + nx * nz + nx * ny)  # the row's written length; the stride pads it
+    assert one.wall_tape_stride() >= stride and one.wall_tape_stride() % 4 == 0
     assert not torch.any(tapes[1][:, :stride] == -7.0)
