@@ -177,6 +177,33 @@ __device__ __forceinline__ void adjoint_march_vec(
 #pragma unroll
             for (int r = 0; r < NR; ++r) vload<real, V>(lpsi + (int64_t)r * n_cells + c, states[r]);
         }
+        // Boundary tape of step m - 1 for the lane's wall cells, read with the other loads so
+        // that waiting for it does not also wait for this plane's stores. Every face plane a
+        // cell lies on holds the same value: the x row on x-wall planes, else the y row for
+        // the first and last rows, else the z rows for the first and last cells of a row.
+        real tv[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) tv[e] = 0;
+        if (want_walls && (XWALL || yz_boundary)) {
+            {
+                const real *src = nullptr;
+                if (XWALL) src = tape_row + (i == 0 ? planes.x_lo : planes.x_hi) + (int64_t)j * nz + k0;
+                else if (!has_ylo || !has_yhi)
+                    src = tape_row + (!has_ylo ? planes.y_lo : planes.y_hi) + (int64_t)i * nz + k0;
+                if (src != nullptr) {
+                    if (a.wall_tape_rows16) {
+                        vload<real, V>(src, tv);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < V; ++e) tv[e] = src[e];
+                    }
+                } else {
+                    const int64_t xy = (int64_t)i * ny + j;
+                    if (!has_zhi) tv[V - 1] = tape_row[planes.z_hi + xy];
+                    if (!has_zlo) tv[0] = tape_row[planes.z_lo + xy];
+                }
+            }
+        }
 
         // New adjoint velocities on the six faces of each cell (adjoint_face_sweep).
         real tx[V], ty_lo[V], ty_hi[V], tz[V + 1];
@@ -307,15 +334,14 @@ __device__ __forceinline__ void adjoint_march_vec(
                     ltil = fu * lhat;
                     lp_new = -fp * lhat;
                     if (want_walls && (XWALL || yz_boundary)) {
-                        const double ga =
-                            -(double)lhat * (double)fu * (double)fu *
-                            (double)tape_load(tape_row, planes, ny, nz, i, j, k0 + e, code);
-                        if (code & 1) acc[kGradWall + 0] += ga;
-                        if (code & 2) acc[kGradWall + 1] += ga;
-                        if (code & 4) acc[kGradWall + 2] += ga;
-                        if (code & 8) acc[kGradWall + 3] += ga;
-                        if (code & 16) acc[kGradWall + 4] += ga;
-                        if (code & 32) acc[kGradWall + 5] += ga;
+                        // Added to each face the cell lies on, in face order.
+                        const double ga = -(double)lhat * (double)fu * (double)fu * (double)tv[e];
+                        if (XWALL && i == 0) acc[kGradWall + 0] += ga;
+                        if (XWALL && i == nx - 1) acc[kGradWall + 1] += ga;
+                        if (!has_ylo) acc[kGradWall + 2] += ga;
+                        if (!has_yhi) acc[kGradWall + 3] += ga;
+                        if (e == 0 && !has_zlo) acc[kGradWall + 4] += ga;
+                        if (e == V - 1 && !has_zhi) acc[kGradWall + 5] += ga;
                     }
                 }
                 lp_new = lp_new + ltil;

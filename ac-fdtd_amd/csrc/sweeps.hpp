@@ -136,6 +136,9 @@ struct SweepArgs {
     real *wall_tape;
     int64_t wall_tape_stride;
     int64_t wall_tape_step0;
+    // Tape rows start 16-byte aligned (aligned base, stride a multiple of 4 elements), so a
+    // lane's V cells of an x or y face row move as one vector access.
+    int wall_tape_rows16;
 };
 
 // Offsets of the six face planes in one row of the boundary tape, and the row length.
@@ -219,6 +222,9 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
     a.wall_tape = static_cast<real *>(d.wall_tape);
     a.wall_tape_stride = d.wall_tape_stride;
     a.wall_tape_step0 = d.wall_tape_step0;
+    a.wall_tape_rows16 = d.wall_tape != nullptr &&
+                         reinterpret_cast<uintptr_t>(d.wall_tape) % 16 == 0 &&
+                         d.wall_tape_stride % 4 == 0;
 }
 
 inline int validate(const acfdtd_hip_desc *d) {
