@@ -105,7 +105,7 @@ class AdjointDesc(_c.Structure):
         ("abi_version", _c.c_uint32),
         ("struct_size", _c.c_uint32),
         ("grads", _c.c_uint32),
-        ("reserved", _c.c_int32),
+        ("time_block", _c.c_int32),
         ("lp", _ptr),
         ("lvx", _ptr),
         ("lvy", _ptr),
@@ -130,6 +130,8 @@ class AdjointDesc(_c.Structure):
         ("lvz_swap", _ptr),
         ("fwd_step0", _c.c_int64),
         ("fwd_count", _c.c_int64),
+        ("lp_swap", _ptr),
+        ("lpsi_swap", _ptr),
     ]
 
 
@@ -176,6 +178,8 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_adjoint_run.argtypes = [
         _c.POINTER(Desc), _c.POINTER(AdjointDesc), _c.c_int64, _c.c_int64, _ptr
     ]
+    lib.acfdtd_hip_adjoint_two_step.restype = _c.c_int
+    lib.acfdtd_hip_adjoint_two_step.argtypes = [_c.POINTER(Desc), _c.POINTER(AdjointDesc)]
     lib.acfdtd_hip_run_states.restype = _c.c_int
     lib.acfdtd_hip_run_states.argtypes = [
         _c.POINTER(Desc), _c.c_int64, _c.c_int64, _ptr, _ptr, _ptr, _ptr, _ptr

@@ -46,9 +46,11 @@ __all__ = ["RoomResponse", "GradientReport"]
 
 @dataclass
 class GradientReport:
-    """What the last backward pass did: adjoint steps, recomputed steps, state checkpoints."""
+    """What the last backward pass did: adjoint steps, recomputed steps, state checkpoints,
+    whether the adjoint ran two steps per sweep."""
 
     adjoint_steps: int = 0
+    adjoint_two_step: bool = False
     recomputed_steps: int = 0
     checkpoints: int = 0
     checkpoint_bytes: int = 0
@@ -70,6 +72,9 @@ class RoomResponse:
         autotune: pick the tilings of the forward and of the adjoint step by timing them
             (:mod:`ac_fdtd_amd.autotune`); gradients then differ from an untuned run only in
             the summation order of their fp64 partials.
+        time_block: 2 (default) advances the forward pass, and the adjoint pass when only
+            wall and signal gradients are asked for, two steps per sweep where the grid allows
+            (same results; the adjoint's fp64 partials summed in another order); 1 one step.
     """
 
     def __init__(
@@ -89,6 +94,7 @@ class RoomResponse:
         fused: bool = True,
         compensated: bool = False,
         autotune: bool = False,
+        time_block: int = 2,
     ) -> None:
         if n_steps < 1:
             raise ValueError("n_steps must be >= 1")
@@ -106,6 +112,9 @@ class RoomResponse:
         self.checkpoint_every = int(checkpoint_every)
         self.fused, self.compensated = fused, compensated
         self.autotune = autotune
+        if time_block not in (1, 2):
+            raise ValueError("time_block must be 1 or 2")
+        self.time_block = int(time_block)
         self.source_points = [point for point, _ in sources]
         self.signals = [np.asarray(signal, dtype=np.float64) for _, signal in sources]
         self.receivers = list(receivers)
@@ -167,6 +176,7 @@ class RoomResponse:
             relaxation_strengths=delta,
             wall_table_always=wall_table_always,
             autotune=self.autotune,
+            time_block=self.time_block,
         )
         signals = self.signals if signal is None else list(
             torch.as_tensor(signal).detach().cpu().double().numpy()
@@ -256,10 +266,15 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
     G = grad_recording.to(**opts).contiguous()
     recording = solver.recording_tensor()
     fwd = solver._descriptor(recording)
+    two_step = (room.fused and room.time_block == 2 and not (want_sponge or want_air)
+                and solver.two_step_applies())
     if room.autotune:
         from .autotune import adjoint_tiling
 
-        fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, fwdgrad=want_sponge or want_air)
+        if two_step:
+            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, two_step=True)
+        else:
+            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, fwdgrad=want_sponge or want_air)
     n_partials = lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd))
     partials = torch.zeros(n_partials, dtype=torch.float64, device=solver.device)
     sources = solver._source_arrays()
@@ -298,6 +313,13 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
         swaps = [torch.zeros(g.shape, **opts)] + [torch.zeros(g.velocity_shape(a), **opts)
                                                   for a in range(3)]
         adj.ldiv_swap, adj.lvx_swap, adj.lvy_swap, adj.lvz_swap = (t.data_ptr() for t in swaps)
+        if two_step:
+            # Two adjoint steps per sweep also ping-pong lp and lpsi.
+            swaps += [torch.zeros(g.shape, **opts), torch.zeros((k.n_relax, *g.shape), **opts)]
+            adj.lp_swap = swaps[4].data_ptr()
+            adj.lpsi_swap = swaps[5].data_ptr() if k.n_relax else None
+            adj.time_block = 2
+    report.adjoint_two_step = bool(lib.acfdtd_hip_adjoint_two_step(ctypes.byref(fwd), ctypes.byref(adj)))
 
     stream = torch.cuda.current_stream(solver.device).cuda_stream
     handle = ctypes.c_void_p(stream)

@@ -220,11 +220,15 @@ def forward_plan(solver) -> tuple[int, int, int]:
     return plan
 
 
-def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tuple[int, int]:
+def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
+                   two_step: bool = False) -> tuple[int, int]:
     """Fastest (tile_j, chunk_x) of the fused adjoint step: with the wall gradient when the
-    solver has a wall table, and with the sponge and air gradients too when ``fwdgrad``."""
+    solver has a wall table, and with the sponge and air gradients too when ``fwdgrad``; of the
+    two-step adjoint sweep (wall gradient only) when ``two_step``."""
     if not solver.fused:
         return solver.tile_j, solver.chunk_x
+    if candidates is None:
+        candidates = CANDIDATES_TWO_STEP if two_step else CANDIDATES
     g, k = solver.grid, solver.coefficients
     lib = solver._lib
     opts = {"device": solver.device, "dtype": solver.dtype}
@@ -253,8 +257,14 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tupl
     adj.lvx_swap, adj.lvy_swap, adj.lvz_swap = (t.data_ptr() for t in lam[1][1:])
     adj.lpsi = lpsi.data_ptr() if k.n_relax else None
     adj.dt = solver.dt
+    second = None
+    if two_step:
+        second = [torch.randn(g.shape, generator=gen, **opts), torch.zeros((k.n_relax, *g.shape), **opts)]
+        adj.lp_swap = second[0].data_ptr()
+        adj.lpsi_swap = second[1].data_ptr() if k.n_relax else None
+        adj.time_block = 2
     ring = weights = None
-    if fwdgrad and (k.has_layer or k.n_relax):
+    if fwdgrad and not two_step and (k.has_layer or k.n_relax):
         # Adjoint steps n .. 1 read forward states 1 .. n + 1.
         ring = [torch.zeros((steps + 2, *g.shape), **opts)] + [
             torch.zeros((steps + 2, *g.velocity_shape(a)), **opts) for a in range(3)]
@@ -282,5 +292,5 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=CANDIDATES) -> tupl
         _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), n, 1,
                                                  handle), "acfdtd_hip_adjoint_run")
 
-    kind = "adjoint-fwdgrad" if ring is not None else "adjoint"
+    kind = "adjoint2" if two_step else "adjoint-fwdgrad" if ring is not None else "adjoint"
     return _pick(solver, kind, launch, candidates)

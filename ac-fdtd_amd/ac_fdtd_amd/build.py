@@ -39,6 +39,8 @@ SOURCES = (
     "adjoint_fused64.hip",
     "fused2.hip",
     "fused2_64.hip",
+    "adjoint2.hip",
+    "adjoint2_64.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

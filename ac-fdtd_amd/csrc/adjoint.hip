@@ -37,6 +37,8 @@
 
 #include "acfdtd_hip.h"
 #include "adjoint.hpp"
+#include "adjoint2.hpp"
+#include "marches2.hpp"
 #include "sweeps.hpp"
 
 namespace acfdtd {
@@ -269,6 +271,8 @@ int validate_adjoint(const acfdtd_hip_desc *d, const acfdtd_hip_adjoint_desc *g)
     if (g == nullptr) return fail(ACFDTD_HIP_EINVAL, "adjoint descriptor is NULL");
     if (g->abi_version != ACFDTD_HIP_ABI_VERSION || g->struct_size != sizeof(acfdtd_hip_adjoint_desc))
         return fail(ACFDTD_HIP_EABI, "adjoint descriptor ABI version or size mismatch");
+    if (g->time_block < 0 || g->time_block > 2)
+        return fail(ACFDTD_HIP_EINVAL, "adjoint time_block must be 0, 1 or 2");
     if (!g->lp || !g->lvx || !g->lvy || !g->lvz || !g->ldiv)
         return fail(ACFDTD_HIP_EINVAL, "adjoint field pointer is NULL");
     if (d->n_relax > 0 && !g->lpsi) return fail(ACFDTD_HIP_EINVAL, "adjoint relaxation states are NULL");
@@ -347,8 +351,11 @@ int adjoint_run(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g, int6
     const int64_t n_steps = step_hi - step_lo + 1;
     int64_t done = 0;
     if (has_adjoint_swap(g)) {
-        done = launch_adjoint_pairs<real>(d, g, step_hi, n_steps, stream);
-        const int status = check_hip(hipGetLastError(), "fused adjoint launch");
+        done = launch_adjoint2_quads<real>(d, g, step_hi, n_steps, stream);
+        int status = check_hip(hipGetLastError(), "two-step adjoint launch");
+        if (status != ACFDTD_HIP_OK) return status;
+        done += launch_adjoint_pairs<real>(d, g, step_hi - done, n_steps - done, stream);
+        status = check_hip(hipGetLastError(), "fused adjoint launch");
         if (status != ACFDTD_HIP_OK) return status;
     }
     for (int64_t m = step_hi - done; m >= step_lo; --m) {
@@ -365,7 +372,10 @@ using namespace acfdtd;
 extern "C" size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc) {
     if (validate(desc) != ACFDTD_HIP_OK) return 0;
     const Tiling t = tiling_for(*desc);
-    const int64_t waves = std::max<int64_t>((int64_t)t.tiles * t.tile_j, adjoint_fused_waves(*desc));
+    int64_t waves = std::max<int64_t>((int64_t)t.tiles * t.tile_j, adjoint_fused_waves(*desc));
+    Tiling2 t2;
+    if (make_tiling2(*desc, desc->real_bytes == 8 ? 2 : 4, t2))
+        waves = std::max<int64_t>(waves, (int64_t)t2.tiles * (t2.threads / 64));
     return (size_t)waves * kGradSlots;
 }
 
@@ -398,6 +408,14 @@ extern "C" int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
     if (desc->real_bytes == 8)
         return cell_sweep<double>(*desc, *adjoint, step, stencil != 0, local != 0, s);
     return cell_sweep<float>(*desc, *adjoint, step, stencil != 0, local != 0, s);
+}
+
+extern "C" int acfdtd_hip_adjoint_two_step(const acfdtd_hip_desc *desc,
+                                          const acfdtd_hip_adjoint_desc *adjoint) {
+    if (validate_adjoint(desc, adjoint) != ACFDTD_HIP_OK) return 0;
+    Tiling2 t2;
+    return adjoint2_applies(*desc, *adjoint) &&
+           make_tiling2(*desc, desc->real_bytes == 8 ? 2 : 4, t2);
 }
 
 extern "C" int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc,

@@ -127,6 +127,14 @@ template <typename real>
 int64_t launch_adjoint_pairs(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g,
                              int64_t step_hi, int64_t n_steps, hipStream_t stream);
 
+// Launch two-step adjoint sweeps for the leading multiple of four of n_steps (steps step_hi,
+// step_hi - 1, ..., each with its local part), in pairs from the first set to the swap set and
+// back; returns the number of steps done (0 when the descriptors do not allow it). Defined in
+// adjoint2.hip and adjoint2_64.hip.
+template <typename real>
+int64_t launch_adjoint2_quads(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g,
+                              int64_t step_hi, int64_t n_steps, hipStream_t stream);
+
 inline bool has_adjoint_swap(const acfdtd_hip_adjoint_desc &g) {
     return g.ldiv_swap && g.lvx_swap && g.lvy_swap && g.lvz_swap;
 }

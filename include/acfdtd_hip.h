@@ -219,7 +219,9 @@ typedef struct acfdtd_hip_adjoint_desc {
     uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
     uint32_t struct_size; /* sizeof(acfdtd_hip_adjoint_desc) */
     uint32_t grads;       /* ACFDTD_HIP_GRAD_* */
-    int32_t reserved;
+    /* acfdtd_hip_adjoint_run only: 2 advances two adjoint steps per sweep where allowed (see
+     * lp_swap), 0 or 1 one. */
+    int32_t time_block;
     /* Adjoint state, same shapes as p, vx, vy, vz, psi; ldiv is one cell field of scratch. */
     void *lp, *lvx, *lvy, *lvz, *lpsi, *ldiv;
     /* dL/d(recording): column (step - grad_recording_step0) of rows of
@@ -249,6 +251,13 @@ typedef struct acfdtd_hip_adjoint_desc {
      * (p^n, v^(n-1/2)) with n = fwd_step0 + q. Needed for sponge and air gradients. */
     int64_t fwd_step0;
     int64_t fwd_count;
+    /* acfdtd_hip_adjoint_run with time_block = 2: second lp and lpsi (same shapes). With them
+     * and the swap set above, the leading multiple of four adjoint steps whose local parts all
+     * exist (steps >= 1) run two per sweep when only wall and signal gradients are asked for,
+     * nz is a multiple of 4 (fp32) or 2 (fp64) and there are at most 64 sources and 256
+     * receivers; the other steps one per sweep. The results are the same bits (the fp64
+     * partials summed in another order); the state ends in the first set. */
+    void *lp_swap, *lpsi_swap;
 } acfdtd_hip_adjoint_desc;
 
 size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc);
@@ -266,6 +275,11 @@ int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
  * last step of a run). The state ends in the first set (lp, ldiv, lvx, lvy, lvz). */
 int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc, const acfdtd_hip_adjoint_desc *adjoint,
                            int64_t step_hi, int64_t step_lo, void *stream);
+
+/* 1 when acfdtd_hip_adjoint_run would advance these descriptors two adjoint steps per sweep
+ * (see acfdtd_hip_adjoint_desc::lp_swap), else 0. No reference counterpart. */
+int acfdtd_hip_adjoint_two_step(const acfdtd_hip_desc *desc,
+                                const acfdtd_hip_adjoint_desc *adjoint);
 
 /* Forward recomputation into a ring of states: slot 0 of p_states, vx_states, vy_states,
  * vz_states (each field's states back to back) holds (p, v) at step step_offset; advances
