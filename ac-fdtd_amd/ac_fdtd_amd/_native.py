@@ -21,6 +21,7 @@ __all__ = [
     "AdjointDesc",
     "COMPENSATED",
     "Desc",
+    "STAGED",
     "GRAD_AIR",
     "GRAD_SLOTS",
     "GRAD_SPONGE",
@@ -34,6 +35,7 @@ ABI_VERSION = 1
 MAX_RELAXATION = 4
 WALL_CODES = 64
 COMPENSATED = 1
+STAGED = 2
 
 _c = ctypes
 _ptr = _c.c_void_p

@@ -182,42 +182,45 @@ def forward_tiling(solver, time_block: int = 1, candidates=None) -> tuple[int, i
     return _pick(solver, kind, launch, candidates)
 
 
-def forward_plan(solver) -> tuple[int, int, int]:
-    """(time_block, tile_j, chunk_x) for ``solver``: the fastest one-step tiling and, when the
-    solver may advance two steps per sweep, the fastest two-step tiling, whichever runs more
-    steps per second."""
+def forward_plan(solver) -> tuple[int, int, int, bool]:
+    """(time_block, tile_j, chunk_x, staged) for ``solver``: the fastest one-step tiling and,
+    when the solver may advance two steps per sweep, the fastest two-step tiling, plain and
+    staged (each plane's inputs requested a plane ahead into LDS), whichever runs more steps
+    per second."""
     single = forward_tiling(solver, 1)
     if not solver.two_step_applies():
-        return 1, single[0], single[1]
+        return 1, single[0], single[1], False
     double = forward_tiling(solver, 2)
     key = _key("plan", solver)
     found = _lookup(key)
     if found is not None:
-        return tuple(found)
+        return (*found, False)[:4] if len(found) == 3 else tuple(found)
     handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
     launches = {}
-    for time_block, shape in ((1, single), (2, double)):
+    for plan, shape in (((1, False), single), ((2, False), double), ((2, True), double)):
         scratch = _Scratch(solver)
-        scratch.desc.time_block = time_block
+        scratch.desc.time_block = plan[0]
         scratch.desc.tile_j, scratch.desc.chunk_x = shape
+        scratch.desc.flags = (scratch.desc.flags & ~_native.STAGED) | (_native.STAGED if plan[1] else 0)
 
         def launch(steps, _shape, desc=scratch.desc, keep=scratch):
             _native.check(solver._lib.acfdtd_hip_run(ctypes.byref(desc), 0, steps, 0, handle),
                           "acfdtd_hip_run")
 
-        launches[time_block] = (launch, shape)
+        launches[plan] = (launch, shape)
     steps = 4 * _steps_per_sample(solver)
-    per_step = {1: float("inf"), 2: float("inf")}
+    per_step = {plan: float("inf") for plan in launches}
     with torch.cuda.device(solver.device):
-        _warm_up(solver, *launches[1], steps)
-        # Interleaved rounds, the best of each: a drift of the clocks favours neither.
+        _warm_up(solver, *launches[(1, False)], steps)
+        # Interleaved rounds, the best of each: a drift of the clocks favours none.
         for _ in range(3):
-            for time_block, (launch, shape) in launches.items():
-                per_step[time_block] = min(per_step[time_block],
-                                           _time(solver, launch, shape, steps, 1))
-    plan = (2, *double) if per_step[2] < per_step[1] else (1, *single)
-    _store(key, plan)
-    return plan
+            for plan, (launch, shape) in launches.items():
+                per_step[plan] = min(per_step[plan], _time(solver, launch, shape, steps, 1))
+    best = min(per_step, key=per_step.get)
+    shape = single if best[0] == 1 else double
+    result = (best[0], shape[0], shape[1], best[1])
+    _store(key, result)
+    return result
 
 
 def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,

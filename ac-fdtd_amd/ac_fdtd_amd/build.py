@@ -41,6 +41,8 @@ SOURCES = (
     "fused2_64.hip",
     "adjoint2.hip",
     "adjoint2_64.hip",
+    "fused2s.hip",
+    "fused2s_64.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

@@ -76,6 +76,10 @@ class AcousticFDTD:
             the run advances two steps per sweep wherever it can (a second set of relaxation
             states is allocated for it); results are bit-identical to single steps. With
             ``autotune`` the faster of the two is kept.
+        staged: two-step sweeps request each plane's inputs one plane ahead, straight into
+            LDS, so that they travel while the previous plane's second step runs (same
+            results; needs 16-byte aligned fields, which torch allocations are). With
+            ``autotune`` it is chosen by timing.
     """
 
     def __init__(
@@ -97,6 +101,7 @@ class AcousticFDTD:
         wall_table_always: bool = False,
         autotune: bool = False,
         time_block: int = 2,
+        staged: bool = False,
     ) -> None:
         if time_block not in (1, 2):
             raise ValueError(f"time_block must be 1 or 2, got {time_block}")
@@ -125,6 +130,8 @@ class AcousticFDTD:
             raise ValueError("compensated summation is an fp32 mode")
         self.compensated = compensated
         self.tile_j, self.chunk_x = tile_j, chunk_x
+        # Two-step sweeps request each plane's inputs a plane ahead, into LDS (same results).
+        self.staged = bool(staged)
         self.dtype = _TORCH_DTYPES[self.np_dtype]
         self.dt = self.coefficients.dt
         self.step_index = 0
@@ -167,7 +174,7 @@ class AcousticFDTD:
         if autotune:
             from .autotune import forward_plan
 
-            self.time_block, self.tile_j, self.chunk_x = forward_plan(self)
+            self.time_block, self.tile_j, self.chunk_x, self.staged = forward_plan(self)
 
     # -- reference API ----------------------------------------------------------------------
     @property
@@ -434,7 +441,8 @@ class AcousticFDTD:
         d.real_bytes = self.np_dtype.itemsize
         d.n_relax = k.n_relax
         d.has_layer = int(k.has_layer)
-        d.flags = _native.COMPENSATED if self.compensated else 0
+        d.flags = (_native.COMPENSATED if self.compensated else 0) | (
+            _native.STAGED if self.staged else 0)
         d.tile_j, d.chunk_x = self.tile_j, self.chunk_x
         d.time_block = self.time_block
         d.vel_coef, d.pres_coef, d.src_coef = k.velocity, k.pressure, k.source

@@ -21,12 +21,14 @@ namespace acfdtd {
 #define ACFDTD_FUSED2_MIN_WAVES 2
 #endif
 
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
 __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused2_step(const SweepArgs<real> a,
                                                              const Fused2IO<real> io,
                                                              const Tiling2 t2, const int64_t step,
                                                              const int64_t column) {
     __shared__ real lds[3 * kFused2Threads * V];
+    // STAGED: each lane's slots for the next plane's first-step inputs (fused2_march).
+    __shared__ __align__(16) unsigned char stage[STAGED ? stage_bytes<real, V, NR>() : 16];
     // Source and receiver tables, copied once: the lanes that hold a source or a receiver scan
     // them every plane, and with a barrier per plane the whole workgroup waits for those lanes.
     // In LDS their reads wait for nothing else; a read from global (or generic) memory waits
@@ -68,10 +70,10 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     }
     for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
     lds_barrier();
-    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED>(
+    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED>(
         a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
         io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
-        len_lds, now_lds, rx_lds);
+        len_lds, now_lds, rx_lds, stage);
 }
 
 template <typename real>
@@ -84,33 +86,58 @@ constexpr int fused2_vec() {
     return sizeof(real) == 4 ? 4 : 2;
 }
 
-template <typename real, int NR>
+template <typename real, int NR, bool STAGED>
 Fused2Kernel<real> pick_fused2_nr(bool walls, bool layer, bool comp) {
     constexpr int V = fused2_vec<real>();
     if constexpr (sizeof(real) == 4) {
         if (comp) {
-            if (walls) return layer ? fused2_step<real, V, NR, true, true, true>
-                                    : fused2_step<real, V, NR, true, false, true>;
-            return layer ? fused2_step<real, V, NR, false, true, true>
-                         : fused2_step<real, V, NR, false, false, true>;
+            if (walls) return layer ? fused2_step<real, V, NR, true, true, true, STAGED>
+                                    : fused2_step<real, V, NR, true, false, true, STAGED>;
+            return layer ? fused2_step<real, V, NR, false, true, true, STAGED>
+                         : fused2_step<real, V, NR, false, false, true, STAGED>;
         }
     }
     (void)comp;
-    if (walls) return layer ? fused2_step<real, V, NR, true, true, false>
-                            : fused2_step<real, V, NR, true, false, false>;
-    return layer ? fused2_step<real, V, NR, false, true, false>
-                 : fused2_step<real, V, NR, false, false, false>;
+    if (walls) return layer ? fused2_step<real, V, NR, true, true, false, STAGED>
+                            : fused2_step<real, V, NR, true, false, false, STAGED>;
+    return layer ? fused2_step<real, V, NR, false, true, false, STAGED>
+                 : fused2_step<real, V, NR, false, false, false, STAGED>;
+}
+
+template <typename real, bool STAGED>
+Fused2Kernel<real> pick_fused2_kind(int nr, bool walls, bool layer, bool comp) {
+    switch (nr) {
+        case 0: return pick_fused2_nr<real, 0, STAGED>(walls, layer, comp);
+        case 1: return pick_fused2_nr<real, 1, STAGED>(walls, layer, comp);
+        case 2: return pick_fused2_nr<real, 2, STAGED>(walls, layer, comp);
+        case 3: return pick_fused2_nr<real, 3, STAGED>(walls, layer, comp);
+        default: return pick_fused2_nr<real, 4, STAGED>(walls, layer, comp);
+    }
+}
+
+// The staged kernels, instantiated in their own translation units (fused2s.hip,
+// fused2s_64.hip) so that they compile in parallel with the others.
+template <typename real>
+Fused2Kernel<real> pick_fused2_staged(int nr, bool walls, bool layer, bool comp);
+
+// Whether the staged variant may run: asked for, and every field it stages 16-byte aligned in
+// both sets (otherwise the plain two-step sweep runs, with the same results).
+inline bool staged_applies(const acfdtd_hip_desc &d) {
+    if (!(d.flags & ACFDTD_HIP_STAGED)) return false;
+    const void *fields[] = {d.p, d.vx, d.vy, d.p_swap, d.vx_swap, d.vy_swap};
+    for (const void *f : fields)
+        if (reinterpret_cast<uintptr_t>(f) % 16 != 0) return false;
+    if (d.n_relax > 0 && (reinterpret_cast<uintptr_t>(d.psi) % 16 != 0 ||
+                          reinterpret_cast<uintptr_t>(d.psi_swap) % 16 != 0))
+        return false;
+    return true;
 }
 
 template <typename real>
-Fused2Kernel<real> pick_fused2(int nr, bool walls, bool layer, bool comp) {
-    switch (nr) {
-        case 0: return pick_fused2_nr<real, 0>(walls, layer, comp);
-        case 1: return pick_fused2_nr<real, 1>(walls, layer, comp);
-        case 2: return pick_fused2_nr<real, 2>(walls, layer, comp);
-        case 3: return pick_fused2_nr<real, 3>(walls, layer, comp);
-        default: return pick_fused2_nr<real, 4>(walls, layer, comp);
-    }
+Fused2Kernel<real> pick_fused2(const acfdtd_hip_desc &d) {
+    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
+    if (staged_applies(d)) return pick_fused2_staged<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    return pick_fused2_kind<real, false>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
 }
 
 // The two directions of a two-step sweep: forward reads the descriptor's fields and writes the
@@ -143,8 +170,7 @@ int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64
     if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2)) return 0;
     SweepArgs<real> a;
     fill_args(d, a);
-    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
-    const Fused2Kernel<real> k = pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    const Fused2Kernel<real> k = pick_fused2<real>(d);
     Fused2IO<real> forward, backward;
     fused2_io<real>(d, forward, backward);
     const dim3 grid(t2.launched), block(t2.threads);
@@ -167,10 +193,9 @@ int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, b
         return fail(ACFDTD_HIP_EINVAL, "the two-step sweep does not apply to this descriptor");
     SweepArgs<real> a;
     fill_args(d, a);
-    const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
     Fused2IO<real> forward, backward;
     fused2_io<real>(d, forward, backward);
-    hipLaunchKernelGGL(pick_fused2<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp),
+    hipLaunchKernelGGL(pick_fused2<real>(d),
                        dim3(t2.launched), dim3(t2.threads), 0, stream, a,
                        reverse ? backward : forward, t2, step, column);
     return check_hip(hipGetLastError(), "two-step sweep launch");

@@ -61,6 +61,138 @@ __device__ __forceinline__ void lds_barrier() {
 // vmcnt in bits 3:0 and 15:14, expcnt in 6:4, lgkmcnt in 11:8; the others at their maximum).
 __device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// s_waitcnt lgkmcnt(0): wait for the wave's outstanding LDS (and scalar memory) operations.
+__device__ __forceinline__ void wait_lds() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// s_waitcnt vmcnt(N): wait until at most the N most recently issued vector memory operations
+// of the wave are outstanding (the counter completes in issue order).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | 0x70 | 0xF00 | ((N >> 4) << 14));
+}
+
+// The stage's reads for one lane, in asm blocks that also wait for them. The compiler's own
+// bookkeeping of direct-to-LDS loads would put a wait for every earlier memory operation (the
+// previous plane's stores too) before these reads; the blocks keep it from seeing them, and
+// the caller's wait_vm covers the stage. Each block's results are complete when it ends (its
+// last instruction waits for them), and its outputs are early-clobber so that no result
+// register overlaps an address still to be read. The blocks declare no memory clobber (that
+// alone makes the compiler wait for all memory operations first); they stay between the
+// caller's wait_vm and the next stage request, side-effecting builtins a volatile asm block is
+// not moved across.
+//
+// Layout (bytes from the stage's start): 16-byte field f of lane t at f * kStageField + 16 t,
+// f < NF; 4-byte word w of lane t at NF * kStageField + w * kStageWord + 4 t. Reads go through
+// two base addresses (lane's slot of field 0, and of field 4) and immediate offsets (< 64 KiB).
+constexpr int kStageField = 16 * kFused2Threads;
+constexpr int kStageWord = 4 * kFused2Threads;
+using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+template <int NF>
+struct StageRead16;
+template <int NF, int NS>
+struct StageRead4;
+
+template <>
+struct StageRead16<6> {
+    __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
+                                               u32x4 (&o)[6]) {
+        asm volatile("ds_read_b128 %0, %6 offset:0\n\tds_read_b128 %1, %6 offset:8192\n\tds_read_b128 %2, %6 offset:16384\n\tds_read_b128 %3, %6 offset:24576\n\tds_read_b128 %4, %7 offset:0\n\tds_read_b128 %5, %7 offset:8192\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5])
+                     : "v"(base0), "v"(base4));
+    }
+};
+
+template <>
+struct StageRead16<7> {
+    __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
+                                               u32x4 (&o)[7]) {
+        asm volatile("ds_read_b128 %0, %7 offset:0\n\tds_read_b128 %1, %7 offset:8192\n\tds_read_b128 %2, %7 offset:16384\n\tds_read_b128 %3, %7 offset:24576\n\tds_read_b128 %4, %8 offset:0\n\tds_read_b128 %5, %8 offset:8192\n\tds_read_b128 %6, %8 offset:16384\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6])
+                     : "v"(base0), "v"(base4));
+    }
+};
+
+template <>
+struct StageRead16<8> {
+    __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
+                                               u32x4 (&o)[8]) {
+        asm volatile("ds_read_b128 %0, %8 offset:0\n\tds_read_b128 %1, %8 offset:8192\n\tds_read_b128 %2, %8 offset:16384\n\tds_read_b128 %3, %8 offset:24576\n\tds_read_b128 %4, %9 offset:0\n\tds_read_b128 %5, %9 offset:8192\n\tds_read_b128 %6, %9 offset:16384\n\tds_read_b128 %7, %9 offset:24576\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7])
+                     : "v"(base0), "v"(base4));
+    }
+};
+
+template <>
+struct StageRead16<9> {
+    __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
+                                               u32x4 (&o)[9]) {
+        asm volatile("ds_read_b128 %0, %9 offset:0\n\tds_read_b128 %1, %9 offset:8192\n\tds_read_b128 %2, %9 offset:16384\n\tds_read_b128 %3, %9 offset:24576\n\tds_read_b128 %4, %10 offset:0\n\tds_read_b128 %5, %10 offset:8192\n\tds_read_b128 %6, %10 offset:16384\n\tds_read_b128 %7, %10 offset:24576\n\tds_read_b128 %8, %10 offset:32768\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8])
+                     : "v"(base0), "v"(base4));
+    }
+};
+
+template <>
+struct StageRead16<10> {
+    __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
+                                               u32x4 (&o)[10]) {
+        asm volatile("ds_read_b128 %0, %10 offset:0\n\tds_read_b128 %1, %10 offset:8192\n\tds_read_b128 %2, %10 offset:16384\n\tds_read_b128 %3, %10 offset:24576\n\tds_read_b128 %4, %11 offset:0\n\tds_read_b128 %5, %11 offset:8192\n\tds_read_b128 %6, %11 offset:16384\n\tds_read_b128 %7, %11 offset:24576\n\tds_read_b128 %8, %11 offset:32768\n\tds_read_b128 %9, %11 offset:40960\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
+                     : "v"(base0), "v"(base4));
+    }
+};
+
+template <int NF>
+struct StageRead4<NF, 7> {
+    __device__ __forceinline__ static void run(const uint32_t base, uint32_t (&o)[7]) {
+        asm volatile("ds_read_b32 %0, %7 offset:0\n\tds_read_b32 %1, %7 offset:2048\n\tds_read_b32 %2, %7 offset:4096\n\tds_read_b32 %3, %7 offset:6144\n\tds_read_b32 %4, %7 offset:8192\n\tds_read_b32 %5, %7 offset:10240\n\tds_read_b32 %6, %7 offset:12288\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6])
+                     : "v"(base));
+    }
+};
+
+template <int NF>
+struct StageRead4<NF, 10> {
+    __device__ __forceinline__ static void run(const uint32_t base, uint32_t (&o)[10]) {
+        asm volatile("ds_read_b32 %0, %10 offset:0\n\tds_read_b32 %1, %10 offset:2048\n\tds_read_b32 %2, %10 offset:4096\n\tds_read_b32 %3, %10 offset:6144\n\tds_read_b32 %4, %10 offset:8192\n\tds_read_b32 %5, %10 offset:10240\n\tds_read_b32 %6, %10 offset:12288\n\tds_read_b32 %7, %10 offset:14336\n\tds_read_b32 %8, %10 offset:16384\n\tds_read_b32 %9, %10 offset:18432\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
+                     : "v"(base));
+    }
+};
+
+// The plane exchange's barrier in the staged sweep: this wave's LDS operations complete
+// (lgkmcnt), then s_barrier. No memory fence: an LDS release fence would also wait for the
+// stage's direct-to-LDS loads still in flight (they write LDS), which is the latency the
+// staging hides; no wave reads another wave's stage slots. The signal fences keep the compiler
+// from moving memory accesses across the barrier.
+__device__ __forceinline__ void exchange_barrier() {
+    wait_lds();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Direct-to-LDS load of 4 bytes per lane: lane l writes the word at `src` to wave_base + 4 l.
+__device__ __forceinline__ void load_to_lds4(const void *src, void *wave_base) {
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                     (void __attribute__((address_space(3))) *)wave_base, 4, 0, 0);
+}
+
+// The LDS byte address of a pointer into a __shared__ array.
+__device__ __forceinline__ uint32_t lds_address(const void *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+// Direct-to-LDS load of 16 bytes per lane: lane l of the wavefront writes the 16 bytes at
+// `src` (its own address, 16-byte aligned) to wave_base + 16 l. `wave_base` is the same for
+// the whole wavefront. Nothing passes through registers; the load counts in the memory
+// counter like any other.
+__device__ __forceinline__ void load_to_lds16(const void *src, void *wave_base) {
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                     (void __attribute__((address_space(3))) *)wave_base, 16, 0, 0);
+}
+
 // Workgroup shape of the two-step sweep: a block of `rows` rows (y) by `seg` lane groups of a
 // row (z, `vec` cells each), plus one halo row above and below and, when a segment is shorter
 // than the row, one halo lane group at each end. Thread t sits at block row t / pitch - 1 and
@@ -395,7 +527,33 @@ __device__ __forceinline__ void record_cells(const SweepArgs<real> &a, const Lan
 // The march of one lane over planes [i0, i1). `valid`: the lane maps to cells of the grid;
 // `owner`: it is one of the workgroup's own lanes (not halo), which alone store and record.
 // Every thread of the workgroup runs the same loop and reaches every barrier.
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+// First-step inputs of one plane for one lane.
+template <typename real, int V, int NR>
+struct Step1In {
+    real vx_v[V], p_up[V], vy_lo[V], vy_hi[V], q_lo[V], q_hi[V], vz_v[V];
+    real vz_top, pz_lo, pz_hi;
+    real psi[NR > 0 ? NR : 1][V];
+};
+
+// Fields of a plane's first-step inputs staged in LDS when STAGED: vx, p above, vy below and
+// above, p of the rows below and above, the relaxation states (each 16 bytes per lane).
+template <int NR>
+constexpr int staged_fields() {
+    return 6 + NR;
+}
+// 4-byte words staged per lane: the V values of vz (its rows are nz + 1 long, so not 16-byte
+// aligned), vz's top value and the two z-neighbour pressures.
+template <typename real, int V>
+constexpr int staged_words() {
+    return (V + 3) * (int)(sizeof(real) / 4);
+}
+// Bytes of a workgroup's stage.
+template <typename real, int V, int NR>
+constexpr int stage_bytes() {
+    return staged_fields<NR>() * kStageField + staged_words<real, V>() * kStageWord;
+}
+
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
 __device__ __forceinline__ void fused2_march(
     const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
     const int k0, const bool valid, const bool owner, const int t, const int64_t step,
@@ -404,7 +562,7 @@ __device__ __forceinline__ void fused2_march(
     const real *__restrict__ psi_in, real *__restrict__ p_out, real *__restrict__ vx_out,
     real *__restrict__ vy_out, real *__restrict__ vz_out, real *__restrict__ psi_out,
     real *__restrict__ lds, const int32_t *src_cell, const int64_t *src_len,
-    const real *src_now, const int32_t *rx_cell) {
+    const real *src_now, const int32_t *rx_cell, unsigned char *__restrict__ stage) {
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
     const int nx = a.nx, nz = a.nz;
@@ -442,43 +600,133 @@ __device__ __forceinline__ void fused2_march(
             s1_vx_lo[e] = damp3<real, LAYER>(w, fxf, c.fyc, c.fzc[e]);
         }
     };
-    // Step 1 of plane s from global memory into o (p^n of plane s in pn_c, new lower face in
-    // s1_vx_lo); then carry plane s + 1's p^n and lower face.
-    auto step1 = [&](const int s, PlaneState<real, V, NR> &o, auto xwall_tag) __attribute__((always_inline)) {
+    // Step 1's inputs of plane s from global memory.
+    auto load_direct = [&](const int s, Step1In<real, V, NR> &in, auto xwall_tag) __attribute__((always_inline)) {
         constexpr bool XWALL = decltype(xwall_tag)::value;
         const int64_t cc = L.cell(s);
         const bool up_interior = !XWALL || s + 1 < nx;
-        real vx_v[V], p_up[V], vy_lo_in[V], vy_hi_in[V], q_lo[V], q_hi[V], vz_v[V];
-        vload<real, V>(vx_in + cc + L.plane, vx_v);
-        vload<real, V>(p_in + cc + (up_interior ? L.plane : 0), p_up);
+        vload<real, V>(vx_in + cc + L.plane, in.vx_v);
+        vload<real, V>(p_in + cc + (up_interior ? L.plane : 0), in.p_up);
         const int64_t cy = L.face_y(s);
-        vload<real, V>(vy_in + cy, vy_lo_in);
-        vload<real, V>(vy_in + cy + nz, vy_hi_in);
-        vload<real, V>(p_in + cc - c.dy_lo, q_lo);
-        vload<real, V>(p_in + cc + c.dy_hi, q_hi);
+        vload<real, V>(vy_in + cy, in.vy_lo);
+        vload<real, V>(vy_in + cy + nz, in.vy_hi);
+        vload<real, V>(p_in + cc - c.dy_lo, in.q_lo);
+        vload<real, V>(p_in + cc + c.dy_hi, in.q_hi);
         const int64_t cz = L.face_z(s);
-        vload<real, V>(vz_in + cz, vz_v);
-        const real vz_top = vz_in[cz + V];
-        const real pz_lo = p_in[cc - c.dz_lo];
-        const real pz_hi = p_in[cc + c.dz_hi];
-        real states[NR > 0 ? NR : 1][V];
+        vload<real, V>(vz_in + cz, in.vz_v);
+        in.vz_top = vz_in[cz + V];
+        in.pz_lo = p_in[cc - c.dz_lo];
+        in.pz_hi = p_in[cc + c.dz_hi];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) vload<real, V>(psi_in + (int64_t)r * n_cells + cc, states[r]);
+        for (int r = 0; r < NR; ++r) vload<real, V>(psi_in + (int64_t)r * n_cells + cc, in.psi[r]);
+    };
+    // STAGED: the same inputs of plane s requested one plane ahead, straight into this lane's
+    // LDS slots (nothing held in registers while they travel): the 16-byte aligned fields as
+    // 16-byte loads, vz (its rows are nz + 1 long, so not aligned) and the single values as
+    // 4-byte words. Layout: see StageRead16.
+    constexpr int NF = staged_fields<NR>();
+    constexpr int NS = staged_words<real, V>();
+    constexpr int WPR = (int)(sizeof(real) / 4);  // 4-byte words per value
+    const bool wave_owner = __ballot(owner) != 0;
+    auto stage_issue = [&](const int s, auto xwall_tag) __attribute__((always_inline)) {
+        constexpr bool XWALL = decltype(xwall_tag)::value;
+        const int64_t cc = L.cell(s);
+        const bool up_interior = !XWALL || s + 1 < nx;
+        unsigned char *base = stage + (t & ~63) * 16;  // this wavefront's slots of field 0
+        load_to_lds16(vx_in + cc + L.plane, base);
+        load_to_lds16(p_in + cc + (up_interior ? L.plane : 0), base + kStageField);
+        const int64_t cy = L.face_y(s);
+        load_to_lds16(vy_in + cy, base + 2 * kStageField);
+        load_to_lds16(vy_in + cy + nz, base + 3 * kStageField);
+        load_to_lds16(p_in + cc - c.dy_lo, base + 4 * kStageField);
+        load_to_lds16(p_in + cc + c.dy_hi, base + 5 * kStageField);
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            load_to_lds16(psi_in + (int64_t)r * n_cells + cc, base + (6 + r) * kStageField);
+        unsigned char *words = stage + NF * kStageField + (t & ~63) * 4;
+        auto word_load = [&](const real *src, const int w) __attribute__((always_inline)) {
+#pragma unroll
+            for (int h = 0; h < WPR; ++h)
+                load_to_lds4(reinterpret_cast<const unsigned char *>(src) + 4 * h,
+                             words + (w * WPR + h) * kStageWord);
+        };
+        const int64_t cz = L.face_z(s);
+#pragma unroll
+        for (int e = 0; e <= V; ++e) word_load(vz_in + cz + e, e);
+        word_load(p_in + cc - c.dz_lo, V + 1);
+        word_load(p_in + cc + c.dz_hi, V + 2);
+    };
+    auto stage_read = [&](Step1In<real, V, NR> &in) __attribute__((always_inline)) {
+        // The stage was requested before the previous plane's second step, whose 3 + NR
+        // unconditional stores (vy, vz, psi, p) a wavefront with an owner lane issued after it:
+        // with at most that many operations outstanding, the stage has landed. A wavefront
+        // with no owner lane (halo rows) stored nothing since, and waits for everything.
+        if (wave_owner) wait_vm<3 + NR>();
+        else wait_memory();
+        const uint32_t a16 = lds_address(stage) + (uint32_t)t * 16;
+        u32x4 raw[NF];
+        StageRead16<NF>::run(a16, a16 + 4 * kStageField, raw);
+        uint32_t w[NS];
+        StageRead4<NF, NS>::run(lds_address(stage) + NF * kStageField + (uint32_t)t * 4, w);
+        static_assert(sizeof(real) * V == sizeof(u32x4), "a lane's staged field is 16 bytes");
+        auto take = [&](const int f, real (&dst)[V]) __attribute__((always_inline)) {
+            __builtin_memcpy(dst, &raw[f], sizeof(u32x4));
+        };
+        take(0, in.vx_v);
+        take(1, in.p_up);
+        take(2, in.vy_lo);
+        take(3, in.vy_hi);
+        take(4, in.q_lo);
+        take(5, in.q_hi);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) take(6 + r, in.psi[r]);
+        auto value = [&](const int k) __attribute__((always_inline)) {
+            real v;
+            __builtin_memcpy(&v, &w[k * WPR], sizeof(real));
+            return v;
+        };
+#pragma unroll
+        for (int e = 0; e < V; ++e) in.vz_v[e] = value(e);
+        in.vz_top = value(V);
+        in.pz_lo = value(V + 1);
+        in.pz_hi = value(V + 2);
+    };
+    auto stage_issue_at = [&](const int s) __attribute__((always_inline)) {
+        if (s == nx - 1) stage_issue(s, Wall{});
+        else stage_issue(s, Interior{});
+    };
+    // Step 1 of plane s from its inputs into o (p^n of plane s in pn_c, new lower face in
+    // s1_vx_lo); then carry plane s + 1's p^n and lower face.
+    auto compute1 = [&](const int s, Step1In<real, V, NR> &in, PlaneState<real, V, NR> &o,
+                        auto xwall_tag) __attribute__((always_inline)) {
+        constexpr bool XWALL = decltype(xwall_tag)::value;
         real tape_value[V];
         advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
-            a, c, s, step, pn_c, p_up, q_lo, q_hi, pz_lo, pz_hi, s1_vx_lo, vx_v, vy_lo_in,
-            vy_hi_in, vz_v, vz_top, states, o, tape_value);
+            a, c, s, step, pn_c, in.p_up, in.q_lo, in.q_hi, in.pz_lo, in.pz_hi, s1_vx_lo, in.vx_v,
+            in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value);
         if (a.wall_tape && owner && s >= i0 && s < i1)
             tape_cells<real, V, XWALL>(a, c, s, step, t2.tape_vec != 0, tape_value);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-            pn_c[e] = p_up[e];
+            pn_c[e] = in.p_up[e];
             s1_vx_lo[e] = o.vx_hi[e];
         }
+    };
+    auto step1 = [&](const int s, PlaneState<real, V, NR> &o, auto xwall_tag) __attribute__((always_inline)) {
+        Step1In<real, V, NR> in;
+        load_direct(s, in, xwall_tag);
+        compute1(s, in, o, xwall_tag);
     };
     auto step1_at = [&](const int s, PlaneState<real, V, NR> &o) __attribute__((always_inline)) {
         if (s == 0 || s == nx - 1) step1(s, o, Wall{});
         else step1(s, o, Interior{});
+    };
+    // Step 1 of plane s from the stage (requested one plane earlier).
+    auto step1_staged_at = [&](const int s, PlaneState<real, V, NR> &o) __attribute__((always_inline)) {
+        Step1In<real, V, NR> in;
+        stage_read(in);
+        if (s == nx - 1) compute1(s, in, o, Wall{});
+        else compute1(s, in, o, Interior{});
     };
 
     // Second step of plane i from the first step's results: A (plane i), p_next (p^(n+1) of
@@ -538,8 +786,14 @@ __device__ __forceinline__ void fused2_march(
         step1_at(i0, A);
         if (owner && has_events(c, i0)) record_cells<real, V>(a, c, i0, column, A.p);
         share(i0, A.p);
+        if (STAGED && i0 + 1 < nx) stage_issue_at(i0 + 1);
     }
-    lds_barrier();
+    if (STAGED) {
+        exchange_barrier();
+        wait_memory();  // the first stage, requested just now: no store follows it yet
+    } else {
+        lds_barrier();
+    }
     if (owner) {
         // v^(n+3/2) on face i0, the lower face of the first cell (the prologue of
         // fused_march_vec with the first step's results as inputs).
@@ -555,12 +809,21 @@ __device__ __forceinline__ void fused2_march(
     for (int i = i0; i < i1; ++i) {
         const bool more = i + 1 < nx;
         if (valid && more) {
-            step1_at(i + 1, B);
+            if (STAGED) step1_staged_at(i + 1, B);
+            else step1_at(i + 1, B);
             if (owner && i + 1 < i1 && has_events(c, i + 1))
                 record_cells<real, V>(a, c, i + 1, column, B.p);
             share(i + 1, B.p);
+            // The next iteration's step 1 (plane i + 2), requested now: it travels during the
+            // barrier and the second step. The stage is refilled only after this lane's own
+            // reads of it are done.
+            if (STAGED && i + 2 < nx && i + 1 < i1) {
+                wait_lds();
+                stage_issue_at(i + 2);
+            }
         }
-        lds_barrier();
+        if (STAGED) exchange_barrier();
+        else lds_barrier();
         if (owner) {
             // Selected by value: a select between the two arrays' addresses would keep both
             // plane states in scratch memory.

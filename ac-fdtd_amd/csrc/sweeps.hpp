@@ -244,7 +244,8 @@ inline int validate(const acfdtd_hip_desc *d) {
         return fail(ACFDTD_HIP_EINVAL, "real_bytes must be 4 or 8");
     if (d->n_relax < 0 || d->n_relax > ACFDTD_HIP_MAX_RELAXATION)
         return fail(ACFDTD_HIP_EINVAL, "n_relax out of range");
-    if ((d->flags & ~ACFDTD_HIP_COMPENSATED) != 0) return fail(ACFDTD_HIP_EINVAL, "unknown flags");
+    if ((d->flags & ~(ACFDTD_HIP_COMPENSATED | ACFDTD_HIP_STAGED)) != 0)
+        return fail(ACFDTD_HIP_EINVAL, "unknown flags");
     if ((d->flags & ACFDTD_HIP_COMPENSATED) && d->real_bytes != 4)
         return fail(ACFDTD_HIP_EINVAL, "compensated summation is an fp32 mode");
     if (d->tile_j < 0 || d->tile_j > 8) return fail(ACFDTD_HIP_EINVAL, "tile_j must be 0 to 8");

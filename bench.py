@@ -158,6 +158,7 @@ def gpu_throughput(size, steps, warmup, dtype, compensated=False, autotune=True,
         "sample_rate": solver.sample_rate,
         "tiling": [solver.tile_j, solver.chunk_x],
         "time_block": solver.time_block,
+        "staged": solver.staged,
     }
 
 
@@ -609,6 +610,7 @@ def main() -> int:
 
     extra = {
         "time_block": f32["time_block"],
+        "staged": f32["staged"],
         "tiling_tile_j_chunk_x": f32["tiling"],
         "kernel_ms": {k: round(v, 4) for k, v in f32["kernels"].items()},
         "roofline_basis_bytes_per_cell_update": update_bytes(f32["n_relax"], 4),

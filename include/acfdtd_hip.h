@@ -41,6 +41,10 @@ extern "C" {
 
 /* Flags. */
 #define ACFDTD_HIP_COMPENSATED 1u /* fp32 only: error-free summation of the pressure increment */
+/* time_block = 2: requests each plane's first-step inputs one plane ahead, straight into LDS
+ * (direct-to-LDS loads), while the previous plane's second step runs. Same results; used when
+ * p, vx, vy, psi and their swap buffers are 16-byte aligned. */
+#define ACFDTD_HIP_STAGED 2u
 
 /* One room's time loop. Scalar coefficients are doubles holding values already rounded to the
  * working precision (real_bytes), so the kernels reproduce the host's bits exactly. Replaces
@@ -54,7 +58,7 @@ typedef struct acfdtd_hip_desc {
     int32_t has_layer;  /* sponge profiles present */
     int32_t n_sources;
     int32_t n_receivers;
-    uint32_t flags;     /* ACFDTD_HIP_COMPENSATED */
+    uint32_t flags;     /* ACFDTD_HIP_COMPENSATED, ACFDTD_HIP_STAGED */
     int32_t tile_j;     /* wavefronts of lanes per workgroup, 1 to 8; 0 = default */
     int32_t chunk_x;    /* planes marched per workgroup; 0 = default */
     int32_t time_block; /* steps per sweep of acfdtd_hip_run: 0 or 1 = one, 2 = two (see psi_swap) */

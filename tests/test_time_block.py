@@ -35,13 +35,13 @@ CASES = [
 ]
 
 
-def _solver(shape, dtype, config, time_block, tile_j, chunk_x, compensated=False):
+def _solver(shape, dtype, config, time_block, tile_j, chunk_x, compensated=False, staged=False):
     from ac_fdtd_amd.scheme import AcousticFDTD
 
     rng = np.random.default_rng(11)
     grid = Grid(dx=0.03, nx=shape[0], ny=shape[1], nz=shape[2])
     solver = AcousticFDTD(grid, dtype=dtype, time_block=time_block, tile_j=tile_j,
-                          chunk_x=chunk_x, compensated=compensated, **config)
+                          chunk_x=chunk_x, compensated=compensated, staged=staged, **config)
     solver.p.copy_(torch.as_tensor(rng.standard_normal(grid.shape)))
     for a in range(3):
         solver.velocity[a].copy_(torch.as_tensor(1e-3 * rng.standard_normal(grid.velocity_shape(a))))
@@ -82,6 +82,29 @@ def test_two_step_sweep_is_bitwise_single_steps(dtype, config, case):
         two.run(n_steps)
     _assert_same(one, two)
     assert one.step_index == two.step_index == 30
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("config", list(CONFIGS))
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_staged_two_step_sweep_is_bitwise_single_steps(dtype, config, case):
+    """The staged sweep (each plane's inputs requested a plane ahead, into LDS) computes the
+    same bits as single steps, with a boundary tape too."""
+    shape, tile_j, chunk_x = CASES[case]
+    one = _solver(shape, dtype, CONFIGS[config], 1, tile_j, chunk_x)
+    two = _solver(shape, dtype, CONFIGS[config], 2, tile_j, chunk_x, staged=True)
+    tapes = None
+    if config == "all":
+        tapes = [torch.full((30, s.wall_tape_stride()), -7.0, dtype=s.dtype, device=s.device)
+                 for s in (one, two)]
+        one.attach_wall_tape(tapes[0], 0)
+        two.attach_wall_tape(tapes[1], 0)
+    for n_steps in (8, 5, 4, 7, 6):
+        one.run(n_steps)
+        two.run(n_steps)
+    _assert_same(one, two)
+    if tapes is not None:
+        assert torch.equal(tapes[0], tapes[1])
 
 
 def test_two_step_compensated_fp32():
