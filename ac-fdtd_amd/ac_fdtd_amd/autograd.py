@@ -267,7 +267,7 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
     recording = solver.recording_tensor()
     fwd = solver._descriptor(recording)
     two_step = (room.fused and room.time_block == 2 and not (want_sponge or want_air)
-                and solver.two_step_applies())
+                and solver.adjoint_two_step_applies())
     if room.autotune:
         from .autotune import adjoint_tiling
 

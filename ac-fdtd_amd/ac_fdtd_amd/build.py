@@ -43,6 +43,9 @@ SOURCES = (
     "adjoint2_64.hip",
     "fused2s.hip",
     "fused2s_64.hip",
+    "fused2_v2.hip",
+    "fused2_v1.hip",
+    "fused2_64_v1.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

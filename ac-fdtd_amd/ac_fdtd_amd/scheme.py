@@ -326,18 +326,22 @@ class AcousticFDTD:
         return {kind: sum(v) / len(v) for kind, v in totals.items()}
 
     def two_step_applies(self) -> bool:
-        """Whether :meth:`run` advances two steps per sweep: ``time_block=2`` on the fused path
-        and nz a multiple of the lane width, 4 cells fp32 and 2 fp64, with at most 64 sources
-        and 256 receivers, whose tables the sweep keeps in LDS (the library's own test,
-        restated)."""
+        """Whether :meth:`run` advances two steps per sweep: ``time_block=2`` on the fused path,
+        with at most 64 sources and 256 receivers, whose tables the sweep keeps in LDS (the
+        library's own test, restated). Any nz: lanes hold 4, 2 or 1 cells (fp32), 2 or 1
+        (fp64), the widest that divides nz."""
         if self.time_block != 2 or not self.fused:
-            return False
-        if len(self._sources) > 64 or len(self._receivers) > 256:
             return False
         if self.coefficients.n_relax and self._psi_swap is None:
             return False
+        return len(self._sources) <= 64 and len(self._receivers) <= 256
+
+    def adjoint_two_step_applies(self) -> bool:
+        """Whether the adjoint of this solver's run can advance two steps per sweep (for wall and
+        signal gradients): as :meth:`two_step_applies`, with 16-byte lanes, so nz a multiple of
+        4 (fp32) or 2 (fp64)."""
         vec = 4 if self.np_dtype.itemsize == 4 else 2
-        return self.grid.nz % vec == 0
+        return self.two_step_applies() and self.grid.nz % vec == 0
 
     def synchronize(self) -> None:
         torch.cuda.synchronize(self.device)

@@ -1,11 +1,14 @@
-// fp32 instantiation of the two-step sweep (templates in fused2.hpp and marches2.hpp).
+// Two-step sweep, fp32: the launchers and the kernels of 4-cell lanes
+// (templates in fused2.hpp and marches2.hpp).
 
 #include "fused2.hpp"
 
 namespace acfdtd {
 
+template Fused2Kernel<float> pick_fused2_kind<float, 4, false>(int, bool, bool, bool);
+
 template int64_t launch_fused2_quads<float>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
-                                            hipStream_t);
+                                             hipStream_t);
 
 template int launch_fused2_once<float>(const acfdtd_hip_desc &, int64_t, int64_t, bool,
                                         hipStream_t);

@@ -1,6 +1,7 @@
 // The two-step sweep: kernel, dispatch, launch of the leading multiple of four steps. Templates
-// only; fused2.hip and fused2_64.hip instantiate them for fp32 (4 cells per lane) and fp64
-// (2 cells per lane).
+// only; fused2.hip and fused2_64.hip instantiate the launchers and the kernels of 16-byte
+// lanes (4 cells fp32, 2 fp64), fused2_v2.hip, fused2_v1.hip and fused2_64_v1.hip those of
+// narrower lanes (nz not a multiple of 4 or 2), fused2s.hip and fused2s_64.hip the staged ones.
 
 #pragma once
 
@@ -80,15 +81,8 @@ template <typename real>
 using Fused2Kernel = void (*)(const SweepArgs<real>, const Fused2IO<real>, const Tiling2,
                               const int64_t, const int64_t);
 
-// Lane width of the two-step sweep: 16-byte lanes (4 cells fp32, 2 fp64).
-template <typename real>
-constexpr int fused2_vec() {
-    return sizeof(real) == 4 ? 4 : 2;
-}
-
-template <typename real, int NR, bool STAGED>
+template <typename real, int V, int NR, bool STAGED>
 Fused2Kernel<real> pick_fused2_nr(bool walls, bool layer, bool comp) {
-    constexpr int V = fused2_vec<real>();
     if constexpr (sizeof(real) == 4) {
         if (comp) {
             if (walls) return layer ? fused2_step<real, V, NR, true, true, true, STAGED>
@@ -104,26 +98,33 @@ Fused2Kernel<real> pick_fused2_nr(bool walls, bool layer, bool comp) {
                  : fused2_step<real, V, NR, false, false, false, STAGED>;
 }
 
-template <typename real, bool STAGED>
+// The kernel for a lane width V (cells per lane), plain or staged.
+template <typename real, int V, bool STAGED>
 Fused2Kernel<real> pick_fused2_kind(int nr, bool walls, bool layer, bool comp) {
     switch (nr) {
-        case 0: return pick_fused2_nr<real, 0, STAGED>(walls, layer, comp);
-        case 1: return pick_fused2_nr<real, 1, STAGED>(walls, layer, comp);
-        case 2: return pick_fused2_nr<real, 2, STAGED>(walls, layer, comp);
-        case 3: return pick_fused2_nr<real, 3, STAGED>(walls, layer, comp);
-        default: return pick_fused2_nr<real, 4, STAGED>(walls, layer, comp);
+        case 0: return pick_fused2_nr<real, V, 0, STAGED>(walls, layer, comp);
+        case 1: return pick_fused2_nr<real, V, 1, STAGED>(walls, layer, comp);
+        case 2: return pick_fused2_nr<real, V, 2, STAGED>(walls, layer, comp);
+        case 3: return pick_fused2_nr<real, V, 3, STAGED>(walls, layer, comp);
+        default: return pick_fused2_nr<real, V, 4, STAGED>(walls, layer, comp);
     }
 }
 
-// The staged kernels, instantiated in their own translation units (fused2s.hip,
-// fused2s_64.hip) so that they compile in parallel with the others.
-template <typename real>
-Fused2Kernel<real> pick_fused2_staged(int nr, bool walls, bool layer, bool comp);
+// Each (precision, lane width, staged) family of kernels is instantiated in a translation unit
+// of its own (fused2*.hip), so that they compile in parallel; elsewhere they are only declared.
+extern template Fused2Kernel<float> pick_fused2_kind<float, 4, false>(int, bool, bool, bool);
+extern template Fused2Kernel<float> pick_fused2_kind<float, 4, true>(int, bool, bool, bool);
+extern template Fused2Kernel<float> pick_fused2_kind<float, 2, false>(int, bool, bool, bool);
+extern template Fused2Kernel<float> pick_fused2_kind<float, 1, false>(int, bool, bool, bool);
+extern template Fused2Kernel<double> pick_fused2_kind<double, 2, false>(int, bool, bool, bool);
+extern template Fused2Kernel<double> pick_fused2_kind<double, 2, true>(int, bool, bool, bool);
+extern template Fused2Kernel<double> pick_fused2_kind<double, 1, false>(int, bool, bool, bool);
 
-// Whether the staged variant may run: asked for, and every field it stages 16-byte aligned in
-// both sets (otherwise the plain two-step sweep runs, with the same results).
-inline bool staged_applies(const acfdtd_hip_desc &d) {
-    if (!(d.flags & ACFDTD_HIP_STAGED)) return false;
+// Whether the staged variant may run: asked for, 16-byte lanes (4 cells fp32, 2 fp64), and
+// every field it stages 16-byte aligned in both sets (otherwise the plain two-step sweep runs,
+// with the same results).
+inline bool staged_applies(const acfdtd_hip_desc &d, const int vec) {
+    if (!(d.flags & ACFDTD_HIP_STAGED) || vec * d.real_bytes != 16) return false;
     const void *fields[] = {d.p, d.vx, d.vy, d.p_swap, d.vx_swap, d.vy_swap};
     for (const void *f : fields)
         if (reinterpret_cast<uintptr_t>(f) % 16 != 0) return false;
@@ -133,11 +134,23 @@ inline bool staged_applies(const acfdtd_hip_desc &d) {
     return true;
 }
 
+// The two-step sweep's kernel for d: lanes as wide as nz allows (as the fused step's).
 template <typename real>
 Fused2Kernel<real> pick_fused2(const acfdtd_hip_desc &d) {
+    const int vec = fused_vec(d);
     const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
-    if (staged_applies(d)) return pick_fused2_staged<real>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
-    return pick_fused2_kind<real, false>(d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    const bool walls = d.wall_mask != 0, layer = d.has_layer != 0;
+    const bool staged = staged_applies(d, vec);
+    if constexpr (sizeof(real) == 4) {
+        if (vec == 4) return staged ? pick_fused2_kind<real, 4, true>(d.n_relax, walls, layer, comp)
+                                    : pick_fused2_kind<real, 4, false>(d.n_relax, walls, layer, comp);
+        if (vec == 2) return pick_fused2_kind<real, 2, false>(d.n_relax, walls, layer, comp);
+        return pick_fused2_kind<real, 1, false>(d.n_relax, walls, layer, comp);
+    } else {
+        if (vec == 2) return staged ? pick_fused2_kind<real, 2, true>(d.n_relax, walls, layer, comp)
+                                    : pick_fused2_kind<real, 2, false>(d.n_relax, walls, layer, comp);
+        return pick_fused2_kind<real, 1, false>(d.n_relax, walls, layer, comp);
+    }
 }
 
 // The two directions of a two-step sweep: forward reads the descriptor's fields and writes the
@@ -167,7 +180,7 @@ template <typename real>
 int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
                             int64_t column, hipStream_t stream) {
     Tiling2 t2;
-    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2)) return 0;
+    if (!fused2_applies(d) || !make_tiling2(d, fused_vec(d), t2)) return 0;
     SweepArgs<real> a;
     fill_args(d, a);
     const Fused2Kernel<real> k = pick_fused2<real>(d);
@@ -189,7 +202,7 @@ template <typename real>
 int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, bool reverse,
                        hipStream_t stream) {
     Tiling2 t2;
-    if (!fused2_applies(d) || !make_tiling2(d, fused2_vec<real>(), t2))
+    if (!fused2_applies(d) || !make_tiling2(d, fused_vec(d), t2))
         return fail(ACFDTD_HIP_EINVAL, "the two-step sweep does not apply to this descriptor");
     SweepArgs<real> a;
     fill_args(d, a);

@@ -1,0 +1,10 @@
+// Two-step sweep, fp64: the kernels of 1-cell lanes (odd nz)
+// (templates in fused2.hpp and marches2.hpp).
+
+#include "fused2.hpp"
+
+namespace acfdtd {
+
+template Fused2Kernel<double> pick_fused2_kind<double, 1, false>(int, bool, bool, bool);
+
+}  // namespace acfdtd

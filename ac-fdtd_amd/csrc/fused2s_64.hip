@@ -1,14 +1,10 @@
-// fp64 instantiation of the staged two-step sweep (templates in fused2.hpp and marches2.hpp).
+// Two-step sweep, fp64: the staged kernels (2-cell lanes)
+// (templates in fused2.hpp and marches2.hpp).
 
 #include "fused2.hpp"
 
 namespace acfdtd {
 
-template <typename real>
-Fused2Kernel<real> pick_fused2_staged(int nr, bool walls, bool layer, bool comp) {
-    return pick_fused2_kind<real, true>(nr, walls, layer, comp);
-}
-
-template Fused2Kernel<double> pick_fused2_staged<double>(int, bool, bool, bool);
+template Fused2Kernel<double> pick_fused2_kind<double, 2, true>(int, bool, bool, bool);
 
 }  // namespace acfdtd

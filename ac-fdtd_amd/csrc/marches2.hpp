@@ -97,8 +97,16 @@ template <>
 struct StageRead16<6> {
     __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
                                                u32x4 (&o)[6]) {
-        asm volatile("ds_read_b128 %0, %6 offset:0\n\tds_read_b128 %1, %6 offset:8192\n\tds_read_b128 %2, %6 offset:16384\n\tds_read_b128 %3, %6 offset:24576\n\tds_read_b128 %4, %7 offset:0\n\tds_read_b128 %5, %7 offset:8192\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5])
+        asm volatile(
+            "ds_read_b128 %0, %6 offset:0\n\t"
+            "ds_read_b128 %1, %6 offset:8192\n\t"
+            "ds_read_b128 %2, %6 offset:16384\n\t"
+            "ds_read_b128 %3, %6 offset:24576\n\t"
+            "ds_read_b128 %4, %7 offset:0\n\t"
+            "ds_read_b128 %5, %7 offset:8192\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5])
                      : "v"(base0), "v"(base4));
     }
 };
@@ -107,8 +115,17 @@ template <>
 struct StageRead16<7> {
     __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
                                                u32x4 (&o)[7]) {
-        asm volatile("ds_read_b128 %0, %7 offset:0\n\tds_read_b128 %1, %7 offset:8192\n\tds_read_b128 %2, %7 offset:16384\n\tds_read_b128 %3, %7 offset:24576\n\tds_read_b128 %4, %8 offset:0\n\tds_read_b128 %5, %8 offset:8192\n\tds_read_b128 %6, %8 offset:16384\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6])
+        asm volatile(
+            "ds_read_b128 %0, %7 offset:0\n\t"
+            "ds_read_b128 %1, %7 offset:8192\n\t"
+            "ds_read_b128 %2, %7 offset:16384\n\t"
+            "ds_read_b128 %3, %7 offset:24576\n\t"
+            "ds_read_b128 %4, %8 offset:0\n\t"
+            "ds_read_b128 %5, %8 offset:8192\n\t"
+            "ds_read_b128 %6, %8 offset:16384\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6])
                      : "v"(base0), "v"(base4));
     }
 };
@@ -117,8 +134,18 @@ template <>
 struct StageRead16<8> {
     __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
                                                u32x4 (&o)[8]) {
-        asm volatile("ds_read_b128 %0, %8 offset:0\n\tds_read_b128 %1, %8 offset:8192\n\tds_read_b128 %2, %8 offset:16384\n\tds_read_b128 %3, %8 offset:24576\n\tds_read_b128 %4, %9 offset:0\n\tds_read_b128 %5, %9 offset:8192\n\tds_read_b128 %6, %9 offset:16384\n\tds_read_b128 %7, %9 offset:24576\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7])
+        asm volatile(
+            "ds_read_b128 %0, %8 offset:0\n\t"
+            "ds_read_b128 %1, %8 offset:8192\n\t"
+            "ds_read_b128 %2, %8 offset:16384\n\t"
+            "ds_read_b128 %3, %8 offset:24576\n\t"
+            "ds_read_b128 %4, %9 offset:0\n\t"
+            "ds_read_b128 %5, %9 offset:8192\n\t"
+            "ds_read_b128 %6, %9 offset:16384\n\t"
+            "ds_read_b128 %7, %9 offset:24576\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7])
                      : "v"(base0), "v"(base4));
     }
 };
@@ -127,8 +154,19 @@ template <>
 struct StageRead16<9> {
     __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
                                                u32x4 (&o)[9]) {
-        asm volatile("ds_read_b128 %0, %9 offset:0\n\tds_read_b128 %1, %9 offset:8192\n\tds_read_b128 %2, %9 offset:16384\n\tds_read_b128 %3, %9 offset:24576\n\tds_read_b128 %4, %10 offset:0\n\tds_read_b128 %5, %10 offset:8192\n\tds_read_b128 %6, %10 offset:16384\n\tds_read_b128 %7, %10 offset:24576\n\tds_read_b128 %8, %10 offset:32768\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8])
+        asm volatile(
+            "ds_read_b128 %0, %9 offset:0\n\t"
+            "ds_read_b128 %1, %9 offset:8192\n\t"
+            "ds_read_b128 %2, %9 offset:16384\n\t"
+            "ds_read_b128 %3, %9 offset:24576\n\t"
+            "ds_read_b128 %4, %10 offset:0\n\t"
+            "ds_read_b128 %5, %10 offset:8192\n\t"
+            "ds_read_b128 %6, %10 offset:16384\n\t"
+            "ds_read_b128 %7, %10 offset:24576\n\t"
+            "ds_read_b128 %8, %10 offset:32768\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8])
                      : "v"(base0), "v"(base4));
     }
 };
@@ -137,8 +175,20 @@ template <>
 struct StageRead16<10> {
     __device__ __forceinline__ static void run(const uint32_t base0, const uint32_t base4,
                                                u32x4 (&o)[10]) {
-        asm volatile("ds_read_b128 %0, %10 offset:0\n\tds_read_b128 %1, %10 offset:8192\n\tds_read_b128 %2, %10 offset:16384\n\tds_read_b128 %3, %10 offset:24576\n\tds_read_b128 %4, %11 offset:0\n\tds_read_b128 %5, %11 offset:8192\n\tds_read_b128 %6, %11 offset:16384\n\tds_read_b128 %7, %11 offset:24576\n\tds_read_b128 %8, %11 offset:32768\n\tds_read_b128 %9, %11 offset:40960\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
+        asm volatile(
+            "ds_read_b128 %0, %10 offset:0\n\t"
+            "ds_read_b128 %1, %10 offset:8192\n\t"
+            "ds_read_b128 %2, %10 offset:16384\n\t"
+            "ds_read_b128 %3, %10 offset:24576\n\t"
+            "ds_read_b128 %4, %11 offset:0\n\t"
+            "ds_read_b128 %5, %11 offset:8192\n\t"
+            "ds_read_b128 %6, %11 offset:16384\n\t"
+            "ds_read_b128 %7, %11 offset:24576\n\t"
+            "ds_read_b128 %8, %11 offset:32768\n\t"
+            "ds_read_b128 %9, %11 offset:40960\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
                      : "v"(base0), "v"(base4));
     }
 };
@@ -146,8 +196,17 @@ struct StageRead16<10> {
 template <int NF>
 struct StageRead4<NF, 7> {
     __device__ __forceinline__ static void run(const uint32_t base, uint32_t (&o)[7]) {
-        asm volatile("ds_read_b32 %0, %7 offset:0\n\tds_read_b32 %1, %7 offset:2048\n\tds_read_b32 %2, %7 offset:4096\n\tds_read_b32 %3, %7 offset:6144\n\tds_read_b32 %4, %7 offset:8192\n\tds_read_b32 %5, %7 offset:10240\n\tds_read_b32 %6, %7 offset:12288\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6])
+        asm volatile(
+            "ds_read_b32 %0, %7 offset:0\n\t"
+            "ds_read_b32 %1, %7 offset:2048\n\t"
+            "ds_read_b32 %2, %7 offset:4096\n\t"
+            "ds_read_b32 %3, %7 offset:6144\n\t"
+            "ds_read_b32 %4, %7 offset:8192\n\t"
+            "ds_read_b32 %5, %7 offset:10240\n\t"
+            "ds_read_b32 %6, %7 offset:12288\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6])
                      : "v"(base));
     }
 };
@@ -155,8 +214,20 @@ struct StageRead4<NF, 7> {
 template <int NF>
 struct StageRead4<NF, 10> {
     __device__ __forceinline__ static void run(const uint32_t base, uint32_t (&o)[10]) {
-        asm volatile("ds_read_b32 %0, %10 offset:0\n\tds_read_b32 %1, %10 offset:2048\n\tds_read_b32 %2, %10 offset:4096\n\tds_read_b32 %3, %10 offset:6144\n\tds_read_b32 %4, %10 offset:8192\n\tds_read_b32 %5, %10 offset:10240\n\tds_read_b32 %6, %10 offset:12288\n\tds_read_b32 %7, %10 offset:14336\n\tds_read_b32 %8, %10 offset:16384\n\tds_read_b32 %9, %10 offset:18432\n\ts_waitcnt lgkmcnt(0)"
-                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
+        asm volatile(
+            "ds_read_b32 %0, %10 offset:0\n\t"
+            "ds_read_b32 %1, %10 offset:2048\n\t"
+            "ds_read_b32 %2, %10 offset:4096\n\t"
+            "ds_read_b32 %3, %10 offset:6144\n\t"
+            "ds_read_b32 %4, %10 offset:8192\n\t"
+            "ds_read_b32 %5, %10 offset:10240\n\t"
+            "ds_read_b32 %6, %10 offset:12288\n\t"
+            "ds_read_b32 %7, %10 offset:14336\n\t"
+            "ds_read_b32 %8, %10 offset:16384\n\t"
+            "ds_read_b32 %9, %10 offset:18432\n\t"
+            "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]),
+                       "=&v"(o[5]), "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9])
                      : "v"(base));
     }
 };
@@ -629,67 +700,71 @@ __device__ __forceinline__ void fused2_march(
     constexpr int WPR = (int)(sizeof(real) / 4);  // 4-byte words per value
     const bool wave_owner = __ballot(owner) != 0;
     auto stage_issue = [&](const int s, auto xwall_tag) __attribute__((always_inline)) {
-        constexpr bool XWALL = decltype(xwall_tag)::value;
-        const int64_t cc = L.cell(s);
-        const bool up_interior = !XWALL || s + 1 < nx;
-        unsigned char *base = stage + (t & ~63) * 16;  // this wavefront's slots of field 0
-        load_to_lds16(vx_in + cc + L.plane, base);
-        load_to_lds16(p_in + cc + (up_interior ? L.plane : 0), base + kStageField);
-        const int64_t cy = L.face_y(s);
-        load_to_lds16(vy_in + cy, base + 2 * kStageField);
-        load_to_lds16(vy_in + cy + nz, base + 3 * kStageField);
-        load_to_lds16(p_in + cc - c.dy_lo, base + 4 * kStageField);
-        load_to_lds16(p_in + cc + c.dy_hi, base + 5 * kStageField);
+        if constexpr (STAGED) {
+            constexpr bool XWALL = decltype(xwall_tag)::value;
+            const int64_t cc = L.cell(s);
+            const bool up_interior = !XWALL || s + 1 < nx;
+            unsigned char *base = stage + (t & ~63) * 16;  // this wavefront's slots of field 0
+            load_to_lds16(vx_in + cc + L.plane, base);
+            load_to_lds16(p_in + cc + (up_interior ? L.plane : 0), base + kStageField);
+            const int64_t cy = L.face_y(s);
+            load_to_lds16(vy_in + cy, base + 2 * kStageField);
+            load_to_lds16(vy_in + cy + nz, base + 3 * kStageField);
+            load_to_lds16(p_in + cc - c.dy_lo, base + 4 * kStageField);
+            load_to_lds16(p_in + cc + c.dy_hi, base + 5 * kStageField);
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
-            load_to_lds16(psi_in + (int64_t)r * n_cells + cc, base + (6 + r) * kStageField);
-        unsigned char *words = stage + NF * kStageField + (t & ~63) * 4;
-        auto word_load = [&](const real *src, const int w) __attribute__((always_inline)) {
+            for (int r = 0; r < NR; ++r)
+                load_to_lds16(psi_in + (int64_t)r * n_cells + cc, base + (6 + r) * kStageField);
+            unsigned char *words = stage + NF * kStageField + (t & ~63) * 4;
+            auto word_load = [&](const real *src, const int w) __attribute__((always_inline)) {
 #pragma unroll
-            for (int h = 0; h < WPR; ++h)
-                load_to_lds4(reinterpret_cast<const unsigned char *>(src) + 4 * h,
-                             words + (w * WPR + h) * kStageWord);
-        };
-        const int64_t cz = L.face_z(s);
+                for (int h = 0; h < WPR; ++h)
+                    load_to_lds4(reinterpret_cast<const unsigned char *>(src) + 4 * h,
+                                 words + (w * WPR + h) * kStageWord);
+            };
+            const int64_t cz = L.face_z(s);
 #pragma unroll
-        for (int e = 0; e <= V; ++e) word_load(vz_in + cz + e, e);
-        word_load(p_in + cc - c.dz_lo, V + 1);
-        word_load(p_in + cc + c.dz_hi, V + 2);
+            for (int e = 0; e <= V; ++e) word_load(vz_in + cz + e, e);
+            word_load(p_in + cc - c.dz_lo, V + 1);
+            word_load(p_in + cc + c.dz_hi, V + 2);
+        }
     };
     auto stage_read = [&](Step1In<real, V, NR> &in) __attribute__((always_inline)) {
-        // The stage was requested before the previous plane's second step, whose 3 + NR
-        // unconditional stores (vy, vz, psi, p) a wavefront with an owner lane issued after it:
-        // with at most that many operations outstanding, the stage has landed. A wavefront
-        // with no owner lane (halo rows) stored nothing since, and waits for everything.
-        if (wave_owner) wait_vm<3 + NR>();
-        else wait_memory();
-        const uint32_t a16 = lds_address(stage) + (uint32_t)t * 16;
-        u32x4 raw[NF];
-        StageRead16<NF>::run(a16, a16 + 4 * kStageField, raw);
-        uint32_t w[NS];
-        StageRead4<NF, NS>::run(lds_address(stage) + NF * kStageField + (uint32_t)t * 4, w);
-        static_assert(sizeof(real) * V == sizeof(u32x4), "a lane's staged field is 16 bytes");
-        auto take = [&](const int f, real (&dst)[V]) __attribute__((always_inline)) {
-            __builtin_memcpy(dst, &raw[f], sizeof(u32x4));
-        };
-        take(0, in.vx_v);
-        take(1, in.p_up);
-        take(2, in.vy_lo);
-        take(3, in.vy_hi);
-        take(4, in.q_lo);
-        take(5, in.q_hi);
+        if constexpr (STAGED) {
+            // The stage was requested before the previous plane's second step, whose 3 + NR
+            // unconditional stores (vy, vz, psi, p) a wavefront with an owner lane issued after it:
+            // with at most that many operations outstanding, the stage has landed. A wavefront
+            // with no owner lane (halo rows) stored nothing since, and waits for everything.
+            if (wave_owner) wait_vm<3 + NR>();
+            else wait_memory();
+            const uint32_t a16 = lds_address(stage) + (uint32_t)t * 16;
+            u32x4 raw[NF];
+            StageRead16<NF>::run(a16, a16 + 4 * kStageField, raw);
+            uint32_t w[NS];
+            StageRead4<NF, NS>::run(lds_address(stage) + NF * kStageField + (uint32_t)t * 4, w);
+            static_assert(sizeof(real) * V == sizeof(u32x4), "a lane's staged field is 16 bytes");
+            auto take = [&](const int f, real (&dst)[V]) __attribute__((always_inline)) {
+                __builtin_memcpy(dst, &raw[f], sizeof(u32x4));
+            };
+            take(0, in.vx_v);
+            take(1, in.p_up);
+            take(2, in.vy_lo);
+            take(3, in.vy_hi);
+            take(4, in.q_lo);
+            take(5, in.q_hi);
 #pragma unroll
-        for (int r = 0; r < NR; ++r) take(6 + r, in.psi[r]);
-        auto value = [&](const int k) __attribute__((always_inline)) {
-            real v;
-            __builtin_memcpy(&v, &w[k * WPR], sizeof(real));
-            return v;
-        };
+            for (int r = 0; r < NR; ++r) take(6 + r, in.psi[r]);
+            auto value = [&](const int k) __attribute__((always_inline)) {
+                real v;
+                __builtin_memcpy(&v, &w[k * WPR], sizeof(real));
+                return v;
+            };
 #pragma unroll
-        for (int e = 0; e < V; ++e) in.vz_v[e] = value(e);
-        in.vz_top = value(V);
-        in.pz_lo = value(V + 1);
-        in.pz_hi = value(V + 2);
+            for (int e = 0; e < V; ++e) in.vz_v[e] = value(e);
+            in.vz_top = value(V);
+            in.pz_lo = value(V + 1);
+            in.pz_hi = value(V + 2);
+        }
     };
     auto stage_issue_at = [&](const int s) __attribute__((always_inline)) {
         if (s == nx - 1) stage_issue(s, Wall{});

@@ -22,7 +22,8 @@ pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("require_gpu")]
 
 # (shape, tile_j, chunk_x): blocks of 2 tile_j rows; whole rows of nz / V lane groups when
 # (2 tile_j + 2) rows fit in 512 lanes, else row segments with a halo lane group at each end
-# (the last two cases, and nz = 512 in fp32 / fp64).
+# (nz = 512 and the 37-row case, and nz = 87 with 1-cell lanes). V is the widest of 4, 2, 1
+# (fp32) or 2, 1 (fp64) that divides nz.
 CASES = [
     ((9, 11, 12), 4, 8),
     ((13, 40, 64), 1, 3),
@@ -32,6 +33,9 @@ CASES = [
     ((34, 30, 128), 2, 4),
     ((9, 13, 512), 2, 4),
     ((11, 37, 128), 8, 4),
+    ((10, 9, 10), 2, 3),   # fp32 lanes of 2 cells
+    ((12, 11, 15), 1, 4),  # lanes of 1 cell
+    ((9, 13, 87), 2, 4),   # 1-cell lanes, rows of 87 lanes in segments
 ]
 
 
@@ -116,10 +120,10 @@ def test_two_step_compensated_fp32():
     _assert_same(one, two)
 
 
-def test_two_step_falls_back_and_writes_the_tape():
-    """nz not a multiple of the lane width runs single steps; with a boundary tape attached the
-    two-step sweep writes the same tape rows as single steps."""
-    shape = (10, 9, 10)  # fp32 lanes of 4 do not divide nz = 10
+def test_two_step_narrow_lanes_and_the_tape():
+    """nz not a multiple of 4 runs two steps per sweep on narrower lanes; with a boundary tape
+    attached the two-step sweep writes the same tape rows as single steps."""
+    shape = (10, 9, 10)  # fp32 lanes of 2 cells
     one = _solver(shape, np.float32, CONFIGS["all"], 1, 0, 0)
     two = _solver(shape, np.float32, CONFIGS["all"], 2, 0, 0)
     one.run(9)
