@@ -51,6 +51,11 @@ class _Group:
         _native.check(status, "acfdtd_hip_batch_create")
         self._handle = handle
 
+    @property
+    def two_step(self) -> bool:
+        """Whether this group advances two steps per sweep."""
+        return bool(self._lib.acfdtd_hip_batch_two_step(self._handle))
+
     def _descriptors(self, recordings):
         array = (_native.Desc * len(self.solvers))()
         for n, solver in enumerate(self.solvers):
@@ -95,6 +100,8 @@ def _bind(lib) -> None:
     lib.acfdtd_hip_batch_run.restype = ctypes.c_int
     lib.acfdtd_hip_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                          ctypes.c_int64, ctypes.c_void_p]
+    lib.acfdtd_hip_batch_two_step.restype = ctypes.c_int
+    lib.acfdtd_hip_batch_two_step.argtypes = [ctypes.c_void_p]
     lib.acfdtd_hip_batch_destroy.restype = None
     lib.acfdtd_hip_batch_destroy.argtypes = [ctypes.c_void_p]
 
@@ -135,6 +142,11 @@ class BatchedRooms:
     @property
     def step_index(self) -> int:
         return self.solvers[0].step_index
+
+    @property
+    def two_step(self) -> list[bool]:
+        """Per lane-width group, whether it advances two steps per sweep."""
+        return [group.two_step for group in self._groups]
 
     @property
     def n_cells(self) -> int:

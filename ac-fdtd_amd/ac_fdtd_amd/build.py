@@ -46,6 +46,11 @@ SOURCES = (
     "fused2_v2.hip",
     "fused2_v1.hip",
     "fused2_64_v1.hip",
+    "batch2.hip",
+    "batch2_v2.hip",
+    "batch2_v1.hip",
+    "batch2_64.hip",
+    "batch2_64_v1.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

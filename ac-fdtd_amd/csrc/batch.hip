@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "acfdtd_hip.h"
+#include "batch2.hpp"
 #include "marches.hpp"
 #include "sweeps.hpp"
 
@@ -149,6 +150,55 @@ std::string pack_rooms(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
     return blob;
 }
 
+// Whether every room may advance two steps per sweep (fused2_applies) with the batch's lane
+// width; then the number of (room, tile) entries and the widest room workgroup.
+inline bool batch_two_step(const acfdtd_hip_desc *rooms, int n_rooms, int vec, int64_t *n_tiles,
+                           int *threads) {
+    int64_t total = 0;
+    int widest = 64;
+    for (int r = 0; r < n_rooms; ++r) {
+        Tiling2 t2;
+        if (!fused2_applies(rooms[r]) || !make_tiling2(rooms[r], vec, t2)) return false;
+        total += t2.tiles;
+        widest = std::max(widest, t2.threads);
+    }
+    if (total > (int64_t)INT32_MAX - 8) return false;
+    *n_tiles = total;
+    *threads = widest;
+    return true;
+}
+
+inline size_t batch_room2_bytes(int real_bytes) {
+    return real_bytes == 8 ? sizeof(BatchRoom2<double>) : sizeof(BatchRoom2<float>);
+}
+
+template <typename real>
+std::string pack_rooms2(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
+    std::string blob(sizeof(BatchRoom2<real>) * (size_t)n_rooms, '\0');
+    for (int r = 0; r < n_rooms; ++r) {
+        BatchRoom2<real> room;
+        std::memset(&room, 0, sizeof(room));
+        fill_args(rooms[r], room.args);
+        fused2_io<real>(rooms[r], room.forward, room.backward);
+        make_tiling2(rooms[r], vec, room.t2);
+        std::memcpy(&blob[sizeof(room) * (size_t)r], &room, sizeof(room));
+    }
+    return blob;
+}
+
+std::string pack_tiles2(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
+    std::string blob;
+    for (int r = 0; r < n_rooms; ++r) {
+        Tiling2 t2;
+        make_tiling2(rooms[r], vec, t2);
+        for (int b = 0; b < t2.tiles; ++b) {
+            const BatchTile2 tile{r, b};
+            blob.append(reinterpret_cast<const char *>(&tile), sizeof(tile));
+        }
+    }
+    return blob;
+}
+
 std::string pack_tiles(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
     std::string blob;
     for (int r = 0; r < n_rooms; ++r) {
@@ -177,6 +227,11 @@ struct acfdtd_hip_batch {
     bool walls, layer, comp;
     void *rooms;  // device: n_rooms BatchRoom<real>
     void *tiles;  // device: n_tiles BatchTile
+    // Two steps per sweep, when every room allows it: its own room and tile tables.
+    bool two_step;
+    int n_tiles2, launched2, threads2;
+    void *rooms2;  // device: n_rooms BatchRoom2<real>
+    void *tiles2;  // device: n_tiles2 BatchTile2
     std::vector<acfdtd_hip_desc> descs;
 };
 
@@ -199,6 +254,47 @@ int batch_pairs(const acfdtd_hip_batch &b, int64_t step_offset, int64_t n_pairs,
     return check_hip(hipGetLastError(), "batch launch");
 }
 
+// Two-step sweeps for the leading multiple of four of n_steps (A -> B, B -> A); returns the
+// number of steps done.
+template <typename real>
+int64_t batch_quads(const acfdtd_hip_batch &b, int64_t step_offset, int64_t n_steps,
+                    int64_t column, hipStream_t stream) {
+    const Batch2Kernel<real> k = pick_batch2<real>(b.vec, b.n_relax, b.walls, b.layer, b.comp);
+    const BatchRoom2<real> *rooms = static_cast<const BatchRoom2<real> *>(b.rooms2);
+    const BatchTile2 *tiles = static_cast<const BatchTile2 *>(b.tiles2);
+    const dim3 block(b.threads2), grid(b.launched2);
+    int64_t n = 0;
+    for (; n + 3 < n_steps; n += 4) {
+        hipLaunchKernelGGL(k, grid, block, 0, stream, rooms, tiles, b.n_tiles2, b.launched2, 0,
+                           step_offset + n, column + n);
+        hipLaunchKernelGGL(k, grid, block, 0, stream, rooms, tiles, b.n_tiles2, b.launched2, 1,
+                           step_offset + n + 2, column + n + 2);
+    }
+    return n;
+}
+
+// Sizes of the workspace parts: one-step room table, tile table, and the two-step ones.
+struct BatchLayout {
+    size_t rooms, tiles, rooms2, tiles2;
+    bool two_step;
+    int64_t n_tiles2;
+    int threads2;
+    size_t total() const { return rooms + tiles + rooms2 + tiles2; }
+};
+
+inline BatchLayout batch_layout(const acfdtd_hip_desc *rooms, int n_rooms, int64_t n_tiles) {
+    BatchLayout l{};
+    const int vec = batch_vec(rooms, n_rooms);
+    l.rooms = align_up(batch_room_bytes(rooms[0].real_bytes) * (size_t)n_rooms, 256);
+    l.tiles = align_up(sizeof(BatchTile) * (size_t)n_tiles, 256);
+    l.two_step = batch_two_step(rooms, n_rooms, vec, &l.n_tiles2, &l.threads2);
+    if (l.two_step) {
+        l.rooms2 = align_up(batch_room2_bytes(rooms[0].real_bytes) * (size_t)n_rooms, 256);
+        l.tiles2 = sizeof(BatchTile2) * (size_t)l.n_tiles2;
+    }
+    return l;
+}
+
 }  // namespace acfdtd
 
 using namespace acfdtd;
@@ -206,8 +302,7 @@ using namespace acfdtd;
 extern "C" size_t acfdtd_hip_batch_workspace_bytes(const acfdtd_hip_desc *rooms, int32_t n_rooms) {
     int64_t n_tiles = 0;
     if (batch_validate(rooms, n_rooms, &n_tiles) != ACFDTD_HIP_OK) return 0;
-    return align_up(batch_room_bytes(rooms[0].real_bytes) * (size_t)n_rooms, 256) +
-           sizeof(BatchTile) * (size_t)n_tiles;
+    return batch_layout(rooms, n_rooms, n_tiles).total();
 }
 
 extern "C" int acfdtd_hip_batch_create(const acfdtd_hip_desc *rooms, int32_t n_rooms,
@@ -218,8 +313,8 @@ extern "C" int acfdtd_hip_batch_create(const acfdtd_hip_desc *rooms, int32_t n_r
     int64_t n_tiles = 0;
     int status = batch_validate(rooms, n_rooms, &n_tiles);
     if (status != ACFDTD_HIP_OK) return status;
-    const size_t room_bytes = align_up(batch_room_bytes(rooms[0].real_bytes) * (size_t)n_rooms, 256);
-    if (workspace == nullptr || workspace_bytes < room_bytes + sizeof(BatchTile) * (size_t)n_tiles)
+    const BatchLayout layout = batch_layout(rooms, n_rooms, n_tiles);
+    if (workspace == nullptr || workspace_bytes < layout.total())
         return fail(ACFDTD_HIP_EINVAL, "batch workspace is NULL or too small");
     acfdtd_hip_batch *b = new acfdtd_hip_batch{};
     b->n_rooms = n_rooms;
@@ -233,12 +328,26 @@ extern "C" int acfdtd_hip_batch_create(const acfdtd_hip_desc *rooms, int32_t n_r
     b->comp = (rooms[0].flags & ACFDTD_HIP_COMPENSATED) != 0;
     b->walls = false;
     for (int r = 0; r < n_rooms; ++r) b->walls = b->walls || rooms[r].wall_mask != 0;
-    b->rooms = workspace;
-    b->tiles = static_cast<char *>(workspace) + room_bytes;
+    char *base = static_cast<char *>(workspace);
+    b->rooms = base;
+    b->tiles = base + layout.rooms;
+    b->two_step = layout.two_step;
+    b->n_tiles2 = (int)layout.n_tiles2;
+    b->launched2 = ceil_div(layout.n_tiles2, 8) * 8;
+    b->threads2 = layout.threads2;
+    b->rooms2 = layout.two_step ? base + layout.rooms + layout.tiles : nullptr;
+    b->tiles2 = layout.two_step ? base + layout.rooms + layout.tiles + layout.rooms2 : nullptr;
     const std::string tiles = pack_tiles(rooms, n_rooms, b->vec);
     hipStream_t s = static_cast<hipStream_t>(stream);
     status = check_hip(hipMemcpyAsync(b->tiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice, s),
                        "batch tile upload");
+    std::string tiles2;
+    if (status == ACFDTD_HIP_OK && b->two_step) {
+        tiles2 = pack_tiles2(rooms, n_rooms, b->vec);
+        status = check_hip(hipMemcpyAsync(b->tiles2, tiles2.data(), tiles2.size(),
+                                          hipMemcpyHostToDevice, s),
+                           "batch tile upload");
+    }
     if (status == ACFDTD_HIP_OK) status = acfdtd_hip_batch_update(b, rooms, n_rooms, stream);
     if (status != ACFDTD_HIP_OK) {
         delete b;
@@ -257,13 +366,25 @@ extern "C" int acfdtd_hip_batch_update(acfdtd_hip_batch *batch, const acfdtd_hip
     if (n_rooms != batch->n_rooms || n_tiles != batch->n_tiles ||
         rooms[0].real_bytes != batch->real_bytes || batch_vec(rooms, n_rooms) != batch->vec)
         return fail(ACFDTD_HIP_EINVAL, "rooms do not match the batch they update");
+    const BatchLayout layout = batch_layout(rooms, n_rooms, n_tiles);
+    if (layout.two_step != batch->two_step ||
+        (layout.two_step && (layout.n_tiles2 != batch->n_tiles2 || layout.threads2 != batch->threads2)))
+        return fail(ACFDTD_HIP_EINVAL, "rooms do not match the batch they update");
     const std::string blob = rooms[0].real_bytes == 8
                                  ? pack_rooms<double>(rooms, n_rooms, batch->vec)
                                  : pack_rooms<float>(rooms, n_rooms, batch->vec);
+    std::string blob2;
+    if (batch->two_step)
+        blob2 = rooms[0].real_bytes == 8 ? pack_rooms2<double>(rooms, n_rooms, batch->vec)
+                                         : pack_rooms2<float>(rooms, n_rooms, batch->vec);
     batch->descs.assign(rooms, rooms + n_rooms);
-    // Synchronous, so the host blob may be freed on return.
+    // Synchronous, so the host blobs may be freed on return.
     hipStream_t s = static_cast<hipStream_t>(stream);
     int st = check_hip(hipMemcpyAsync(batch->rooms, blob.data(), blob.size(), hipMemcpyHostToDevice, s),
+                       "batch room upload");
+    if (st == ACFDTD_HIP_OK && batch->two_step)
+        st = check_hip(hipMemcpyAsync(batch->rooms2, blob2.data(), blob2.size(),
+                                      hipMemcpyHostToDevice, s),
                        "batch room upload");
     if (st == ACFDTD_HIP_OK) st = check_hip(hipStreamSynchronize(s), "batch room upload");
     return st;
@@ -280,17 +401,30 @@ extern "C" int acfdtd_hip_batch_run(acfdtd_hip_batch *batch, int64_t step_offset
     if (n_steps == 0) return ACFDTD_HIP_OK;
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int64_t pairs = n_steps / 2;
+    int64_t done = 0;
+    if (batch->two_step) {
+        done = batch->real_bytes == 8
+                   ? batch_quads<double>(*batch, step_offset, n_steps, recording_column, s)
+                   : batch_quads<float>(*batch, step_offset, n_steps, recording_column, s);
+        const int status = check_hip(hipGetLastError(), "batch two-step launch");
+        if (status != ACFDTD_HIP_OK) return status;
+    }
+    const int64_t rest = n_steps - done;
+    const int64_t pairs = rest / 2;
     int status = batch->real_bytes == 8
-                     ? batch_pairs<double>(*batch, step_offset, pairs, recording_column, s)
-                     : batch_pairs<float>(*batch, step_offset, pairs, recording_column, s);
-    if (status != ACFDTD_HIP_OK || n_steps % 2 == 0) return status;
+                     ? batch_pairs<double>(*batch, step_offset + done, pairs, recording_column + done, s)
+                     : batch_pairs<float>(*batch, step_offset + done, pairs, recording_column + done, s);
+    if (status != ACFDTD_HIP_OK || rest % 2 == 0) return status;
     for (const acfdtd_hip_desc &d : batch->descs) {
         status = acfdtd_hip_run(&d, step_offset + n_steps - 1, 1, recording_column + n_steps - 1,
                                 stream);
         if (status != ACFDTD_HIP_OK) return status;
     }
     return ACFDTD_HIP_OK;
+}
+
+extern "C" int acfdtd_hip_batch_two_step(const acfdtd_hip_batch *batch) {
+    return batch != nullptr && batch->two_step;
 }
 
 extern "C" void acfdtd_hip_batch_destroy(acfdtd_hip_batch *batch) { delete batch; }

@@ -187,6 +187,10 @@ int acfdtd_hip_batch_update(acfdtd_hip_batch *batch, const acfdtd_hip_desc *room
                             int32_t n_rooms, void *stream);
 int acfdtd_hip_batch_run(acfdtd_hip_batch *batch, int64_t step_offset, int64_t n_steps,
                          int64_t recording_column, void *stream);
+/* 1 when the batch advances its rooms two steps per sweep (every room's descriptor allows it:
+ * time_block = 2 with the swap set and psi_swap, at most 64 sources and 256 receivers), the
+ * leading multiple of four steps of each run; else 0. No reference counterpart. */
+int acfdtd_hip_batch_two_step(const acfdtd_hip_batch *batch);
 void acfdtd_hip_batch_destroy(acfdtd_hip_batch *batch);
 
 /* Stride of a boundary tape row: 2 (ny nz + nx nz + nx ny) elements rounded up to a multiple

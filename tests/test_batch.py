@@ -49,6 +49,7 @@ def test_batch_equals_rooms_alone(dtype, layer, shapes):
     alone = [_room(shape, n, dtype, layer) for n, shape in enumerate(shapes)]
     batched = [_room(shape, n, dtype, layer) for n, shape in enumerate(shapes)]
     batch = BatchedRooms(batched)
+    assert all(batch.two_step)  # every room allows two steps per sweep (any nz)
     for n_steps in (6, 3, 10):
         batch.run(n_steps)
         for solver in alone:
