@@ -32,7 +32,7 @@ template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED
 __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void batch_fused2_step(
     const BatchRoom2<real> *rooms, const BatchTile2 *tiles, const int n_tiles, const int launched,
     const int reverse, const int64_t step, const int64_t column) {
-    __shared__ Fused2Shared<real, V, NR, false> sh;
+    ACFDTD_FUSED2_SHARED(real, V, NR, false);
     const unsigned n = (unsigned)launched;
     unsigned b = blockIdx.x;
     b = (b & 7u) * (n >> 3) + (b >> 3);  // contiguous tiles per XCD, as decode_tile
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void batch
     const BatchRoom2<real> &room = rooms[tile.room];
     fused2_block<real, V, NR, WALLS, LAYER, COMPENSATED, false>(
         room.args, reverse ? room.backward : room.forward, room.t2, (unsigned)tile.tile, step,
-        column, sh);
+        column, lds, src_lds, len_lds, now_lds, rx_lds, stage);
 }
 
 template <typename real>

@@ -22,28 +22,20 @@ namespace acfdtd {
 #define ACFDTD_FUSED2_MIN_WAVES 2
 #endif
 
-// LDS of a two-step workgroup: the exchanged p^(n+1) planes, the source and receiver tables
-// (copied once: the lanes that hold a source or a receiver scan them every plane, and with a
+// One workgroup of the two-step sweep on tile b of a room (b < t2.tiles): the body of
+// fused2_step, also run by batched rooms. The LDS it uses, declared by the kernel: lds, the
+// exchanged p^(n+1) planes (3 x kFused2Threads x V values); the source and receiver tables,
+// copied once (the lanes that hold a source or a receiver scan them every plane, and with a
 // barrier per plane the whole workgroup waits for those lanes; in LDS their reads wait for
 // nothing else, while a read from global memory waits for every store the lane issued before
-// it), and, when STAGED, each lane's slots for the next plane's first-step inputs.
-template <typename real, int V, int NR, bool STAGED>
-struct Fused2Shared {
-    real lds[3 * kFused2Threads * V];
-    int32_t src[2 * kFused2Sources];
-    int64_t len[kFused2Sources];
-    real now[2 * kFused2Sources];  // samples of steps `step` and `step` + 1
-    int32_t rx[2 * kFused2Receivers];
-    alignas(16) unsigned char stage[STAGED ? stage_bytes<real, V, NR>() : 16];
-};
-
-// One workgroup of the two-step sweep on tile b of a room (b < t2.tiles): the body of
-// fused2_step, also run by batched rooms.
+// it); and, when STAGED, each lane's slots for the next plane's first-step inputs.
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
 __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fused2IO<real> &io,
                                              const Tiling2 &t2, const unsigned b,
                                              const int64_t step, const int64_t column,
-                                             Fused2Shared<real, V, NR, STAGED> &sh) {
+                                             real *lds, int32_t *src_lds, int64_t *len_lds,
+                                             real *now_lds, int32_t *rx_lds,
+                                             unsigned char *stage) {
     const int blk = (int)(b % (unsigned)t2.nblk);
     const int xc = (int)(b / (unsigned)t2.nblk);
     const int bg = blk % t2.nblk_g, bj = blk / t2.nblk_g;
@@ -61,34 +53,90 @@ __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fus
     const int i1 = min(i0 + t2.chunk, a.nx);
     // fused2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
     for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
-        sh.src[2 * q] = a.source_cell[2 * q];
-        sh.src[2 * q + 1] = a.source_cell[2 * q + 1];
+        src_lds[2 * q] = a.source_cell[2 * q];
+        src_lds[2 * q + 1] = a.source_cell[2 * q + 1];
         const int64_t len = a.source_length[q];
-        sh.len[q] = len;
+        len_lds[q] = len;
         // Indexed by step parity; a sample past the signal's end is never read.
         for (int h = 0; h < 2; ++h) {
             const int64_t n = step + h;
-            sh.now[2 * q + (int)(n & 1)] = n < len ? a.source_signal[q * a.source_stride + n] : real(0);
+            now_lds[2 * q + (int)(n & 1)] = n < len ? a.source_signal[q * a.source_stride + n] : real(0);
         }
     }
-    for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) sh.rx[q] = a.receiver_cell[q];
+    for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
     lds_barrier();
     fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED>(
         a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
-        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, sh.lds, sh.src,
-        sh.len, sh.now, sh.rx, sh.stage);
+        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
+        len_lds, now_lds, rx_lds, stage);
 }
 
+// The LDS declarations of a two-step kernel (see fused2_block).
+#define ACFDTD_FUSED2_SHARED(real, V, NR, STAGED)                                          \
+    __shared__ real lds[3 * kFused2Threads * V];                                           \
+    __shared__ __align__(16) unsigned char stage[STAGED ? stage_bytes<real, V, NR>() : 16]; \
+    __shared__ int32_t src_lds[2 * kFused2Sources];                                        \
+    __shared__ int64_t len_lds[kFused2Sources];                                            \
+    __shared__ real now_lds[2 * kFused2Sources];                                           \
+    __shared__ int32_t rx_lds[2 * kFused2Receivers]
+
+// The single-room kernel keeps its own copy of the workgroup body (fused2_block, which the
+// batched kernel runs, is the same code): through the shared function the compiler spills
+// more scalar registers here (about 800 more lane reads of spilled scalars per kernel), which
+// measured 6 % slower at 256^3.
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
-__global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused2_step(
-    const SweepArgs<real> a, const Fused2IO<real> io, const Tiling2 t2, const int64_t step,
-    const int64_t column) {
-    __shared__ Fused2Shared<real, V, NR, STAGED> sh;
+__global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused2_step(const SweepArgs<real> a,
+                                                             const Fused2IO<real> io,
+                                                             const Tiling2 t2, const int64_t step,
+                                                             const int64_t column) {
+    __shared__ real lds[3 * kFused2Threads * V];
+    // STAGED: each lane's slots for the next plane's first-step inputs (fused2_march).
+    __shared__ __align__(16) unsigned char stage[STAGED ? stage_bytes<real, V, NR>() : 16];
+    // Source and receiver tables, copied once: the lanes that hold a source or a receiver scan
+    // them every plane, and with a barrier per plane the whole workgroup waits for those lanes.
+    // In LDS their reads wait for nothing else; a read from global (or generic) memory waits
+    // for every store the lane issued before it.
+    __shared__ int32_t src_lds[2 * kFused2Sources];
+    __shared__ int64_t len_lds[kFused2Sources];
+    __shared__ real now_lds[2 * kFused2Sources];  // samples of steps `step` and `step` + 1
+    __shared__ int32_t rx_lds[2 * kFused2Receivers];
     const unsigned n = (unsigned)t2.launched;
     unsigned b = blockIdx.x;
     b = (b & 7u) * (n >> 3) + (b >> 3);  // contiguous tiles per XCD, as decode_tile
     if (b >= (unsigned)t2.tiles) return;  // whole workgroup, before any barrier
-    fused2_block<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED>(a, io, t2, b, step, column, sh);
+    const int blk = (int)(b % (unsigned)t2.nblk);
+    const int xc = (int)(b / (unsigned)t2.nblk);
+    const int bg = blk % t2.nblk_g, bj = blk / t2.nblk_g;
+    const int t = (int)threadIdx.x;
+    const int r = t / t2.pitch - 1;             // block row, -1 and rows: halo
+    const int sl = t % t2.pitch - t2.zhalo;     // segment position, -1 and seg: halo
+    const int jj = bj * t2.rows + r;
+    const int gg = bg * t2.seg + sl;
+    const bool valid = t < (t2.rows + 2) * t2.pitch && jj >= 0 && jj < a.ny && gg >= 0 &&
+                       gg < t2.groups;
+    const bool owner = valid && r >= 0 && r < t2.rows && sl >= 0 && sl < t2.seg;
+    const int j = valid ? jj : 0;
+    const int k0 = valid ? gg * V : 0;
+    const int i0 = xc * t2.chunk;
+    const int i1 = min(i0 + t2.chunk, a.nx);
+    // fused2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
+    for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
+        src_lds[2 * q] = a.source_cell[2 * q];
+        src_lds[2 * q + 1] = a.source_cell[2 * q + 1];
+        const int64_t len = a.source_length[q];
+        len_lds[q] = len;
+        // Indexed by step parity; a sample past the signal's end is never read.
+        for (int h = 0; h < 2; ++h) {
+            const int64_t n = step + h;
+            now_lds[2 * q + (int)(n & 1)] = n < len ? a.source_signal[q * a.source_stride + n] : real(0);
+        }
+    }
+    for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
+    lds_barrier();
+    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED>(
+        a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
+        io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
+        len_lds, now_lds, rx_lds, stage);
 }
 
 template <typename real>
