@@ -32,8 +32,10 @@ __all__ = ["CANDIDATES", "CANDIDATES_TWO_STEP", "adjoint_tiling", "forward_plan"
 #: (tile_j, chunk_x) shapes tried for one step per sweep.
 CANDIDATES = tuple((tile_j, chunk) for tile_j in (1, 2, 4, 8) for chunk in (1, 2, 3, 4, 6, 8))
 #: (tile_j, chunk_x) shapes tried for two steps per sweep (2 tile_j rows per block): taller
-#: blocks spread the halo rows thinner, longer marches the halo planes.
-CANDIDATES_TWO_STEP = tuple((tile_j, chunk) for tile_j in (2, 3, 4, 6, 8) for chunk in (8, 16, 32))
+#: blocks spread the halo rows thinner, longer marches the halo planes; short marches keep
+#: small grids' workgroups many and their per-plane latency chain short.
+CANDIDATES_TWO_STEP = tuple((tile_j, chunk) for tile_j in (2, 3, 4, 6, 8)
+                            for chunk in (1, 2, 3, 4, 6, 8, 16, 32))
 
 _cache: dict[str, tuple[int, ...]] = {}
 _loaded = False

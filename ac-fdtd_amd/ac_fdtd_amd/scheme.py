@@ -33,7 +33,7 @@ from .coefficients import build_coefficients
 from .grid import Grid
 from .medium import AIR, Medium
 
-__all__ = ["AcousticFDTD", "synchronize"]
+__all__ = ["AcousticFDTD", "CapturedSteps", "synchronize"]
 
 _TORCH_DTYPES = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
 
@@ -270,6 +270,13 @@ class AcousticFDTD:
         if recording is not None:
             self._runs.append(recording)
 
+    def capture(self, n_steps: int, record: bool = True) -> "CapturedSteps":
+        """Capture steps ``step_index`` .. ``step_index + n_steps - 1`` of :meth:`run` into one
+        HIP graph (:class:`CapturedSteps`) without running them. A replay runs those steps
+        again on whatever the fields hold, e.g. after :meth:`load_state` in an optimisation
+        loop that restarts every pass from the same state."""
+        return CapturedSteps(self, n_steps, record)
+
     def timed_steps(self, n_steps: int) -> dict[str, float]:
         """Advance ``n_steps`` like :meth:`run`, one kernel per library call, with HIP events
         around every kernel on the launch stream; returns the mean duration in ms of each
@@ -495,3 +502,49 @@ class AcousticFDTD:
         if self._psi_swap is not None:
             d.psi_swap = self._checked(self._psi_swap, (k.n_relax, *g.shape), "psi swap")
         return d
+
+
+class CapturedSteps:
+    """A step loop of :class:`AcousticFDTD` captured into one HIP graph.
+
+    The graph holds every launch :meth:`AcousticFDTD.run` makes for ``n_steps`` steps from the
+    solver's ``step_index`` at capture time, with the field, swap, relaxation, source and
+    receiver buffers of that time: :meth:`replay` runs the same steps (the same source samples
+    and recording columns) on the fields' current contents and writes the receivers' samples
+    into :attr:`recording` (``(channel, n_steps)``). It changes neither the solver's
+    ``step_index`` nor its recordings, and must not outlive the buffers it captured (a later
+    :meth:`AcousticFDTD.add_volume_source` or receiver replaces the source and receiver
+    tables). A boundary tape is not captured. The graph is submitted as one unit; the kernels
+    are those of :meth:`AcousticFDTD.run`, so a replay computes the same bits.
+    """
+
+    def __init__(self, solver: AcousticFDTD, n_steps: int, record: bool = True) -> None:
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            raise ValueError("capture needs at least one step")
+        self.solver = solver
+        self.n_steps = n_steps
+        self.step0 = solver.step_index
+        self.recording = None
+        if solver._receivers and record:
+            self.recording = torch.zeros((len(solver._receivers), n_steps), **solver._opts)
+        # Device tables (sources, receivers) are uploaded here, before the capture: a capture
+        # records no host-to-device copies.
+        solver._source_arrays()
+        self._desc = solver._descriptor(self.recording, tape=False)
+        self.graph = torch.cuda.CUDAGraph()
+        lib = solver._lib
+        with torch.cuda.device(solver.device):
+            capture_stream = torch.cuda.Stream(solver.device)
+            capture_stream.wait_stream(torch.cuda.current_stream(solver.device))
+            with torch.cuda.graph(self.graph, stream=capture_stream):
+                handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
+                status = lib.acfdtd_hip_run(ctypes.byref(self._desc), self.step0, n_steps, 0,
+                                            handle)
+            torch.cuda.current_stream(solver.device).wait_stream(capture_stream)
+        _native.check(status, "acfdtd_hip_run (captured)")
+
+    def replay(self) -> None:
+        """Run the captured steps on the current stream (asynchronous)."""
+        with torch.cuda.device(self.solver.device):
+            self.graph.replay()

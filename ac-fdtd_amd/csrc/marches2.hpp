@@ -288,14 +288,68 @@ inline bool fused2_applies(const acfdtd_hip_desc &d) {
            d.n_sources <= kFused2Sources && d.n_receivers <= kFused2Receivers;
 }
 
-// Shape of the two-step sweep: R = 2 tile_j own rows (default 6), whole rows when R + 2 rows of
-// them fit in kFused2Threads lanes, else row segments as long as fit; chunk_x planes per march
-// (default 8). False when nz is not a multiple of the lane width or no segment fits.
+// Compute units of the current device, read once per device (256 when there is none).
+inline int device_compute_units() {
+    static int known[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    if (known[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0) {
+            (void)hipGetLastError();
+            n = 256;
+        }
+        known[dev] = n;
+    }
+    return known[dev];
+}
+
+// Default planes per march of a launch of `nblk` (y, z) blocks of `waves` wavefronts each, whose
+// marches also step through `halo` planes that they do not own. A compute unit works through
+// ceil(workgroups / units) marches of (planes + halo) steps; up to 8 wavefronts per unit overlap
+// their memory latency, so a unit with fewer costs the same as one with 8 and beyond that the
+// cost grows with the wavefronts. The candidate with the lowest cost wins, on ties the longer
+// march (fewer halo planes read). Short marches win on small grids, where too few workgroups
+// would leave units idle and each march's per-plane latency sets the time; long ones on large
+// grids, where every unit has several workgroups and the halo planes are extra traffic.
+inline int default_march_planes(int nx, int64_t nblk, int waves, int halo) {
+    static const int kPlanes[] = {1, 2, 3, 4, 6, 8, 16, 32};
+    const int64_t units = device_compute_units();
+    double best = 0;
+    int pick = 1;
+    for (int c : kPlanes) {
+        const int64_t per_unit = ceil_div(nblk * ceil_div(nx, c), units);
+        const double load = (double)per_unit * waves / 8.0;
+        const double cost = (double)(c + halo) * (load > 1.0 ? load : 1.0);
+        if (c == 1 || cost <= best) {
+            best = cost;
+            pick = c;
+        }
+        if (c >= nx) break;
+    }
+    return pick;
+}
+
+// Shape of the two-step sweep: R = 2 tile_j own rows, whole rows when R + 2 rows of them fit in
+// kFused2Threads lanes, else row segments as long as fit; chunk_x planes per march. Defaults:
+// 6 rows, or 4 when only 4 whole rows fit, or 8 in row segments; default_march_planes planes.
+// False when nz is not a multiple of the lane width or no segment fits.
 inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
     if (d.nz % vec != 0) return false;
     t.vec = vec;
     t.groups = d.nz / vec;
-    t.rows = d.tile_j > 0 ? 2 * d.tile_j : 6;
+    if (d.tile_j > 0)
+        t.rows = 2 * d.tile_j;
+    else if (8 * t.groups <= kFused2Threads)
+        t.rows = 6;  // whole rows, 6 of them
+    else if (6 * t.groups <= kFused2Threads)
+        t.rows = 4;  // whole rows, 4 of them: without z halo lanes
+    else
+        t.rows = 8;  // row segments: taller blocks spread the y halo rows thinner
     if ((t.rows + 2) * t.groups <= kFused2Threads) {
         t.seg = t.groups;
         t.zhalo = 0;
@@ -307,10 +361,10 @@ inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
         if (t.seg < 1) return false;
     }
     t.threads = ceil_div((int64_t)(t.rows + 2) * t.pitch, 64) * 64;
-    t.chunk = d.chunk_x > 0 ? d.chunk_x : 8;
     t.nblk_g = ceil_div(t.groups, t.seg);
     t.nblk_j = ceil_div(d.ny, t.rows);
     const int64_t nblk = (int64_t)t.nblk_g * t.nblk_j;
+    t.chunk = d.chunk_x > 0 ? d.chunk_x : default_march_planes(d.nx, nblk, t.threads / 64, 2);
     t.nchunk = ceil_div(d.nx, t.chunk);
     const int64_t tiles = nblk * t.nchunk;
     if (tiles > (int64_t)INT32_MAX - 8) return false;

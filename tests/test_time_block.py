@@ -36,6 +36,8 @@ CASES = [
     ((10, 9, 10), 2, 3),   # fp32 lanes of 2 cells
     ((12, 11, 15), 1, 4),  # lanes of 1 cell
     ((9, 13, 87), 2, 4),   # 1-cell lanes, rows of 87 lanes in segments
+    ((7, 10, 160), 0, 0),  # default shape: 6 whole rows (fp32), 4 whole rows (fp64)
+    ((6, 19, 87), 0, 0),   # default shape: 8 rows in segments
 ]
 
 
