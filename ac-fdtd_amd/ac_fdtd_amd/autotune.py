@@ -35,7 +35,7 @@ CANDIDATES = tuple((tile_j, chunk) for tile_j in (1, 2, 4, 8) for chunk in (1, 2
 #: blocks spread the halo rows thinner, longer marches the halo planes; short marches keep
 #: small grids' workgroups many and their per-plane latency chain short.
 CANDIDATES_TWO_STEP = tuple((tile_j, chunk) for tile_j in (2, 3, 4, 6, 8)
-                            for chunk in (1, 2, 3, 4, 6, 8, 16, 32))
+                            for chunk in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32))
 
 _cache: dict[str, tuple[int, ...]] = {}
 _loaded = False

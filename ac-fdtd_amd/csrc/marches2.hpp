@@ -317,7 +317,7 @@ inline int device_compute_units() {
 // would leave units idle and each march's per-plane latency sets the time; long ones on large
 // grids, where every unit has several workgroups and the halo planes are extra traffic.
 inline int default_march_planes(int nx, int64_t nblk, int waves, int halo) {
-    static const int kPlanes[] = {1, 2, 3, 4, 6, 8, 16, 32};
+    static const int kPlanes[] = {1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 28, 32};
     const int64_t units = device_compute_units();
     double best = 0;
     int pick = 1;
@@ -355,10 +355,13 @@ inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
         t.zhalo = 0;
         t.pitch = t.groups;
     } else {
-        t.pitch = kFused2Threads / (t.rows + 2);
-        t.seg = t.pitch - 2;
+        // As few segments as fit, of equal length (a short last segment would hold a
+        // workgroup's slot for a march of mostly idle lanes).
+        const int widest = kFused2Threads / (t.rows + 2) - 2;
+        if (widest < 1) return false;
+        t.seg = ceil_div(t.groups, ceil_div(t.groups, widest));
+        t.pitch = t.seg + 2;
         t.zhalo = 1;
-        if (t.seg < 1) return false;
     }
     t.threads = ceil_div((int64_t)(t.rows + 2) * t.pitch, 64) * 64;
     t.nblk_g = ceil_div(t.groups, t.seg);
