@@ -175,9 +175,15 @@ class RoomResponse:
             sponge_strength=sigma,
             relaxation_strengths=delta,
             wall_table_always=wall_table_always,
-            autotune=self.autotune,
+            autotune=self.autotune and not wall_table_always,
             time_block=self.time_block,
         )
+        if self.autotune and wall_table_always:
+            # The wall gradient's forward pass writes the boundary tape: tune with one.
+            from .autotune import forward_plan
+
+            solver.time_block, solver.tile_j, solver.chunk_x, solver.staged = forward_plan(
+                solver, tape=True)
         signals = self.signals if signal is None else list(
             torch.as_tensor(signal).detach().cpu().double().numpy()
         )

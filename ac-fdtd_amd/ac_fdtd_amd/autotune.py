@@ -72,14 +72,15 @@ def _store(key: str, value: tuple[int, ...]) -> None:
         path.write_text(json.dumps(stored, indent=1, sort_keys=True))
 
 
-def _key(kind: str, solver) -> str:
+def _key(kind: str, solver, tape: bool = False) -> str:
     g, k = solver.grid, solver.coefficients
     props = torch.cuda.get_device_properties(solver.device)
     # The architecture and CU count identify the device type; the marketing name is not
     # reliable (it comes from a table some installations lack).
     name = f"{getattr(props, 'gcnArchName', 'gpu').split(':')[0]}x{props.multi_processor_count}"
     return (f"{kind}:{name}:{g.nx}x{g.ny}x{g.nz}:f{solver.np_dtype.itemsize * 8}:r{k.n_relax}"
-            f":w{int(solver._wall_mask != 0)}:l{int(k.has_layer)}:c{int(solver.compensated)}")
+            f":w{int(solver._wall_mask != 0)}:l{int(k.has_layer)}:c{int(solver.compensated)}"
+            + (":t1" if tape else ""))
 
 
 def _steps_per_sample(solver) -> int:
@@ -93,9 +94,10 @@ def _steps_per_sample(solver) -> int:
 
 class _Scratch:
     """Scratch forward fields (both sets) with a copy of the solver's descriptor on them; no
-    sources, receivers or tape."""
+    sources or receivers, and a scratch boundary tape of ``tape_rows`` rows when that is
+    nonzero and the solver has walls."""
 
-    def __init__(self, solver) -> None:
+    def __init__(self, solver, tape_rows: int = 0) -> None:
         g, k = solver.grid, solver.coefficients
         opts = {"device": solver.device, "dtype": solver.dtype}
         gen = torch.Generator(device=solver.device).manual_seed(0)
@@ -115,6 +117,12 @@ class _Scratch:
         d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.sets[1])
         d.psi = self.psi.data_ptr() if k.n_relax else None
         d.psi_swap = self.psi_swap.data_ptr() if k.n_relax else None
+        self.tape = None
+        if tape_rows > 0 and solver._wall_table is not None:
+            self.tape = torch.zeros((tape_rows, solver.wall_tape_stride()), **opts)
+            d.wall_tape = self.tape.data_ptr()
+            d.wall_tape_stride, d.wall_tape_step0 = self.tape.shape[1], 0
+            d.wall_tape_rows = tape_rows
         self.desc = d
 
 
@@ -149,8 +157,8 @@ def _warm_up(solver, launch, shape, steps: int, seconds: float = 0.05) -> None:
     torch.cuda.synchronize(solver.device)
 
 
-def _pick(solver, kind, launch, candidates):
-    key = _key(kind, solver)
+def _pick(solver, kind, launch, candidates, tape: bool = False):
+    key = _key(kind, solver, tape)
     found = _lookup(key)
     if found is not None:
         return found
@@ -163,14 +171,15 @@ def _pick(solver, kind, launch, candidates):
     return best
 
 
-def forward_tiling(solver, time_block: int = 1, candidates=None) -> tuple[int, int]:
+def forward_tiling(solver, time_block: int = 1, candidates=None,
+                   tape: bool = False) -> tuple[int, int]:
     """Fastest (tile_j, chunk_x) of the fused forward sweep advancing ``time_block`` steps, for
-    ``solver``'s configuration."""
+    ``solver``'s configuration (writing a boundary tape when ``tape``)."""
     if not solver.fused:
         return solver.tile_j, solver.chunk_x
     if candidates is None:
         candidates = CANDIDATES_TWO_STEP if time_block == 2 else CANDIDATES
-    scratch = _Scratch(solver)
+    scratch = _Scratch(solver, _steps_per_sample(solver) if tape else 0)
     scratch.desc.time_block = time_block
     lib = solver._lib
     handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
@@ -181,26 +190,28 @@ def forward_tiling(solver, time_block: int = 1, candidates=None) -> tuple[int, i
                       "acfdtd_hip_run")
 
     kind = "forward2" if time_block == 2 else "forward"
-    return _pick(solver, kind, launch, candidates)
+    return _pick(solver, kind, launch, candidates, tape)
 
 
-def forward_plan(solver) -> tuple[int, int, int, bool]:
+def forward_plan(solver, tape: bool = False) -> tuple[int, int, int, bool]:
     """(time_block, tile_j, chunk_x, staged) for ``solver``: the fastest one-step tiling and,
     when the solver may advance two steps per sweep, the fastest two-step tiling, plain and
     staged (each plane's inputs requested a plane ahead into LDS), whichever runs more steps
-    per second."""
-    single = forward_tiling(solver, 1)
+    per second. With ``tape`` every candidate writes a boundary tape, as a forward pass of the
+    wall gradient does (the tape's stores change which plan is fastest)."""
+    single = forward_tiling(solver, 1, tape=tape)
     if not solver.two_step_applies():
         return 1, single[0], single[1], False
-    double = forward_tiling(solver, 2)
-    key = _key("plan", solver)
+    double = forward_tiling(solver, 2, tape=tape)
+    key = _key("plan", solver, tape)
     found = _lookup(key)
     if found is not None:
         return (*found, False)[:4] if len(found) == 3 else tuple(found)
     handle = ctypes.c_void_p(torch.cuda.current_stream(solver.device).cuda_stream)
     launches = {}
+    steps = 4 * _steps_per_sample(solver)
     for plan, shape in (((1, False), single), ((2, False), double), ((2, True), double)):
-        scratch = _Scratch(solver)
+        scratch = _Scratch(solver, steps if tape else 0)
         scratch.desc.time_block = plan[0]
         scratch.desc.tile_j, scratch.desc.chunk_x = shape
         scratch.desc.flags = (scratch.desc.flags & ~_native.STAGED) | (_native.STAGED if plan[1] else 0)
@@ -210,7 +221,6 @@ def forward_plan(solver) -> tuple[int, int, int, bool]:
                           "acfdtd_hip_run")
 
         launches[plan] = (launch, shape)
-    steps = 4 * _steps_per_sample(solver)
     per_step = {plan: float("inf") for plan in launches}
     with torch.cuda.device(solver.device):
         _warm_up(solver, *launches[(1, False)], steps)
