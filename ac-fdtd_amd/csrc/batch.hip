@@ -2,7 +2,8 @@
 //
 // A table of tiles (room, plane block, x-chunk) covers every room; each workgroup looks its
 // tile up and runs the fused step's march on that room's arguments, which sit in device memory
-// (one BatchRoom per room, read through wave-uniform scalar loads). The lanes of a room flatten
+// (one BatchRoom per room, read through wave-uniform scalar loads; its field pointers are
+// declared global, see global_ptr). The lanes of a room flatten
 // its (y, z / V) plane as in the single-room fused kernel, with one V for the whole batch (the
 // widest every room's nz allows). The XCD remap applies to the global tile index, so each XCD
 // owns a contiguous run of tiles of one or a few rooms. Steps go in fused pairs (A -> B -> A);
@@ -35,22 +36,32 @@ struct BatchTile {
 };
 
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
-__global__ __launch_bounds__(512) void batch_fused_step(const BatchRoom<real> *rooms,
-                                                        const BatchTile *tiles, const int n_tiles,
-                                                        const int launched, const int reverse,
-                                                        const int64_t step, const int64_t column) {
+__global__ __launch_bounds__(512) void batch_fused_step(const BatchRoom<real> *__restrict__ rooms,
+                                                        const BatchTile *__restrict__ tiles,
+                                                        const int n_tiles, const int launched,
+                                                        const int reverse, const int64_t step,
+                                                        const int64_t column) {
     const unsigned n = (unsigned)launched;
     unsigned b = blockIdx.x;
     b = (b & 7u) * (n >> 3) + (b >> 3);
     if (b >= (unsigned)n_tiles) return;
     const BatchTile t = tiles[b];
     const BatchRoom<real> &room = rooms[t.room];
-    const SweepArgs<real> &a = room.args;
+    SweepArgs<real> a = room.args;
+    global_ptrs(a);
     int j, k0;
     if (!flat_lane(a.tiles, a.ny, a.nz, t.jb, j, k0)) return;
     const int i0 = t.xc * a.tiles.chunk;
     const int i1 = min(i0 + a.tiles.chunk, a.nx);
-    const FusedIO<real> &io = reverse ? room.backward : room.forward;
+    FusedIO<real> io = reverse ? room.backward : room.forward;
+    io.p_in = global_ptr(io.p_in);
+    io.vx_in = global_ptr(io.vx_in);
+    io.vy_in = global_ptr(io.vy_in);
+    io.vz_in = global_ptr(io.vz_in);
+    io.p_out = global_ptr(io.p_out);
+    io.vx_out = global_ptr(io.vx_out);
+    io.vy_out = global_ptr(io.vy_out);
+    io.vz_out = global_ptr(io.vz_out);
     fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(
         a, i0, i1, j, k0, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
         io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
@@ -151,12 +162,14 @@ std::string pack_rooms(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
 }
 
 // A room's descriptor as the batched two-step sweep tiles it: the room's own tile_j and chunk_x,
-// or, where they are 0, 4 and 16 (8 own rows, 16-plane marches: measured best for the 64-room
-// workload, where the single-room defaults of 6 rows and 8 planes run 11 % slower).
+// or, where they are 0, 4 and 24 (8 own rows, 24-plane marches: measured best for the 64-room
+// workload, 110.5 Gcell/s against 99.6 to 109.9 for 4 to 12 rows and 8 to 24 planes). The
+// single-room default march length does not apply: it sizes the marches for the workgroups of
+// one room, and a batch launch holds every room's.
 inline acfdtd_hip_desc batch2_desc(const acfdtd_hip_desc &d) {
     acfdtd_hip_desc t = d;
     if (t.tile_j <= 0) t.tile_j = 4;
-    if (t.chunk_x <= 0) t.chunk_x = 16;
+    if (t.chunk_x <= 0) t.chunk_x = 24;
     return t;
 }
 
