@@ -186,11 +186,15 @@ __device__ __forceinline__ void adjoint_march_vec(
         for (int e = 0; e < V; ++e) tv[e] = 0;
         if (want_walls && (XWALL || yz_boundary)) {
             {
-                const real *src = nullptr;
-                if (XWALL) src = tape_row + (i == 0 ? planes.x_lo : planes.x_hi) + (int64_t)j * nz + k0;
+                // Offsets, not a pointer that may be null: such a pointer is no longer known to
+                // point to global memory, and the loads through it become flat ones (which the
+                // LDS waits also wait for).
+                int64_t row = -1;
+                if (XWALL) row = (i == 0 ? planes.x_lo : planes.x_hi) + (int64_t)j * nz + k0;
                 else if (!has_ylo || !has_yhi)
-                    src = tape_row + (!has_ylo ? planes.y_lo : planes.y_hi) + (int64_t)i * nz + k0;
-                if (src != nullptr) {
+                    row = (!has_ylo ? planes.y_lo : planes.y_hi) + (int64_t)i * nz + k0;
+                if (row >= 0) {
+                    const real *src = tape_row + row;
                     if (a.wall_tape_rows16) {
                         vload<real, V>(src, tv);
                     } else {
