@@ -563,11 +563,12 @@ def main() -> int:
                         help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
         "--mode",
-        choices=("bench", "copy", "gradient"),
+        choices=("bench", "copy", "gradient", "batch"),
         default="bench",
         help="copy: five plain 512 MiB device-to-device copies, the known-size reference a "
         "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by; gradient: only the "
-        "wall-gradient cost at 128^3 (for profiling the adjoint kernels)",
+        "wall-gradient cost at 128^3 (for profiling the adjoint kernels); batch: only the "
+        "batched rooms (for profiling the batch kernels)",
     )
     args = parser.parse_args()
 
@@ -597,6 +598,10 @@ def main() -> int:
         return 0
     if args.mode == "gradient":
         print(json.dumps(gradient_cost(128, 2000)), flush=True)
+        return 0
+    if args.mode == "batch":
+        print(json.dumps(batched_rooms(args.batch_rooms, args.batch_steps, args.warmup,
+                                       check=not args.skip_check)), flush=True)
         return 0
 
     log(f"fp32 {args.size}^3, {args.steps} steps")
