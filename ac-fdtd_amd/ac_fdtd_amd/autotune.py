@@ -84,11 +84,12 @@ def _key(kind: str, solver, tape: bool = False) -> str:
 
 
 def _steps_per_sample(solver) -> int:
-    """Enough steps for about 2 ms per timing, at least 4, at most 60, a multiple of 4 (so a
-    two-step timing runs two-step sweeps only)."""
+    """Enough steps for about 5 ms per timing, at least 8 (four two-step launches: fewer made
+    the timings of large grids too noisy to rank close candidates), at most 60, a multiple of 4
+    (so a two-step timing runs two-step sweeps only)."""
     bytes_per_cell = solver.np_dtype.itemsize * (8 + 2 * solver.coefficients.n_relax)
     seconds = solver.grid.n_cells * bytes_per_cell / 4e12
-    steps = int(min(60, max(4, 2e-3 / max(seconds, 1e-9))))
+    steps = int(min(60, max(8, 5e-3 / max(seconds, 1e-9))))
     return steps + (-steps) % 4
 
 
