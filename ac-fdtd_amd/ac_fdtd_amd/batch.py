@@ -2,8 +2,9 @@
 
 Each room is an ordinary :class:`~ac_fdtd_amd.scheme.AcousticFDTD` (its own grid, walls, air,
 sources, receivers and state tensors); the batch advances them together through the library's
-batch API (``acfdtd_hip_batch_*`` in ``include/acfdtd_hip.h``), one fused launch per step over
-a tile table that covers every room. Results are bit-identical to running each room alone.
+batch API (``acfdtd_hip_batch_*`` in ``include/acfdtd_hip.h``), one launch per two steps (per
+lane-width group) over a tile table that covers every room. Results are bit-identical to
+running each room alone. :meth:`BatchedRooms.capture` records a step loop into a HIP graph.
 There is no reference counterpart: the reference runs one room at a time.
 """
 
@@ -16,7 +17,7 @@ import torch
 from . import _native
 from .scheme import AcousticFDTD
 
-__all__ = ["BatchedRooms", "lane_width"]
+__all__ = ["BatchedRooms", "CapturedBatch", "lane_width"]
 
 
 def lane_width(solver: AcousticFDTD) -> int:
@@ -62,19 +63,29 @@ class _Group:
             array[n] = solver._descriptor(None if recordings is None else recordings[n], tape=False)
         return array
 
-    def run(self, step_index: int, n_steps: int) -> list:
-        recordings = [
-            torch.zeros((len(s._receivers), n_steps), **s._opts) if s._receivers else None
-            for s in self.solvers
-        ]
+    def upload(self, recordings) -> None:
+        """Point the device room table at the rooms' current buffers and ``recordings``
+        (synchronises the stream)."""
         descs = self._descriptors(recordings)
         stream = ctypes.c_void_p(_stream(self.device))
         with torch.cuda.device(self.device):
             _native.check(self._lib.acfdtd_hip_batch_update(self._handle, descs,
                                                             len(self.solvers), stream),
                           "acfdtd_hip_batch_update")
+
+    def launch(self, step_index: int, n_steps: int) -> None:
+        stream = ctypes.c_void_p(_stream(self.device))
+        with torch.cuda.device(self.device):
             _native.check(self._lib.acfdtd_hip_batch_run(self._handle, step_index, n_steps, 0,
                                                          stream), "acfdtd_hip_batch_run")
+
+    def run(self, step_index: int, n_steps: int) -> list:
+        recordings = [
+            torch.zeros((len(s._receivers), n_steps), **s._opts) if s._receivers else None
+            for s in self.solvers
+        ]
+        self.upload(recordings)
+        self.launch(step_index, n_steps)
         return recordings
 
     def close(self) -> None:
@@ -126,6 +137,7 @@ class BatchedRooms:
             if solver.step_index != first.step_index:
                 raise ValueError("rooms of a batch must be at the same step")
         self.solvers = list(solvers)
+        self._tables = None  # the CapturedBatch whose buffers the room tables point at, if any
         self._lib = _native.library()
         _bind(self._lib)
         by_width: dict[int, list[AcousticFDTD]] = {}
@@ -160,12 +172,18 @@ class BatchedRooms:
         if not self._groups:
             raise RuntimeError("the batch has been closed")
         step = self.step_index
+        self._tables = None
         for group in self._groups:
             recordings = group.run(step, n_steps)
             for solver, recording in zip(group.solvers, recordings):
                 solver.step_index += n_steps
                 if recording is not None:
                     solver._runs.append(recording)
+
+    def capture(self, n_steps: int) -> "CapturedBatch":
+        """Capture steps ``step_index`` .. ``step_index + n_steps - 1`` of :meth:`run` into one
+        HIP graph (:class:`CapturedBatch`) without running them."""
+        return CapturedBatch(self, n_steps)
 
     def close(self) -> None:
         for group in getattr(self, "_groups", []):
@@ -174,3 +192,59 @@ class BatchedRooms:
 
     def __del__(self):
         self.close()
+
+
+class CapturedBatch:
+    """A step loop of :class:`BatchedRooms` captured into one HIP graph.
+
+    The graph holds every launch of ``n_steps`` steps of all groups from the batch's step at
+    capture time. The launches read each group's device room table (the rooms' buffers and
+    where their receivers record); the table is pointed at this capture's recordings before
+    the capture and again before a replay that follows an eager :meth:`BatchedRooms.run`
+    (which re-points it, and synchronises the stream to do so). :meth:`replay` runs the same
+    steps on the rooms' current fields and writes :attr:`recordings` (one
+    ``(channel, n_steps)`` tensor or None per room, in the batch's order); it changes neither
+    the rooms' ``step_index`` nor their recordings.
+    """
+
+    def __init__(self, batch: BatchedRooms, n_steps: int) -> None:
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            raise ValueError("capture needs at least one step")
+        if not batch._groups:
+            raise RuntimeError("the batch has been closed")
+        self.batch = batch
+        self.n_steps = n_steps
+        self.step0 = batch.step_index
+        self.recordings = [
+            torch.zeros((len(s._receivers), n_steps), **s._opts) if s._receivers else None
+            for s in batch.solvers
+        ]
+        index = {id(s): n for n, s in enumerate(batch.solvers)}
+        self._group_recordings = [[self.recordings[index[id(s)]] for s in group.solvers]
+                                  for group in batch._groups]
+        self._point_tables()
+        device = batch.solvers[0].device
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.device(device):
+            capture_stream = torch.cuda.Stream(device)
+            capture_stream.wait_stream(torch.cuda.current_stream(device))
+            with torch.cuda.graph(self.graph, stream=capture_stream):
+                for group in batch._groups:
+                    group.launch(self.step0, n_steps)
+            torch.cuda.current_stream(device).wait_stream(capture_stream)
+
+    def _point_tables(self) -> None:
+        for group, recordings in zip(self.batch._groups, self._group_recordings):
+            group.upload(recordings)
+        self.batch._tables = self
+
+    def replay(self) -> None:
+        """Run the captured steps on the current stream (asynchronous unless the room tables
+        had to be re-pointed first)."""
+        if not self.batch._groups:
+            raise RuntimeError("the batch has been closed")
+        if self.batch._tables is not self:
+            self._point_tables()
+        with torch.cuda.device(self.batch.solvers[0].device):
+            self.graph.replay()

@@ -56,3 +56,37 @@ def test_captured_steps_replay_the_eager_run(dtype, time_block, n_steps):
             assert torch.equal(got, ref), f"replay {replay}"
         assert torch.equal(captured.recording, want_recording), f"replay {replay}"
     assert graph.step_index == 5
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_captured_batch_replays_the_eager_batch(dtype):
+    """A batch step loop in a graph: replays from restored states give the eager batch's
+    fields and samples, also after an eager run has re-pointed the room tables."""
+    from ac_fdtd_amd.batch import BatchedRooms
+
+    shapes = [(14, 11, 16), (9, 13, 10), (12, 7, 9)]  # lane widths 4, 2, 1 (fp32)
+    eager = [_solver(shape, dtype, 2) for shape in shapes]
+    graph = [_solver(shape, dtype, 2) for shape in shapes]
+    eager_batch, graph_batch = BatchedRooms(eager), BatchedRooms(graph)
+    eager_batch.run(3)
+    graph_batch.run(3)
+    starts = [s.save_state() for s in graph]
+    captured = graph_batch.capture(14)
+    eager_batch.run(14)
+    want = [s.save_state() for s in eager]
+    want_recordings = [s.recording_tensor()[:, 3:].clone() for s in eager]
+    for replay in range(3):
+        for solver, start in zip(graph, starts):
+            solver.load_state(start, 3)
+        if replay == 1:
+            graph_batch.run(2)  # an eager run re-points the room tables
+            for solver, start in zip(graph, starts):
+                solver.load_state(start, 3)
+        captured.replay()
+        torch.cuda.synchronize()
+        for solver, ref, ref_rec, rec in zip(graph, want, want_recordings, captured.recordings):
+            for got, exp in zip(solver.save_state(), ref):
+                assert torch.equal(got, exp), f"replay {replay}"
+            assert torch.equal(rec, ref_rec), f"replay {replay}"
+    eager_batch.close()
+    graph_batch.close()
