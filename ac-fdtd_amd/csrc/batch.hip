@@ -162,13 +162,13 @@ std::string pack_rooms(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
 }
 
 // A room's descriptor as the batched two-step sweep tiles it: the room's own tile_j and chunk_x,
-// or, where they are 0, 4 and 24 (8 own rows, 24-plane marches: measured best for the 64-room
-// workload, 110.5 Gcell/s against 99.6 to 109.9 for 4 to 12 rows and 8 to 24 planes). The
+// or, where they are 0, 6 and 24 (12 own rows, 24-plane marches: measured best for the 64-room
+// workload, 114.6 Gcell/s against 109.9 to 114.4 for 6 to 16 rows and 16 to 32 planes). The
 // single-room default march length does not apply: it sizes the marches for the workgroups of
 // one room, and a batch launch holds every room's.
 inline acfdtd_hip_desc batch2_desc(const acfdtd_hip_desc &d) {
     acfdtd_hip_desc t = d;
-    if (t.tile_j <= 0) t.tile_j = 4;
+    if (t.tile_j <= 0) t.tile_j = 6;
     if (t.chunk_x <= 0) t.chunk_x = 24;
     return t;
 }
