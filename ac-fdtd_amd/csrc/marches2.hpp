@@ -655,13 +655,14 @@ __device__ __forceinline__ void record_cells(const SweepArgs<real> &a, const Lan
 // The march of one lane over planes [i0, i1). `valid`: the lane maps to cells of the grid;
 // `owner`: it is one of the workgroup's own lanes (not halo), which alone store and record.
 // Every thread of the workgroup runs the same loop and reaches every barrier.
-// The plain (not staged) two-step sweep with 4-cell fp32 lanes loads plane i + 2's first-step
-// inputs into registers right after step 1 of plane i + 1, so that they travel during the
-// barrier and step 2 of plane i (measured 2 to 15 % faster on 4-cell lanes from 146 x 117 x 88
-// to 384^3, one case 5 % slower). It takes up to 43 more VGPRs per lane, which fit under the
-// 256 of two wavefronts per SIMD with up to three relaxation states; with four, or in fp64,
-// the kernel would spill. On 2- and 1-cell lanes it measured slower (up to 32 %), so there,
-// as in those cases, each plane's inputs are loaded when they are needed.
+// The plain (not staged) two-step sweep with 16-byte lanes (4 fp32 or 2 fp64 cells) loads
+// plane i + 2's first-step inputs into registers right after step 1 of plane i + 1, so that
+// they travel during the barrier and step 2 of plane i (measured on fp32 2 to 15 % faster from
+// 146 x 117 x 88 to 384^3, one case 5 % slower; on fp64 without a sponge 10 to 19 % faster).
+// It takes up to 43 more VGPRs per lane, which fit under the 256 of two wavefronts per SIMD
+// with up to three relaxation states, and in fp64 only without the sponge's factors; beyond
+// that the kernel would spill. On narrower lanes it measured slower (up to 32 %). Where it does
+// not apply, each plane's inputs are loaded when they are needed.
 #ifndef ACFDTD_FUSED2_PREFETCH
 #define ACFDTD_FUSED2_PREFETCH 1
 #endif
@@ -923,7 +924,8 @@ __device__ __forceinline__ void fused2_march(
     // Prologue: step 1 of the halo plane i0 - 1 (its p^(n+1) only) and of plane i0.
     PlaneState<real, V, NR> A, B;
     constexpr bool PREFETCH =
-        ACFDTD_FUSED2_PREFETCH && !STAGED && NR <= 3 && sizeof(real) == 4 && V == 4;
+        ACFDTD_FUSED2_PREFETCH && !STAGED && NR <= 3 &&
+        (sizeof(real) == 4 ? V == 4 : (V == 2 && !LAYER));
     Step1In<real, V, NR> ahead;  // PREFETCH: the next plane's step-1 inputs, in registers
     real p_prev[V];
     real v_half_lo[V];  // v^(n+1/2) on face i0
