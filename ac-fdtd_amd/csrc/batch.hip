@@ -161,15 +161,17 @@ std::string pack_rooms(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
     return blob;
 }
 
-// A room's descriptor as the batched two-step sweep tiles it: the room's own tile_j and chunk_x,
-// or, where they are 0, 6 and 24 (12 own rows, 24-plane marches: measured best for the 64-room
-// workload, 114.6 Gcell/s against 109.9 to 114.4 for 6 to 16 rows and 16 to 32 planes). The
-// single-room default march length does not apply: it sizes the marches for the workgroups of
-// one room, and a batch launch holds every room's.
-inline acfdtd_hip_desc batch2_desc(const acfdtd_hip_desc &d) {
+// A room's descriptor as the batched two-step sweep tiles it with lanes of `vec` cells: the
+// room's own tile_j and chunk_x, or, where they are 0, per lane width the shape measured best on
+// the 64-room workload's rooms of that width (among 4 to 16 rows and 16 to 32 planes): 4-cell
+// lanes 12 rows and 24-plane marches (157.7 Gcell/s, next 146.5), 2-cell lanes 8 rows and 32
+// planes (125.5, next 118.7), 1-cell lanes 8 rows and 32 planes (108.0, within 2 % of the
+// others). The single-room default march length does not apply: it sizes the marches for the
+// workgroups of one room, and a batch launch holds every room's.
+inline acfdtd_hip_desc batch2_desc(const acfdtd_hip_desc &d, int vec) {
     acfdtd_hip_desc t = d;
-    if (t.tile_j <= 0) t.tile_j = 6;
-    if (t.chunk_x <= 0) t.chunk_x = 24;
+    if (t.tile_j <= 0) t.tile_j = vec == 4 ? 6 : 4;
+    if (t.chunk_x <= 0) t.chunk_x = vec == 4 ? 24 : 32;
     return t;
 }
 
@@ -181,7 +183,7 @@ inline bool batch_two_step(const acfdtd_hip_desc *rooms, int n_rooms, int vec, i
     int widest = 64;
     for (int r = 0; r < n_rooms; ++r) {
         Tiling2 t2;
-        if (!fused2_applies(rooms[r]) || !make_tiling2(batch2_desc(rooms[r]), vec, t2)) return false;
+        if (!fused2_applies(rooms[r]) || !make_tiling2(batch2_desc(rooms[r], vec), vec, t2)) return false;
         total += t2.tiles;
         widest = std::max(widest, t2.threads);
     }
@@ -203,7 +205,7 @@ std::string pack_rooms2(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
         std::memset(&room, 0, sizeof(room));
         fill_args(rooms[r], room.args);
         fused2_io<real>(rooms[r], room.forward, room.backward);
-        make_tiling2(batch2_desc(rooms[r]), vec, room.t2);
+        make_tiling2(batch2_desc(rooms[r], vec), vec, room.t2);
         std::memcpy(&blob[sizeof(room) * (size_t)r], &room, sizeof(room));
     }
     return blob;
@@ -213,7 +215,7 @@ std::string pack_tiles2(const acfdtd_hip_desc *rooms, int n_rooms, int vec) {
     std::string blob;
     for (int r = 0; r < n_rooms; ++r) {
         Tiling2 t2;
-        make_tiling2(batch2_desc(rooms[r]), vec, t2);
+        make_tiling2(batch2_desc(rooms[r], vec), vec, t2);
         for (int b = 0; b < t2.tiles; ++b) {
             const BatchTile2 tile{r, b};
             blob.append(reinterpret_cast<const char *>(&tile), sizeof(tile));
