@@ -241,11 +241,12 @@ void fused2_io(const acfdtd_hip_desc &d, Fused2IO<real> &forward, Fused2IO<real>
 template <typename real>
 int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps,
                             int64_t column, hipStream_t stream) {
+    if (!fused2_applies(d)) return 0;
+    const Fused2Kernel<real> k = pick_fused2<real>(d);
     Tiling2 t2;
-    if (!fused2_applies(d) || !make_tiling2(d, fused_vec(d), t2)) return 0;
+    if (!make_tiling2(d, fused_vec(d), t2, reinterpret_cast<const void *>(k))) return 0;
     SweepArgs<real> a;
     fill_args(d, a);
-    const Fused2Kernel<real> k = pick_fused2<real>(d);
     Fused2IO<real> forward, backward;
     fused2_io<real>(d, forward, backward);
     const dim3 grid(t2.launched), block(t2.threads);
@@ -264,13 +265,14 @@ template <typename real>
 int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, bool reverse,
                        hipStream_t stream) {
     Tiling2 t2;
-    if (!fused2_applies(d) || !make_tiling2(d, fused_vec(d), t2))
+    const Fused2Kernel<real> k = fused2_applies(d) ? pick_fused2<real>(d) : nullptr;
+    if (k == nullptr || !make_tiling2(d, fused_vec(d), t2, reinterpret_cast<const void *>(k)))
         return fail(ACFDTD_HIP_EINVAL, "the two-step sweep does not apply to this descriptor");
     SweepArgs<real> a;
     fill_args(d, a);
     Fused2IO<real> forward, backward;
     fused2_io<real>(d, forward, backward);
-    hipLaunchKernelGGL(pick_fused2<real>(d),
+    hipLaunchKernelGGL(k,
                        dim3(t2.launched), dim3(t2.threads), 0, stream, a,
                        reverse ? backward : forward, t2, step, column);
     return check_hip(hipGetLastError(), "two-step sweep launch");

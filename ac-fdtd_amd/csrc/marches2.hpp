@@ -334,36 +334,77 @@ inline int default_march_planes(int nx, int64_t nblk, int waves, int halo) {
     return pick;
 }
 
-// Shape of the two-step sweep: R = 2 tile_j own rows, whole rows when R + 2 rows of them fit in
-// kFused2Threads lanes, else row segments as long as fit; chunk_x planes per march. Defaults:
-// 6 rows, or 4 when only 4 whole rows fit, or 8 in row segments; default_march_planes planes.
-// False when nz is not a multiple of the lane width or no segment fits.
-inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t) {
-    if (d.nz % vec != 0) return false;
-    t.vec = vec;
-    t.groups = d.nz / vec;
-    if (d.tile_j > 0)
-        t.rows = 2 * d.tile_j;
-    else if (8 * t.groups <= kFused2Threads)
-        t.rows = 6;  // whole rows, 6 of them
-    else if (6 * t.groups <= kFused2Threads)
-        t.rows = 4;  // whole rows, 4 of them: without z halo lanes
-    else
-        t.rows = 8;  // row segments: taller blocks spread the y halo rows thinner
-    if ((t.rows + 2) * t.groups <= kFused2Threads) {
-        t.seg = t.groups;
+// Lane groups of a block of `rows` own rows: whole rows when rows + 2 of them fit in
+// kFused2Threads lanes, else as few row segments as fit, of equal length (a short last segment
+// would hold a workgroup's slot for a march of mostly idle lanes). False when none fits.
+inline bool rows_shape(const int groups, const int rows, Tiling2 &t) {
+    t.rows = rows;
+    if ((rows + 2) * groups <= kFused2Threads) {
+        t.seg = groups;
         t.zhalo = 0;
-        t.pitch = t.groups;
+        t.pitch = groups;
     } else {
-        // As few segments as fit, of equal length (a short last segment would hold a
-        // workgroup's slot for a march of mostly idle lanes).
-        const int widest = kFused2Threads / (t.rows + 2) - 2;
+        const int widest = kFused2Threads / (rows + 2) - 2;
         if (widest < 1) return false;
-        t.seg = ceil_div(t.groups, ceil_div(t.groups, widest));
+        t.seg = ceil_div(groups, ceil_div(groups, widest));
         t.pitch = t.seg + 2;
         t.zhalo = 1;
     }
-    t.threads = ceil_div((int64_t)(t.rows + 2) * t.pitch, 64) * 64;
+    t.threads = ceil_div((int64_t)(rows + 2) * t.pitch, 64) * 64;
+    return true;
+}
+
+// Default own rows per block for `kernel` when whole rows fit: of 4 to 12 rows, the shape that
+// keeps the most own lanes resident on a CU, (workgroups per CU) x rows x lane groups, the
+// workgroups per CU being what the kernel's registers and LDS allow for that block size; on
+// ties the fewer rows (more workgroups). A block whose wavefronts do not divide the CU's
+// wavefront slots leaves slots idle: with two wavefronts per SIMD, 4-wavefront blocks run two
+// per CU and 5-wavefront blocks one, so 8 rows of 22 to 24 lane groups (4 wavefronts) beat 6
+// rows (3 wavefronts, two per CU), measured 10 to 18 % faster, and 6 rows of 32 (4 wavefronts)
+// beat 8 rows (5), 25 % faster. Row segments are left to the fixed rule (there the count of
+// resident lanes ranked shapes wrongly: 12 rows of segments against 8 at 256^3 fp64 measured
+// 9 % slower). 0 when no whole rows fit or the occupancy cannot be read.
+inline int default_rows(const int groups, const void *kernel) {
+    int best_rows = 0;
+    int64_t best = 0;
+    for (int rows = 4; rows <= 12; rows += 2) {
+        Tiling2 s;
+        if (!rows_shape(groups, rows, s) || s.zhalo != 0) continue;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, s.threads, 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        const int64_t resident = (int64_t)per_cu * rows * s.seg;
+        if (resident > best) {
+            best = resident;
+            best_rows = rows;
+        }
+    }
+    return best_rows;
+}
+
+// Shape of the two-step sweep: R = 2 tile_j own rows (rows_shape) and chunk_x planes per march.
+// Defaults: the rows default_rows picks for `kernel` when it is given, else 6 rows, or 4 when
+// only 4 whole rows fit, or 8 in row segments; default_march_planes planes. False when nz is
+// not a multiple of the lane width or no segment fits.
+inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t,
+                         const void *kernel = nullptr) {
+    if (d.nz % vec != 0) return false;
+    t.vec = vec;
+    t.groups = d.nz / vec;
+    int rows = d.tile_j > 0 ? 2 * d.tile_j : 0;
+    if (rows == 0 && kernel != nullptr) rows = default_rows(t.groups, kernel);
+    if (rows == 0) {
+        if (8 * t.groups <= kFused2Threads)
+            rows = 6;  // whole rows, 6 of them
+        else if (6 * t.groups <= kFused2Threads)
+            rows = 4;  // whole rows, 4 of them: without z halo lanes
+        else
+            rows = 8;  // row segments: taller blocks spread the y halo rows thinner
+    }
+    if (!rows_shape(t.groups, rows, t)) return false;
     t.nblk_g = ceil_div(t.groups, t.seg);
     t.nblk_j = ceil_div(d.ny, t.rows);
     const int64_t nblk = (int64_t)t.nblk_g * t.nblk_j;
