@@ -91,17 +91,31 @@ def _run(cmd: list[str], verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr[-4000:]}")
 
 
-def build_library(verbose: bool = False) -> Path:
-    """Compile every source (in parallel) and link :data:`LIBRARY`, replacing it atomically."""
+def _stale(source: Path, obj: Path) -> bool:
+    """Whether ``obj`` is older than its source or than any header of the library."""
+    if not obj.exists():
+        return True
+    built = obj.stat().st_mtime
+    headers = [*SOURCE_DIR.glob("*.hpp"), *INCLUDE_DIR.glob("*.h")]
+    return any(path.stat().st_mtime > built for path in (source, *headers))
+
+
+def build_library(verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile every source (in parallel) and link :data:`LIBRARY`, replacing it atomically.
+
+    ``incremental`` keeps objects in ``_build/`` beside the sources and recompiles only those
+    older than their source or a header (a development convenience; the default rebuilds
+    everything)."""
     LIBRARY.parent.mkdir(parents=True, exist_ok=True)
     with tempfile.TemporaryDirectory(prefix="acfdtd_build_") as scratch:
-        objects = [Path(scratch) / (Path(name).stem + ".o") for name in SOURCES]
-        jobs = min(len(SOURCES), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        folder = SOURCE_DIR.parent / "_build" if incremental else Path(scratch)
+        folder.mkdir(exist_ok=True)
+        objects = [folder / (Path(name).stem + ".o") for name in SOURCES]
+        todo = [(SOURCE_DIR / name, obj) for name, obj in zip(SOURCES, objects)
+                if not incremental or _stale(SOURCE_DIR / name, obj)]
+        jobs = min(max(1, len(todo)), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
         with ThreadPoolExecutor(max_workers=jobs) as pool:
-            for future in [
-                pool.submit(_run, compile_command(SOURCE_DIR / name, obj), verbose)
-                for name, obj in zip(SOURCES, objects)
-            ]:
+            for future in [pool.submit(_run, compile_command(src, obj), verbose) for src, obj in todo]:
                 future.result()
         staging = LIBRARY.with_name(LIBRARY.name + f".tmp{os.getpid()}")
         try:
@@ -114,4 +128,4 @@ def build_library(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    print(build_library(verbose="-v" in sys.argv))
+    print(build_library(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -286,6 +286,13 @@ class AcousticFDTD:
         n_steps = int(n_steps)
         if n_steps <= 0:
             return {}
+        if self._wall_tape is not None:
+            # Checked before the first launch, so a short tape leaves the state untouched.
+            rows, step0 = self._wall_tape[0].shape[0], self._wall_tape[1]
+            if self.step_index < step0 or self.step_index + n_steps - step0 > rows:
+                raise ValueError(
+                    f"the attached wall tape holds steps {step0} to {step0 + rows - 1}; "
+                    f"steps {self.step_index} to {self.step_index + n_steps - 1} do not fit")
         recording = None
         if self._receivers:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)

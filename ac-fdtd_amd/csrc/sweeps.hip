@@ -364,6 +364,10 @@ extern "C" int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step,
         return fail(ACFDTD_HIP_EINVAL, "step and recording_column must be >= 0");
     if (desc->n_receivers > 0 && recording_column + 1 >= desc->recording_stride)
         return fail(ACFDTD_HIP_EINVAL, "recording columns out of range");
+    // The sweep writes tape rows for steps `step` and `step + 1`.
+    if (desc->wall_tape && (step < desc->wall_tape_step0 ||
+                            step + 1 - desc->wall_tape_step0 >= desc->wall_tape_rows))
+        return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these two steps");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8)

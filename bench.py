@@ -9,18 +9,22 @@ advancing ``--steps`` steps, bracketed by HIP events and synchronisations.
 
 Printed: one JSON line. ``value`` is fp32 Gcell-updates/s at ``--size``^3 (two steps per
 sweep where the autotuner finds that faster, tile shapes tuned). ``roofline`` is for the
-dominant kernel: ``achieved`` counts the algorithmic bytes of the reference's two-sweep step,
-(12 + 2 n_relax) elements per cell-update (72 B in fp32 with air), times the cell-updates one
-launch performs, over the launch's mean duration from HIP events on the launch stream;
-``traffic`` is the HBM bytes per launch rocprofv3 measured for the same kernel and workload
-(committed profile), when there is one. Under ``extra``: the same run at one step per sweep,
-the fp64 plain kernel and the compensated fp32 kernel, the error of every fp32 run against the
-fp64 run (receiver relative L2, T20 per octave band), rigid-box energy drift in fp64 and fp32,
-a 384^3 fp32 run, 64 uneven rooms advanced as one batch (each checked against its own fp64
-single-room run), the cost of a wall-admittance gradient against a forward pass at 128^3, a
-512^3 room with its T20 per octave band against fp64 and its wall gradient, and the stream
-ceiling. ``cpu_baseline`` is the reference's own compiled C loop (``oracle/_ref``, all host
-cores this process may use) on a bounded sample of the same workload.
+dominant kernel: ``achieved`` counts the bytes that kernel must move per launch (each field
+element it reads or writes counted once: 28 B per cell-update for the two-step sweep in fp32
+with three relaxation states, two cell-updates per launch) over the launch's mean duration from
+HIP events on the launch stream; ``traffic`` is the HBM bytes per launch rocprofv3 measured for
+the same kernel and workload (committed profile), when there is one. Under ``extra``: the same
+launch time on the byte count of the reference's two-sweep step (72 B per cell-update, a
+"reference-step equivalent" rate that exceeds the HBM peak when a kernel advances two steps
+per pass), the run at one step per sweep, the fp64 plain kernel and the compensated fp32
+kernel, the error of fp32 against fp64 on runs of ``--accuracy-steps`` steps whatever
+``--steps`` is (receiver relative L2, T20 per octave band), rigid-box energy drift in fp64 and
+fp32 over as many steps, a 384^3 fp32 run, 64 uneven rooms advanced as one batch (each checked
+against its own fp64 single-room run), the cost of a wall-admittance gradient against a forward
+pass at 128^3, a 512^3 room with its T20 per octave band against fp64 and its wall gradient,
+and the stream ceiling. ``cpu_baseline`` is the reference's own compiled C loop
+(``oracle/_ref``, all host cores this process may use) on a bounded sample of the same
+workload.
 """
 
 from __future__ import annotations
@@ -102,6 +106,12 @@ def launch_bytes(kind: str, n_relax: int, itemsize: int) -> int:
         "velocity": 7 * itemsize,
         "pressure": (5 + 2 * n_relax) * itemsize,
     }[kind]
+
+
+def launch_minimum_bytes(kind: str, n_relax: int, itemsize: int) -> float:
+    """Bytes per cell a launch of ``kind`` must move: :func:`minimum_bytes` times the
+    cell-updates it performs (two for the two-step sweep)."""
+    return minimum_bytes(kind, n_relax, itemsize) * (2 if kind == "fused2" else 1)
 
 
 def minimum_bytes(kind: str, n_relax: int, itemsize: int) -> float:
@@ -357,20 +367,13 @@ def gradient_cost(size, steps, seed=4):
 LARGE_OCTAVES = (250.0, 500.0, 1000.0, 2000.0, 4000.0)
 
 
-def large_room(size, steps, seed=5):
-    """``size``^3 at dx = 1 cm: the 16-cell sponge on the two x faces (rigid behind it), walls
-    of absorption 0.1 on the four others, air at 20 C / 30 % RH, a seeded Gaussian-derivative
-    pulse, 32 seeded receivers, ``steps`` steps. fp32 through the differentiable room (boundary
-    tape attached, so the forward pass is the gradient's) and its wall-admittance gradient,
-    then fp64 for the T20 deviation per octave band."""
-    import torch
-
+def large_room_workload(size, steps, seed=5):
+    """The large-room configuration: ``size``^3 at dx = 1 cm, the 16-cell sponge on the two x
+    faces (rigid behind it), walls of absorption 0.1 on the four others, air at 20 C / 30 % RH,
+    a seeded Gaussian-derivative pulse of ``steps`` samples, 32 seeded receivers."""
     from ac_fdtd_amd import (AIR, AbsorbingLayer, AirAbsorption, Grid, WallAdmittances,
                              admittance_from_absorption)
-    from ac_fdtd_amd.autograd import RoomResponse
     from ac_fdtd_amd.grid import max_stable_time_step
-    from ac_fdtd_amd.metrics import octave_band_reverberation
-    from ac_fdtd_amd.scheme import AcousticFDTD
     from ac_fdtd_amd.sources import gaussian_derivative_pulse
 
     grid = Grid(dx=DX, nx=size, ny=size, nz=size)
@@ -389,7 +392,21 @@ def large_room(size, steps, seed=5):
         "air_absorption": AirAbsorption(temperature=20.0, relative_humidity=30.0),
     }
     dt = max_stable_time_step(DX, AIR.sound_speed)
-    signal = gaussian_derivative_pulse(steps, dt, width)
+    return grid, config, source, receivers, gaussian_derivative_pulse(steps, dt, width), dt
+
+
+def large_room(size, steps, seed=5):
+    """The large-room configuration (:func:`large_room_workload`) for ``steps`` steps: fp32
+    through the differentiable room (boundary tape attached, so the forward pass is the
+    gradient's) and its wall-admittance gradient, then fp64 for the T20 deviation per octave
+    band."""
+    import torch
+
+    from ac_fdtd_amd.autograd import RoomResponse
+    from ac_fdtd_amd.metrics import octave_band_reverberation
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    grid, config, source, receivers, signal, dt = large_room_workload(size, steps, seed)
 
     def seconds(call):
         torch.cuda.synchronize()
@@ -559,6 +576,8 @@ def main() -> int:
     parser.add_argument("--large-steps", type=int, default=8000,
                         help="steps of the 512^3 room (0 skips it)")
     parser.add_argument("--batch-steps", type=int, default=3000)
+    parser.add_argument("--accuracy-steps", type=int, default=4000,
+                        help="steps of the fp32-vs-fp64 and energy-drift runs (any --steps)")
     parser.add_argument("--skip-check", action="store_true",
                         help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
@@ -607,9 +626,12 @@ def main() -> int:
     log(f"fp32 {args.size}^3, {args.steps} steps")
     f32 = gpu_throughput(args.size, args.steps, args.warmup, np.float32)
     log(f"fp32: {f32['gcells']:.2f} Gcell/s, kernels {f32['kernels']}")
-    # The dominant kernel: the one with the largest mean launch time.
+    # The dominant kernel: the one with the largest mean launch time. Its roofline counts the
+    # bytes that kernel itself must move per launch (each field element it reads or writes
+    # once, e.g. 28 B per cell-update for the two-step sweep in fp32 with three relaxation
+    # states, two cell-updates per launch), so the fraction is one of HBM bandwidth.
     dominant = max(f32["kernels"], key=f32["kernels"].get)
-    sweep_bytes = launch_bytes(dominant, f32["n_relax"], 4)
+    sweep_bytes = launch_minimum_bytes(dominant, f32["n_relax"], 4)
     achieved = sweep_bytes * f32["cells"] / (f32["kernels"][dominant] * 1e-3) / 1e9
     traffic = measured_traffic(args.size, dominant)
 
@@ -618,12 +640,19 @@ def main() -> int:
         "staged": f32["staged"],
         "tiling_tile_j_chunk_x": f32["tiling"],
         "kernel_ms": {k: round(v, 4) for k, v in f32["kernels"].items()},
-        "roofline_basis_bytes_per_cell_update": update_bytes(f32["n_relax"], 4),
         "kernel_minimum_bytes_per_cell_update": minimum_bytes(dominant, f32["n_relax"], 4),
-        # Bandwidth the kernel itself needs at this rate: its own minimum bytes per update.
-        "kernel_minimum_bytes_hbm_fraction": round(
-            minimum_bytes(dominant, f32["n_relax"], 4) * f32["gcells"] / HBM_PEAK_GBS, 4),
+        # The same launch time on the bytes of the reference's own step structure (two sweeps,
+        # every field across HBM every step: 72 B per cell-update in fp32 with air). Above 1
+        # for kernels that advance several steps per pass over memory: not an HBM fraction.
+        "reference_step_equivalent": {
+            "bytes_per_cell_update": update_bytes(f32["n_relax"], 4),
+            "gbs": round(launch_bytes(dominant, f32["n_relax"], 4) * f32["cells"]
+                         / (f32["kernels"][dominant] * 1e-3) / 1e9, 1),
+        },
     }
+    if traffic is not None:
+        extra["measured_traffic_over_minimum"] = round(traffic / (sweep_bytes * f32["cells"]), 4)
+        extra["measured_traffic_gbs"] = round(traffic / (f32["kernels"][dominant] * 1e-3) / 1e9, 1)
     if not args.skip_extra:
         log("fp32 one step per sweep")
         one = gpu_throughput(args.size, args.steps, args.warmup, np.float32, time_block=1)
@@ -632,31 +661,42 @@ def main() -> int:
             "gcells": round(one["gcells"], 3),
             "tiling_tile_j_chunk_x": one["tiling"],
             "kernel_ms": {k: round(v, 4) for k, v in one["kernels"].items()},
-            "roofline_frac": round(launch_bytes(kind, one["n_relax"], 4) * one["cells"]
+            "roofline_frac": round(launch_minimum_bytes(kind, one["n_relax"], 4) * one["cells"]
                                    / (one["kernels"][kind] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "bitwise_equal_recordings": bool(np.array_equal(one["recorded"], f32["recorded"])),
         }
         del one
-        log("fp32 compensated")
-        comp = gpu_throughput(args.size, args.steps, args.warmup, np.float32, compensated=True)
+        # Accuracy runs: a fixed number of steps whatever --steps is, long enough for every
+        # receiver to record the direct sound and the first reflections.
+        log(f"accuracy: fp32, fp32 compensated, fp64 over {args.accuracy_steps} steps")
+        acc32 = (f32 if args.steps == args.accuracy_steps else
+                 gpu_throughput(args.size, args.accuracy_steps, args.warmup, np.float32))
+        comp = gpu_throughput(args.size, args.accuracy_steps, args.warmup, np.float32,
+                              compensated=True)
         extra["fp32_compensated_gcells"] = round(comp["gcells"], 3)
         log("fp64 plain")
-        f64 = gpu_throughput(args.size, args.steps, args.warmup, np.float64)
+        f64 = gpu_throughput(args.size, args.accuracy_steps, args.warmup, np.float64)
         extra["fp64_plain_gcells"] = round(f64["gcells"], 3)
         extra["fp32_over_fp64_speed"] = round(f32["gcells"] / f64["gcells"], 3)
         extra["fp64_time_block"] = f64["time_block"]
-        extra["fp64_hbm_fraction"] = round(
-            update_bytes(f64["n_relax"], 8) * f64["gcells"] / HBM_PEAK_GBS, 4)
-        extra["fp32_error_vs_fp64"] = error_against(f32, f64)
+        kind64 = max(f64["kernels"], key=f64["kernels"].get)
+        extra["fp64_dominant_kernel_hbm_fraction"] = round(
+            launch_minimum_bytes(kind64, f64["n_relax"], 8) * f64["cells"]
+            / (f64["kernels"][kind64] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        extra["accuracy_steps"] = args.accuracy_steps
+        extra["fp32_error_vs_fp64"] = error_against(acc32, f64)
         extra["fp32_compensated_error_vs_fp64"] = error_against(comp, f64)
-        del comp, f64
+        for key in ("fp32_error_vs_fp64", "fp32_compensated_error_vs_fp64"):
+            if not math.isfinite(extra[key]["receiver_rel_l2"]):
+                raise RuntimeError(f"{key}: non-finite receiver error")
+        del comp, f64, acc32
         log("fp32 384^3")
         big = gpu_throughput(384, args.steps, args.warmup, np.float32)
         extra["fp32_384_gcells"] = round(big["gcells"], 3)
         del big
-        log("rigid-box energy drift, fp64 and fp32")
-        extra["fp64_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.steps)
-        extra["fp32_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.steps,
+        log(f"rigid-box energy drift, fp64 and fp32, {args.accuracy_steps} steps")
+        extra["fp64_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.accuracy_steps)
+        extra["fp32_rigid_energy_drift_256"] = rigid_energy_drift(args.size, args.accuracy_steps,
                                                                   np.float32)
         torch.cuda.empty_cache()
         if args.batch_rooms > 0:

@@ -208,7 +208,7 @@ int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out,
 /* Stream ceiling for the roofline: copy `bytes` (a multiple of 16, both pointers 16-byte
  * aligned) from src to dst, one 16-byte vector per lane. Measures the HBM bandwidth a plain
  * streaming kernel reaches on this device; no reference counterpart (the reference times a
- * NumPy add, benchmarks/backend_benchmark.py:86-115). */
+ * NumPy add, scripts/backend_benchmark.py:86-115). */
 int acfdtd_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------------------------ adjoint --

@@ -103,6 +103,24 @@ def test_null_descriptor_and_bad_run_arguments(lib):
     assert lib.acfdtd_hip_run(ctypes.byref(desc), 0, 0, 0, None) == 0
 
 
+def test_short_wall_tapes_are_refused_before_any_launch(lib):
+    """A tape must hold a row for every step a call writes: the two-step sweep writes rows for
+    ``step`` and ``step + 1``, the fused step and the pressure sweep one row, the run all of
+    them."""
+    tape = {"wall_tape": 4096, "wall_tape_stride": lib.acfdtd_hip_wall_tape_stride(4, 4, 4),
+            "wall_tape_step0": 10, "wall_tape_rows": 4}
+    swaps = {"p_swap": 4096, "vx_swap": 4096, "vy_swap": 4096, "vz_swap": 4096, "time_block": 2}
+    desc = _desc(**tape, **swaps)
+    for step in (9, 13, 14):  # before the tape, its last row alone, and past its end
+        assert lib.acfdtd_hip_fused2_step(ctypes.byref(desc), step, 0, 0, None) == EINVAL
+        assert "wall tape" in lib.acfdtd_hip_last_error().decode()
+    for step in (9, 14):
+        assert lib.acfdtd_hip_fused_step(ctypes.byref(desc), step, 0, 0, None) == EINVAL
+        assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(desc), step, 0, None) == EINVAL
+    assert lib.acfdtd_hip_run(ctypes.byref(desc), 11, 4, 0, None) == EINVAL
+    assert "wall tape" in lib.acfdtd_hip_last_error().decode()
+
+
 def test_energy_workspace_size_follows_the_tiling(lib):
     small = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc()))
     large = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc(nx=200, ny=130, nz=70)))
