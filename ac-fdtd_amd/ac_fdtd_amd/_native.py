@@ -33,6 +33,8 @@ __all__ = [
 
 ABI_VERSION = 1
 MAX_RELAXATION = 4
+#: Workgroups whose energy partials fit the library's fixed scratch (acfdtd_hip_energy_noalloc).
+ENERGY_SCRATCH_TILES = 32768
 WALL_CODES = 64
 COMPENSATED = 1
 STAGED = 2
@@ -192,6 +194,8 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_energy_workspace_bytes.argtypes = [_c.POINTER(Desc)]
     lib.acfdtd_hip_energy.restype = _c.c_int
     lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
+    lib.acfdtd_hip_energy_noalloc.restype = _c.c_int
+    lib.acfdtd_hip_energy_noalloc.argtypes = [_c.POINTER(Desc), _ptr, _ptr]
     if lib.acfdtd_hip_abi_version() != ABI_VERSION:
         raise RuntimeError(
             f"library ABI {lib.acfdtd_hip_abi_version()} does not match binding {ABI_VERSION}"

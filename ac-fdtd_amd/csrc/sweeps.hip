@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "acfdtd_hip.h"
@@ -237,6 +238,7 @@ template <typename real>
 int energy_terms(const acfdtd_hip_desc &d, void *workspace, double *out, hipStream_t stream) {
     SweepArgs<real> a;
     fill_args(d, a);
+    if (a.tiles.launched > INT32_MAX / 3) return fail(ACFDTD_HIP_EINVAL, "too many energy tiles");
     double *partials = static_cast<double *>(workspace);
     hipLaunchKernelGGL(energy_partials<real>, dim3(a.tiles.launched), dim3(64, a.tiles.tile_j), 0,
                        stream, a, partials);
@@ -330,6 +332,75 @@ extern "C" int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, d
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return energy_terms<double>(*desc, workspace, out, s);
     return energy_terms<float>(*desc, workspace, out, s);
+}
+
+// ------------------------------------------------------------- energy without allocation --
+// acfdtd_hip_energy_noalloc keeps the per-workgroup partials in this library-owned array
+// instead of a caller workspace. Each device holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for the current device
+// (the symbol's name is a host-side stand-in, not a device address).
+namespace acfdtd {
+
+static __device__ __align__(256) double scratch[3 * ACFDTD_HIP_ENERGY_SCRATCH_TILES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_address(double **out) {
+    int device = -1;
+    const int status = check_hip(hipGetDevice(&device), "hipGetDevice");
+    if (status != ACFDTD_HIP_OK) return status;
+    if (device < 0 || device >= kMaxDevices)
+        return fail(ACFDTD_HIP_EDEVICE, "device index " + std::to_string(device) + " out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device] == nullptr) {
+        void *address = nullptr;
+        const int rc = check_hip(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)),
+                                 "hipGetSymbolAddress(scratch)");
+        if (rc != ACFDTD_HIP_OK) return rc;
+        g_scratch[device] = static_cast<double *>(address);
+    }
+    *out = g_scratch[device];
+    return ACFDTD_HIP_OK;
+}
+
+// The descriptor's own tiling when its partials fit the scratch (then the sums are those of
+// acfdtd_hip_energy bit for bit), else 8-row blocks with the shortest marches that fit.
+int scratch_tiling(acfdtd_hip_desc &d) {
+    if (tiling_for(d).launched <= ACFDTD_HIP_ENERGY_SCRATCH_TILES) return ACFDTD_HIP_OK;
+    d.tile_j = 8;
+    const Tiling t = make_tiling(d.nx, d.ny, d.nz, d.tile_j, 1);
+    const int64_t columns = (int64_t)t.nblk_k * t.nblk_j;
+    const int64_t chunks = ACFDTD_HIP_ENERGY_SCRATCH_TILES / columns;
+    if (chunks < 1)
+        return fail(ACFDTD_HIP_EINVAL,
+                    "grid too large for the energy scratch: ceil(ny / 8) * ceil(nz / 64) = " +
+                        std::to_string(columns) + " workgroup columns exceed " +
+                        std::to_string(ACFDTD_HIP_ENERGY_SCRATCH_TILES));
+    d.chunk_x = (int)((d.nx + chunks - 1) / chunks);
+    return tiling_for(d).launched <= ACFDTD_HIP_ENERGY_SCRATCH_TILES
+               ? ACFDTD_HIP_OK
+               : fail(ACFDTD_HIP_EINVAL, "energy tiling does not fit the scratch");
+}
+
+}  // namespace acfdtd
+
+extern "C" int acfdtd_hip_energy_noalloc(const acfdtd_hip_desc *desc, double *out, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (out == nullptr) return fail(ACFDTD_HIP_EINVAL, "energy needs an output buffer");
+    acfdtd_hip_desc d = *desc;
+    int rc = scratch_tiling(d);
+    if (rc != ACFDTD_HIP_OK) return rc;
+    double *partials = nullptr;
+    rc = scratch_address(&partials);
+    if (rc != ACFDTD_HIP_OK) return rc;
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (d.real_bytes == 8) return energy_terms<double>(d, partials, out, s);
+    return energy_terms<float>(d, partials, out, s);
 }
 
 namespace acfdtd {

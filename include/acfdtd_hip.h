@@ -205,6 +205,21 @@ int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
 size_t acfdtd_hip_energy_workspace_bytes(const acfdtd_hip_desc *desc);
 int acfdtd_hip_energy(const acfdtd_hip_desc *desc, void *workspace, double *out, void *stream);
 
+/* acfdtd_hip_energy without a workspace and without any allocation: the per-workgroup partials
+ * live in a library-owned device array of 3 * ACFDTD_HIP_ENERGY_SCRATCH_TILES doubles (768 KiB,
+ * held by each device the library runs on until the process ends; its address is looked up
+ * once per device, on that device's first call). Same results as acfdtd_hip_energy bit for bit
+ * when the descriptor's tiling has at most ACFDTD_HIP_ENERGY_SCRATCH_TILES workgroups; larger
+ * tilings are replaced by 8-row blocks with longer marches (the fp64 sums then differ in
+ * order only), and grids with more than ACFDTD_HIP_ENERGY_SCRATCH_TILES workgroup columns
+ * (ceil(ny / 8) * ceil(nz / 64)) are refused with ACFDTD_HIP_EINVAL. Every workgroup writes its
+ * own partials, so nothing needs clearing between calls; calls that overlap on one device
+ * (other streams or threads) share the array and race. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation). */
+#define ACFDTD_HIP_ENERGY_SCRATCH_TILES 32768
+int acfdtd_hip_energy_noalloc(const acfdtd_hip_desc *desc, double *out, void *stream);
+
 /* Stream ceiling for the roofline: copy `bytes` (a multiple of 16, both pointers 16-byte
  * aligned) from src to dst, one 16-byte vector per lane. Measures the HBM bandwidth a plain
  * streaming kernel reaches on this device; no reference counterpart (the reference times a

@@ -121,6 +121,19 @@ def test_short_wall_tapes_are_refused_before_any_launch(lib):
     assert "wall tape" in lib.acfdtd_hip_last_error().decode()
 
 
+def test_energy_without_allocation_refuses_grids_beyond_its_scratch(lib):
+    """ceil(ny / 8) * ceil(nz / 64) workgroup columns must fit the fixed scratch; refused before
+    any HIP call."""
+    columns = -(-2400 // 8) * -(-12800 // 64)
+    assert columns > _native.ENERGY_SCRATCH_TILES
+    desc = _desc(nx=2, ny=2400, nz=12800)
+    out = ctypes.c_double * 3
+    assert lib.acfdtd_hip_energy_noalloc(ctypes.byref(desc), out(), None) == EINVAL
+    assert "energy scratch" in lib.acfdtd_hip_last_error().decode()
+    assert lib.acfdtd_hip_energy_noalloc(ctypes.byref(_desc()), None, None) == EINVAL
+    assert lib.acfdtd_hip_energy_noalloc(None, out(), None) == EINVAL
+
+
 def test_energy_workspace_size_follows_the_tiling(lib):
     small = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc()))
     large = lib.acfdtd_hip_energy_workspace_bytes(ctypes.byref(_desc(nx=200, ny=130, nz=70)))
