@@ -55,8 +55,55 @@ class RelaxationProcess:
 
 
 @dataclass(frozen=True)
+class _Atmosphere:
+    """The ISO 9613-1 quantities of one atmosphere, worked out once: absolute temperature,
+    pressure and temperature ratios to the reference state, the molar water concentration
+    ``h`` (%), the O2 and N2 relaxation frequencies (Hz), and the coefficients of the classical
+    term (Np/(m Hz^2)) and of the two relaxation terms (Np/m at the relaxation maximum per
+    unit frequency ratio)."""
+
+    kelvin: float
+    pressure_ratio: float
+    temperature_ratio: float
+    water: float
+    oxygen_frequency: float
+    nitrogen_frequency: float
+    classical: float
+    oxygen: float
+    nitrogen: float
+
+    @classmethod
+    def of(cls, celsius: float, humidity: float, pressure: float) -> "_Atmosphere":
+        # Each expression keeps the reference's order of operations (air.py:137-175), so the
+        # strengths handed to the kernels match it bit for bit.
+        kelvin = celsius + _KELVIN
+        pr = pressure / _P_REF
+        tr = kelvin / _T_REF
+        saturation_exponent = -6.8346 * (_T_TRIPLE / kelvin) ** 1.261 + 4.6151
+        h = humidity * 10.0**saturation_exponent / pr
+        return cls(
+            kelvin=kelvin,
+            pressure_ratio=pr,
+            temperature_ratio=tr,
+            water=h,
+            oxygen_frequency=pr * (24.0 + 4.04e4 * h * (0.02 + h) / (0.391 + h)),
+            nitrogen_frequency=pr * tr**-0.5
+            * (9.0 + 280.0 * h * math.exp(-4.170 * (tr ** (-1 / 3) - 1.0))),
+            classical=1.84e-11 / pr * tr**0.5,
+            oxygen=tr**-2.5 * 0.01275 * math.exp(-2239.1 / kelvin),
+            nitrogen=tr**-2.5 * 0.1068 * math.exp(-3352.0 / kelvin),
+        )
+
+    def relaxations(self) -> tuple[tuple[float, float], ...]:
+        """(coefficient, relaxation frequency) of O2, then N2."""
+        return ((self.oxygen, self.oxygen_frequency), (self.nitrogen, self.nitrogen_frequency))
+
+
+@dataclass(frozen=True)
 class AirAbsorption:
-    """Atmospheric conditions (deg C, % RH, Pa) and which loss terms to model (air.py:67-112)."""
+    """Atmospheric conditions (deg C, % RH, Pa) and which loss terms to model (air.py:67-112):
+    ``classical`` the viscous, thermal and rotational ``f^2`` term, ``relaxation`` the O2 and
+    N2 relaxations that dominate the audio band."""
 
     temperature: float = 20.0
     relative_humidity: float = 50.0
@@ -66,100 +113,69 @@ class AirAbsorption:
 
     def __post_init__(self) -> None:
         if not 0.0 <= self.relative_humidity <= 100.0:
-            raise ValueError(f"relative_humidity is a percentage, got {self.relative_humidity}")
+            raise ValueError(f"relative humidity is given in percent (0 to 100); "
+                             f"{self.relative_humidity} is outside that")
         if self.temperature <= -_KELVIN:
-            raise ValueError(f"temperature must be above absolute zero, got {self.temperature}")
+            raise ValueError(f"{self.temperature} C lies at or below absolute zero")
+
+    @property
+    def _atmosphere(self) -> _Atmosphere:
+        return _Atmosphere.of(self.temperature, self.relative_humidity, self.static_pressure)
 
     @property
     def kelvin(self) -> float:
-        return self.temperature + _KELVIN
-
-    @property
-    def _pa(self) -> float:
-        return self.static_pressure / _P_REF
-
-    @property
-    def _tr(self) -> float:
-        return self.kelvin / _T_REF
+        return self._atmosphere.kelvin
 
     @property
     def molar_water_concentration(self) -> float:
-        """``h`` in molar %, from the ISO saturation-pressure fit (air.py:137-146)."""
-        exponent = -6.8346 * (_T_TRIPLE / self.kelvin) ** 1.261 + 4.6151
-        return self.relative_humidity * 10.0**exponent / self._pa
+        """``h`` in molar %, from the ISO saturation-pressure fit."""
+        return self._atmosphere.water
 
     @property
     def oxygen_relaxation_frequency(self) -> float:
-        h = self.molar_water_concentration
-        return self._pa * (24.0 + 4.04e4 * h * (0.02 + h) / (0.391 + h))
+        return self._atmosphere.oxygen_frequency
 
     @property
     def nitrogen_relaxation_frequency(self) -> float:
-        h = self.molar_water_concentration
-        return (
-            self._pa
-            * self._tr**-0.5
-            * (9.0 + 280.0 * h * math.exp(-4.170 * (self._tr ** (-1 / 3) - 1.0)))
-        )
-
-    @property
-    def _classical(self) -> float:
-        """``B`` of the classical term ``B f^2`` in Np/(m Hz^2)."""
-        return 1.84e-11 / self._pa * self._tr**0.5
-
-    @property
-    def _oxygen(self) -> float:
-        return self._tr**-2.5 * 0.01275 * math.exp(-2239.1 / self.kelvin)
-
-    @property
-    def _nitrogen(self) -> float:
-        return self._tr**-2.5 * 0.1068 * math.exp(-3352.0 / self.kelvin)
-
-    def _relaxation_terms(self):
-        return (
-            (self._oxygen, self.oxygen_relaxation_frequency),
-            (self._nitrogen, self.nitrogen_relaxation_frequency),
-        )
+        return self._atmosphere.nitrogen_frequency
 
     def attenuation(self, frequencies: np.ndarray) -> np.ndarray:
-        """ISO 9613-1 attenuation in dB/m — the target, not the model (air.py:177-190)."""
+        """ISO 9613-1 attenuation in dB/m: the curve the model is matched to (air.py:177-190)."""
         f = np.asarray(frequencies, dtype=float)
-        total = np.zeros_like(f)
+        atmosphere = self._atmosphere
+        terms = []
         if self.classical:
-            total += self._classical * f**2
+            terms.append(atmosphere.classical * f**2)
         if self.relaxation:
-            for coefficient, relax in self._relaxation_terms():
-                total += coefficient * f**2 * relax / (relax**2 + f**2)
-        return _DB_PER_NEPER * total
+            terms += [a * f**2 * fr / (fr**2 + f**2) for a, fr in atmosphere.relaxations()]
+        nepers = np.zeros_like(f)
+        for term in terms:
+            nepers += term
+        return _DB_PER_NEPER * nepers
 
     def processes(self, sound_speed: float, dt: float) -> tuple[RelaxationProcess, ...]:
-        """O2, N2 (if ``relaxation``), then classical (if ``classical``) (air.py:192-218)."""
-        out = []
-        if self.relaxation:
-            for coefficient, relax in self._relaxation_terms():
-                out.append(
-                    RelaxationProcess(
-                        strength=coefficient * sound_speed / math.pi,
-                        relaxation_time=1.0 / (2.0 * math.pi * relax),
-                    )
-                )
+        """The relaxation states the time step carries: O2 and N2 (``relaxation``), then the
+        classical term as a process relaxing at the Nyquist frequency of ``dt``
+        (``classical``): ``Delta = a c / pi``, ``tau = 1 / (2 pi F)`` (air.py:192-218)."""
+        atmosphere = self._atmosphere
+        pairs = list(atmosphere.relaxations()) if self.relaxation else []
         if self.classical:
             nyquist = 0.5 / dt
-            out.append(
-                RelaxationProcess(
-                    strength=self._classical * nyquist * sound_speed / math.pi,
-                    relaxation_time=1.0 / (2.0 * math.pi * nyquist),
-                )
-            )
-        return tuple(out)
+            pairs.append((atmosphere.classical * nyquist, nyquist))
+        return tuple(
+            RelaxationProcess(strength=a * sound_speed / math.pi,
+                              relaxation_time=1.0 / (2.0 * math.pi * fr))
+            for a, fr in pairs
+        )
 
     def model_attenuation(self, frequencies: np.ndarray, sound_speed: float, dt: float):
-        """dB/m of the implemented relaxing medium, before discretisation (air.py:220-233)."""
+        """dB/m of the relaxing medium the solver implements, before discretisation: the ISO
+        curve except where the classical term's surrogate relaxation rolls off."""
         f = np.asarray(frequencies, dtype=float)
-        total = np.zeros_like(f)
+        nepers = np.zeros_like(f)
         for process in self.processes(sound_speed, dt):
-            total += process.attenuation(f, sound_speed)
-        return _DB_PER_NEPER * total
+            nepers += process.attenuation(f, sound_speed)
+        return _DB_PER_NEPER * nepers
 
     def total_strength(self, sound_speed: float, dt: float) -> float:
         return sum(process.strength for process in self.processes(sound_speed, dt))

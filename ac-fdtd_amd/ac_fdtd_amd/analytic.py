@@ -40,8 +40,9 @@ def discrete_mode_frequency(grid: Grid, mode, medium: Medium, dt: float) -> floa
     argument = medium.sound_speed * dt / dx * spatial
     if argument > 1.0:
         raise ValueError(
-            f"mode {mode} is not representable: dispersion argument {argument:.6f} > 1, "
-            "which means this time step is unstable for this grid"
+            f"the discrete dispersion relation has no real frequency for mode {mode} "
+            f"(sin(pi f dt) would be {argument:.6f}): the time step exceeds the stability limit "
+            "of this grid"
         )
     return math.asin(argument) / (math.pi * dt)
 

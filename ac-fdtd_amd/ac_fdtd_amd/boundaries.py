@@ -58,7 +58,7 @@ WALL_CODE_HIGH = 2
 def admittance_from_absorption(absorption: float) -> float:
     """``xi = (1 - R) / (1 + R)`` with ``R = sqrt(1 - alpha)`` (boundaries.py:76-87)."""
     if not 0.0 <= absorption <= 1.0:
-        raise ValueError(f"absorption must be in [0, 1], got {absorption}")
+        raise ValueError(f"an absorption coefficient lies between 0 and 1; {absorption} was given")
     reflection = math.sqrt(1.0 - absorption)
     return (1.0 - reflection) / (1.0 + reflection)
 
@@ -92,8 +92,8 @@ class WallAdmittances:
         for face, value in zip(FACES, self.values):
             if value < 0.0:
                 raise ValueError(
-                    f"wall {face} has negative admittance {value}: that is a wall that "
-                    "generates sound, and it will make the run diverge"
+                    f"the admittance of wall {face} is {value} < 0: such a wall would feed "
+                    "energy into the room (it generates sound) and the run would diverge"
                 )
 
     @property
@@ -211,9 +211,9 @@ class AbsorbingLayer:
 
     def __post_init__(self) -> None:
         if self.thickness < 1:
-            raise ValueError(f"thickness must be at least one cell, got {self.thickness}")
+            raise ValueError(f"the layer needs a thickness of one cell or more; {self.thickness} was given")
         if not 0.0 < self.target_reflection < 1.0:
-            raise ValueError(f"target_reflection must be in (0, 1), got {self.target_reflection}")
+            raise ValueError(f"the layer's target reflection must lie strictly between 0 and 1; {self.target_reflection} was given")
 
     def max_damping(self, dx: float, sound_speed: float) -> float:
         """``sigma_max = -(order + 1) c ln(R) / (2 d)`` with ``d = thickness * dx``."""

@@ -59,6 +59,13 @@ class SchemeCoefficients:
         return self.cell_profiles is not None
 
 
+def time_step(grid: Grid, medium: Medium, courant: float) -> float:
+    """``courant`` times the 3D Courant limit ``dx / (c sqrt 3)`` (scheme.py:127)."""
+    if not 0.0 < courant <= 1.0:
+        raise ValueError(f"the Courant number must lie in (0, 1]; {courant} was given")
+    return courant * max_stable_time_step(grid.dx, medium.sound_speed)
+
+
 def _round(dtype: np.dtype, value) -> float:
     return float(dtype.type(value))
 
@@ -80,14 +87,12 @@ def build_coefficients(
     model would give, for differentiating with respect to them; ``wall_table_always`` builds
     the wall table even for rigid walls (needed by the adjoint, which differentiates every
     face)."""
-    if not 0.0 < courant <= 1.0:
-        raise ValueError(f"courant must be in (0, 1], got {courant}")
+    dt = time_step(grid, medium, courant)
     dtype = np.dtype(dtype)
     if dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
         raise ValueError(f"dtype must be float32 or float64, got {dtype}")
     walls = walls or WallAdmittances()
     c, rho, dx = medium.sound_speed, medium.density, grid.dx
-    dt = courant * max_stable_time_step(dx, c)
 
     out = SchemeCoefficients(
         dtype=dtype,

@@ -40,9 +40,9 @@ class Grid:
 
     def __post_init__(self) -> None:
         if not self.dx > 0.0:
-            raise ValueError(f"dx must be positive, got {self.dx}")
+            raise ValueError(f"the cell size dx must be positive; {self.dx} was given")
         if min(self.nx, self.ny, self.nz) < 1:
-            raise ValueError(f"cell counts must be >= 1, got {self.shape}")
+            raise ValueError(f"every axis needs at least one cell; the shape was {self.shape}")
 
     @classmethod
     def from_lengths(cls, lengths: tuple[float, float, float], dx: float) -> Grid:
@@ -60,9 +60,9 @@ class Grid:
     ) -> Grid:
         """Spacing ``c / (ppw * f_max)``, then :meth:`from_lengths` (grid.py:76-96)."""
         if not max_frequency > 0.0:
-            raise ValueError(f"max_frequency must be positive, got {max_frequency}")
+            raise ValueError(f"the band limit must be a positive frequency; {max_frequency} Hz was given")
         if not points_per_wavelength > 0.0:
-            raise ValueError(f"points_per_wavelength must be positive, got {points_per_wavelength}")
+            raise ValueError(f"the resolution must be a positive number of points per wavelength; {points_per_wavelength} was given")
         return cls.from_lengths(lengths, sound_speed / (points_per_wavelength * max_frequency))
 
     @property

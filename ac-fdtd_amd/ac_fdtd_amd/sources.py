@@ -42,14 +42,15 @@ def band_limited_impulse(n_steps: int, dt: float, max_frequency: float) -> np.nd
     """
     if max_frequency >= 0.5 / dt:
         raise ValueError(
-            f"max_frequency {max_frequency} is at or above the Nyquist frequency {0.5 / dt}"
+            f"a band limit of {max_frequency} Hz does not lie below this time step's Nyquist "
+            f"frequency, {0.5 / dt} Hz"
         )
     half = int(np.ceil(4.0 / (max_frequency * dt)))
     taps = 2 * half + 1
     if taps > n_steps:
         raise ValueError(
-            f"a pulse band-limited to {max_frequency} Hz needs {taps} steps, "
-            f"but only {n_steps} were asked for"
+            f"{n_steps} steps cannot hold the {taps}-tap impulse band-limited to "
+            f"{max_frequency} Hz"
         )
     band = 2.0 * max_frequency * dt
     kernel = band * np.sinc(band * np.arange(-half, half + 1))

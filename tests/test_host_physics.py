@@ -347,5 +347,5 @@ def test_coefficients_agree_with_the_oracle(dtype):
 def test_courant_outside_the_unit_interval_is_refused():
     grid = Grid(dx=0.1, nx=2, ny=2, nz=2)
     for bad in (0.0, 1.01, -0.5):
-        with pytest.raises(ValueError, match="courant"):
+        with pytest.raises(ValueError, match="(?i)courant"):
             build_coefficients(grid, AIR, bad, np.float64, None, None, None)
