@@ -22,6 +22,7 @@ __all__ = [
     "COMPENSATED",
     "Desc",
     "STAGED",
+    "WaveDesc",
     "GRAD_AIR",
     "GRAD_SLOTS",
     "GRAD_SPONGE",
@@ -139,6 +140,36 @@ class AdjointDesc(_c.Structure):
     ]
 
 
+class WaveDesc(_c.Structure):
+    """Mirror of ``acfdtd_hip_wave_desc`` (the pressure-only scheme)."""
+
+    _fields_ = [
+        ("abi_version", _c.c_uint32),
+        ("struct_size", _c.c_uint32),
+        ("nx", _c.c_int32),
+        ("ny", _c.c_int32),
+        ("nz", _c.c_int32),
+        ("real_bytes", _c.c_int32),
+        ("n_sources", _c.c_int32),
+        ("n_receivers", _c.c_int32),
+        ("tile_j", _c.c_int32),
+        ("chunk_x", _c.c_int32),
+        ("courant2", _c.c_double),
+        ("src_coef", _c.c_double),
+        ("wall_mask", _c.c_uint64),
+        ("wall_table", _ptr),
+        ("p", _ptr),
+        ("p_prev", _ptr),
+        ("source_cell", _ptr),
+        ("source_length", _ptr),
+        ("source_signal", _ptr),
+        ("source_stride", _c.c_int64),
+        ("receiver_cell", _ptr),
+        ("recording", _ptr),
+        ("recording_stride", _c.c_int64),
+    ]
+
+
 _library: ctypes.CDLL | None = None
 
 
@@ -196,6 +227,10 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
     lib.acfdtd_hip_energy_noalloc.restype = _c.c_int
     lib.acfdtd_hip_energy_noalloc.argtypes = [_c.POINTER(Desc), _ptr, _ptr]
+    lib.acfdtd_hip_wave_struct_size.restype = _c.c_size_t
+    lib.acfdtd_hip_wave_run.restype = _c.c_int
+    lib.acfdtd_hip_wave_run.argtypes = [_c.POINTER(WaveDesc), _c.c_int64, _c.c_int64, _c.c_int64,
+                                        _ptr]
     if lib.acfdtd_hip_abi_version() != ABI_VERSION:
         raise RuntimeError(
             f"library ABI {lib.acfdtd_hip_abi_version()} does not match binding {ABI_VERSION}"
@@ -205,6 +240,9 @@ def library() -> ctypes.CDLL:
             "the ctypes mirror of acfdtd_hip_desc no longer matches the library: "
             f"{ctypes.sizeof(Desc)} bytes here, {lib.acfdtd_hip_struct_size()} there"
         )
+    for name, mirror in (("acfdtd_hip_wave_struct_size", WaveDesc),):
+        if getattr(lib, name)() != ctypes.sizeof(mirror):
+            raise RuntimeError(f"the ctypes mirror {mirror.__name__} no longer matches the library")
     _library = lib
     return lib
 

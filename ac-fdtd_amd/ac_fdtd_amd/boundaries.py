@@ -38,11 +38,13 @@ __all__ = [
     "WallAdmittances",
     "admittance_from_absorption",
     "axis_wall_codes",
+    "cell_wall_codes",
     "edge_slabs",
     "layer_factors",
     "layer_weights",
     "random_incidence_absorption",
     "reflection_coefficient",
+    "wall_coefficient_sums",
     "wall_coefficient_table",
     "wall_update_coefficients",
 ]
@@ -171,6 +173,27 @@ def wall_coefficient_table(
         updated[code] = 1.0 / (1.0 + a)
         previous[code] = a / (1.0 + a)
     return is_wall, updated, previous
+
+
+def wall_coefficient_sums(
+    grid: Grid, medium: Medium, dt: float, walls: WallAdmittances
+) -> tuple[np.ndarray, np.ndarray]:
+    """``(is_wall, a)`` per wall code ``cx + 4 cy + 16 cz``: whether any touched face absorbs,
+    and the summed ``a`` of the touched faces (0 for interior and rigid codes), fp64."""
+    face_a = _face_coefficients(grid, medium, dt, walls)
+    is_wall = np.zeros(64, dtype=bool)
+    total = np.zeros(64)
+    for code in range(64):
+        cx, cy, cz = code & 3, (code >> 2) & 3, (code >> 4) & 3
+        total[code] = _summed_coefficient(face_a, cx, cy, cz)
+        is_wall[code] = total[code] != 0.0
+    return is_wall, total
+
+
+def cell_wall_codes(grid: Grid) -> np.ndarray:
+    """Wall code ``cx + 4 cy + 16 cz`` of every cell, shape ``grid.shape``."""
+    cx, cy, cz = (axis_wall_codes(n) for n in grid.shape)
+    return cx[:, None, None] + 4 * cy[None, :, None] + 16 * cz[None, None, :]
 
 
 def wall_update_coefficients(

@@ -51,6 +51,7 @@ SOURCES = (
     "batch2_v1.hip",
     "batch2_64.hip",
     "batch2_64_v1.hip",
+    "wave.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

@@ -314,6 +314,53 @@ int acfdtd_hip_run_states(const acfdtd_hip_desc *desc, int64_t step_offset, int6
                           void *p_states, void *vx_states, void *vy_states, void *vz_states,
                           void *stream);
 
+/* ------------------------------------------------------ pressure-only (wave) scheme --
+ * The second-order wave equation on pressure alone: the variant the reference plans and does
+ * not implement ("two fields instead of four", PLAN.md:52-56, milestone M8 at PLAN.md:116).
+ * It is the first-order step (scheme.py:285-363) with the velocity eliminated, exactly, for
+ * rigid and locally reacting walls and soft sources (no sponge and no air absorption: those
+ * attach to the velocity and the relaxation states). With lambda^2 = (c dt / dx)^2, a the
+ * summed wall coefficient of the cell (0 inside) and L the 7-point Laplacian over the cell's
+ * interior faces:
+ *   p^{n+1} = ((2 p^n - (1 - a) p^{n-1}) + lambda^2 L p^n) / (1 + a)
+ *   then, per source in insertion order, p^{n+1}[cell] += src s[n] - src s[n - 1].
+ * One sweep per step reads p^n and p^{n-1} and writes p^{n+1} over p^{n-1}: 3 elements per
+ * cell-update. No reference counterpart (the reference has no such scheme). */
+typedef struct acfdtd_hip_wave_desc {
+    uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
+    uint32_t struct_size; /* sizeof(acfdtd_hip_wave_desc) */
+    int32_t nx, ny, nz;
+    int32_t real_bytes; /* 4 or 8 */
+    int32_t n_sources;
+    int32_t n_receivers;
+    int32_t tile_j;  /* wavefronts per workgroup, 1 to 8; 0 = 4 */
+    int32_t chunk_x; /* planes marched per workgroup; 0 = default */
+    double courant2; /* lambda^2, already rounded to the working precision */
+    double src_coef; /* rho c^2 dt, already rounded */
+    /* Bit c of wall_mask: wall code c (as in acfdtd_hip_desc) touches an absorbing face;
+     * wall_table (device, working precision) holds 1 / (1 + a) at [c] and 1 - a at [64 + c]. */
+    uint64_t wall_mask;
+    const void *wall_table;
+    /* p^n and p^{n-1} of the call's first step (device, nx*ny*nz each, distinct). The two
+     * swap roles every step: after an even n_steps p holds the newest field, after an odd one
+     * p_prev does. */
+    void *p, *p_prev;
+    /* Sources and receivers, laid out as in acfdtd_hip_desc. */
+    const int32_t *source_cell;
+    const int64_t *source_length;
+    const void *source_signal;
+    int64_t source_stride;
+    const int32_t *receiver_cell;
+    void *recording;
+    int64_t recording_stride;
+} acfdtd_hip_wave_desc;
+
+size_t acfdtd_hip_wave_struct_size(void);
+/* Advance n_steps from absolute step step_offset on `stream`; receivers record p^{n+1} of
+ * each step into columns recording_column onwards. */
+int acfdtd_hip_wave_run(const acfdtd_hip_wave_desc *desc, int64_t step_offset, int64_t n_steps,
+                        int64_t recording_column, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);
 
