@@ -14,8 +14,8 @@
 // L1/L2. p^{n-1} is read once and p^{n+1} written over it. That is 3 elements per
 // cell-update (12 B fp32), an HBM-bound stencil with no MFMA work. Workgroup ids are remapped
 // so each of the 8 XCDs owns a contiguous range of tiles (neighbouring rows share one L2).
-// Sources and receivers, a handful of cells, run in a one-workgroup kernel after the sweep,
-// sources in insertion order.
+// Sources (in insertion order) and receivers are handled by the lanes that own their cells,
+// on the values still in registers.
 
 #include <hip/hip_runtime.h>
 
@@ -57,11 +57,7 @@ struct WaveArgs {
     int nx, ny, nz;
     real courant2;
     const real *wall_table;
-};
-
-template <typename real>
-struct WaveEvents {
-    int nx, ny, nz;
+    // Sources and receivers (cells as i and j * nz + k), handled by the lanes that own them.
     int n_sources, n_receivers;
     real src;
     const int32_t *source_cell;
@@ -77,7 +73,8 @@ __device__ __forceinline__ int axis_class(int i, int n) { return (i == 0 ? 1 : 0
 
 template <typename real, int V, bool WALLS>
 __global__ __launch_bounds__(512) void wave_step(const WaveArgs<real> a, const real *__restrict__ p,
-                                                 real *__restrict__ q) {
+                                                 real *__restrict__ q, const int64_t step,
+                                                 const int64_t column) {
     const WaveTiling &t = a.t;
     unsigned b = blockIdx.x;
     b = (b & 7u) * ((unsigned)t.launched >> 3) + (b >> 3);
@@ -93,6 +90,7 @@ __global__ __launch_bounds__(512) void wave_step(const WaveArgs<real> a, const r
     const int i1 = min(i0 + t.chunk, nx);
     const int64_t plane = (int64_t)ny * nz;
     const int64_t row = (int64_t)j * nz + k0;
+    const int jk0 = j * nz + k0;
     // Neighbour offsets within a plane; a missing neighbour is read from the lane's own cells
     // (in bounds) and its flux replaced by 0.
     const bool has_ylo = j > 0, has_yhi = j + 1 < ny;
@@ -103,9 +101,22 @@ __global__ __launch_bounds__(512) void wave_step(const WaveArgs<real> a, const r
     const int64_t z_hi = has_zhi ? row + V : row;
     const int code_y = axis_class(j, ny) << 2;
     const real zero = real(0), two = real(2), lam2 = a.courant2;
-
     Cells<real, V> below = load_cells<real, V>(p, (int64_t)(i0 > 0 ? i0 - 1 : i0) * plane + row);
     Cells<real, V> here = load_cells<real, V>(p, (int64_t)i0 * plane + row);
+    // Whether a source or receiver lies in this workgroup's march (worked out after the first
+    // loads are issued, so the table reads overlap them): lane g owns the flattened
+    // cells j * nz + k in [g V, g V + V), so the workgroup owns one contiguous range of them.
+    const int jk_first = blk * t.threads * V, jk_last = jk_first + t.threads * V;
+    bool events = false;
+    for (int s = 0; s < a.n_sources; ++s) {
+        const int is = a.source_cell[2 * s], jk = a.source_cell[2 * s + 1];
+        events = events || (is >= i0 && is < i1 && jk >= jk_first && jk < jk_last);
+    }
+    for (int r = 0; r < a.n_receivers; ++r) {
+        const int ir = a.receiver_cell[2 * r], jk = a.receiver_cell[2 * r + 1];
+        events = events || (ir >= i0 && ir < i1 && jk >= jk_first && jk < jk_last);
+    }
+
     for (int i = i0; i < i1; ++i) {
         const int64_t base = (int64_t)i * plane;
         const bool has_xhi = i + 1 < nx;
@@ -140,31 +151,33 @@ __global__ __launch_bounds__(512) void wave_step(const WaveArgs<real> a, const r
                 out.c[m] = (c * two - old.c[m]) + lam2 * lap;
             }
         }
+        // Sources in insertion order, then receivers, in the few workgroups that hold one.
+        const bool plane_events = events;
+        for (int s = 0; plane_events && s < a.n_sources; ++s) {
+            if (a.source_cell[2 * s] != i) continue;
+            const int at = a.source_cell[2 * s + 1] - jk0;
+            if (at < 0 || at >= V) continue;
+            const int64_t length = a.source_length[s];
+            const real *signal = a.source_signal + (int64_t)s * a.source_stride;
+            const real now = step < length ? a.src * signal[step] : zero;
+            const real before = (step >= 1 && step - 1 < length) ? a.src * signal[step - 1] : zero;
+#pragma unroll
+            for (int m = 0; m < V; ++m)
+                if (m == at) out.c[m] = out.c[m] + (now - before);
+        }
         *reinterpret_cast<Cells<real, V> *>(q + base + row) = out;
+        for (int r = 0; plane_events && r < a.n_receivers; ++r) {
+            if (a.receiver_cell[2 * r] != i) continue;
+            const int at = a.receiver_cell[2 * r + 1] - jk0;
+            if (at < 0 || at >= V) continue;
+            real value = out.c[0];
+#pragma unroll
+            for (int m = 1; m < V; ++m)
+                if (m == at) value = out.c[m];
+            a.recording[(int64_t)r * a.recording_stride + column] = value;
+        }
         below = here;
         here = above;
-    }
-}
-
-// Sources (insertion order, one thread) then receivers, on the step's new field.
-template <typename real>
-__global__ __launch_bounds__(256) void wave_events(const WaveEvents<real> e, real *__restrict__ p_new,
-                                                   const int64_t step, const int64_t column) {
-    const int64_t plane = (int64_t)e.ny * e.nz;
-    if (threadIdx.x == 0) {
-        for (int s = 0; s < e.n_sources; ++s) {
-            const int64_t at = (int64_t)e.source_cell[2 * s] * plane + e.source_cell[2 * s + 1];
-            const int64_t length = e.source_length[s];
-            const real *signal = e.source_signal + (int64_t)s * e.source_stride;
-            const real now = step < length ? e.src * signal[step] : real(0);
-            const real before = (step >= 1 && step - 1 < length) ? e.src * signal[step - 1] : real(0);
-            p_new[at] += now - before;
-        }
-    }
-    __syncthreads();
-    for (int r = (int)threadIdx.x; r < e.n_receivers; r += (int)blockDim.x) {
-        const int64_t at = (int64_t)e.receiver_cell[2 * r] * plane + e.receiver_cell[2 * r + 1];
-        e.recording[(int64_t)r * e.recording_stride + column] = p_new[at];
     }
 }
 
@@ -184,11 +197,14 @@ WaveTiling wave_tiling(const acfdtd_hip_wave_desc &d) {
     t.blocks_per_plane = ceil_div(t.lanes, t.threads);
     int chunk = d.chunk_x;
     if (chunk <= 0) {
-        // About 2048 workgroups (8 per CU), never marches shorter than 8 planes: the plane
-        // before a march is read twice, once per neighbouring march.
-        const int wanted = ceil_div(2048, t.blocks_per_plane);
+        // About 8192 workgroups of 4 wavefronts (32 per CU), never marches shorter than 4
+        // planes (the plane before a march is read by two marches). Measured at 256^3, walls,
+        // fp32 / fp64, microseconds per step by marching length 4, 8, 16, 32, 64 planes:
+        // 31.2, 32.1, 32.2, 46.7, 80.0 / 54.7, 58.1, 59.6, 56.2, 79.7: the short marches'
+        // occupancy outweighs their re-read planes.
+        const int wanted = ceil_div(8192, t.blocks_per_plane);
         chunk = ceil_div(d.nx, wanted > 0 ? wanted : 1);
-        if (chunk < 8) chunk = 8;
+        if (chunk < 4) chunk = 4;
     }
     t.chunk = chunk;
     t.nchunk = ceil_div(d.nx, chunk);
@@ -230,13 +246,14 @@ int validate_wave(const acfdtd_hip_wave_desc *d) {
 }
 
 template <typename real, int V>
-void launch_wave_step(const WaveArgs<real> &a, bool walls, const real *p, real *q, hipStream_t s) {
+void launch_wave_step(const WaveArgs<real> &a, bool walls, const real *p, real *q, int64_t step,
+                      int64_t column, hipStream_t s) {
     if (walls)
         hipLaunchKernelGGL((wave_step<real, V, true>), dim3(a.t.launched), dim3(a.t.threads), 0, s, a,
-                           p, q);
+                           p, q, step, column);
     else
         hipLaunchKernelGGL((wave_step<real, V, false>), dim3(a.t.launched), dim3(a.t.threads), 0, s,
-                           a, p, q);
+                           a, p, q, step, column);
 }
 
 template <typename real>
@@ -249,34 +266,29 @@ int wave_run(const acfdtd_hip_wave_desc &d, int64_t step_offset, int64_t n_steps
     a.nz = d.nz;
     a.courant2 = (real)d.courant2;
     a.wall_table = static_cast<const real *>(d.wall_table);
-    WaveEvents<real> e{};
-    e.nx = d.nx;
-    e.ny = d.ny;
-    e.nz = d.nz;
-    e.n_sources = d.n_sources;
-    e.n_receivers = d.n_receivers;
-    e.src = (real)d.src_coef;
-    e.source_cell = d.source_cell;
-    e.source_length = d.source_length;
-    e.source_signal = static_cast<const real *>(d.source_signal);
-    e.source_stride = d.source_stride;
-    e.receiver_cell = d.receiver_cell;
-    e.recording = static_cast<real *>(d.recording);
-    e.recording_stride = d.recording_stride;
+    a.n_sources = d.n_sources;
+    a.n_receivers = d.n_receivers;
+    a.src = (real)d.src_coef;
+    a.source_cell = d.source_cell;
+    a.source_length = d.source_length;
+    a.source_signal = static_cast<const real *>(d.source_signal);
+    a.source_stride = d.source_stride;
+    a.receiver_cell = d.receiver_cell;
+    a.recording = static_cast<real *>(d.recording);
+    a.recording_stride = d.recording_stride;
     const bool walls = d.wall_mask != 0;
     real *current = static_cast<real *>(d.p);
     real *previous = static_cast<real *>(d.p_prev);
     for (int64_t n = 0; n < n_steps; ++n) {
+        const int64_t step = step_offset + n, col = column + n;
         if (a.t.vec == 4) {
-            if constexpr (sizeof(real) == 4) launch_wave_step<real, 4>(a, walls, current, previous, s);
+            if constexpr (sizeof(real) == 4)
+                launch_wave_step<real, 4>(a, walls, current, previous, step, col, s);
         } else if (a.t.vec == 2) {
-            launch_wave_step<real, 2>(a, walls, current, previous, s);
+            launch_wave_step<real, 2>(a, walls, current, previous, step, col, s);
         } else {
-            launch_wave_step<real, 1>(a, walls, current, previous, s);
+            launch_wave_step<real, 1>(a, walls, current, previous, step, col, s);
         }
-        if (d.n_sources > 0 || d.n_receivers > 0)
-            hipLaunchKernelGGL(wave_events<real>, dim3(1), dim3(256), 0, s, e, previous,
-                               step_offset + n, column + n);
         real *newest = previous;
         previous = current;
         current = newest;

@@ -468,6 +468,76 @@ def large_room(size, steps, seed=5):
         "t20_max_rel_deviation_by_octave": {f"{int(b)}": float(v) for b, v in deviation.items()},
     }
 
+def timed_run(solver, steps: int, warmup: int) -> float:
+    """Milliseconds per step of ``solver.run(steps)`` after ``warmup`` steps, from HIP events
+    on the current stream."""
+    import torch
+
+    solver.run(warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    solver.run(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def scheme_variants(size: int, steps: int, warmup: int) -> dict:
+    """The throughput workload's grid, source and receivers with walls of absorption 0.15 and
+    neither sponge nor air (what the pressure-only scheme supports): the pressure-only scheme
+    (3 elements per cell-update) in fp32 and fp64 against the first-order scheme on the same
+    configuration."""
+    import torch
+
+    from ac_fdtd_amd import Grid
+    from ac_fdtd_amd.scheme import AcousticFDTD
+    from ac_fdtd_amd.sources import gaussian_derivative_pulse
+    from ac_fdtd_amd.wave import PressureOnlyFDTD
+
+    grid, config, source, receivers, width = workload(size, steps)
+    walls = {"walls": config["walls"]}
+    out = {"grid": [size] * 3, "steps": steps, "configuration": "walls alpha 0.15, no sponge, no air"}
+    cells = grid.n_cells
+    recordings = {}
+    for name, cls, dtype, kwargs in (
+        ("pressure_only_fp32", PressureOnlyFDTD, np.float32, {}),
+        ("pressure_only_fp64", PressureOnlyFDTD, np.float64, {}),
+        ("first_order_fp32", AcousticFDTD, np.float32, {"autotune": True}),
+    ):
+        solver = cls(grid, dtype=dtype, **walls, **kwargs)
+        solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt,
+                                                                   width))
+        for r in receivers:
+            solver.add_pressure_receiver(r)
+        ms = timed_run(solver, steps, warmup)
+        entry = {"gcells": round(cells / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 5)}
+        if cls is PressureOnlyFDTD:
+            itemsize = np.dtype(dtype).itemsize
+            entry["bytes_per_cell_update"] = 3 * itemsize
+            entry["hbm_fraction"] = round(3 * itemsize * cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        recordings[name] = solver.recorded_pressure.astype(np.float64)
+        out[name] = entry
+        del solver
+        torch.cuda.empty_cache()
+    # Beyond the 256 MB last-level cache: two fp32 fields of 512^3 are 1.07 GB.
+    big = Grid(dx=DX, nx=2 * size, ny=2 * size, nz=2 * size)
+    solver = PressureOnlyFDTD(big, dtype=np.float32, **walls)
+    solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt, width))
+    ms = timed_run(solver, max(steps // 4, 20), warmup)
+    out["pressure_only_fp32_double_size"] = {
+        "grid": [2 * size] * 3, "gcells": round(big.n_cells / (ms * 1e-3) / 1e9, 3),
+        "ms_per_step": round(ms, 5),
+        "hbm_fraction": round(12 * big.n_cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del solver
+    torch.cuda.empty_cache()
+    out["pressure_only_over_first_order_speed"] = round(
+        out["pressure_only_fp32"]["gcells"] / out["first_order_fp32"]["gcells"], 3)
+    out["pressure_only_fp32_vs_fp64_receiver_rel_l2"] = relative_l2(
+        recordings["pressure_only_fp32"], recordings["pressure_only_fp64"])
+    return out
+
+
 def stream_ceiling(gib: float = 1.0, copies: int = 20) -> float:
     """GB/s (read + write) of the library's 16-byte-per-lane copy over ``gib`` GiB, four times
     the 256 MB MALL: the HBM bandwidth a plain streaming kernel reaches here."""
@@ -576,13 +646,15 @@ def main() -> int:
     parser.add_argument("--large-steps", type=int, default=8000,
                         help="steps of the 512^3 room (0 skips it)")
     parser.add_argument("--batch-steps", type=int, default=3000)
+    parser.add_argument("--steps-variants", type=int, default=400,
+                        help="steps of each scheme-variant run (pressure-only, fourth order)")
     parser.add_argument("--accuracy-steps", type=int, default=4000,
                         help="steps of the fp32-vs-fp64 and energy-drift runs (any --steps)")
     parser.add_argument("--skip-check", action="store_true",
                         help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
         "--mode",
-        choices=("bench", "copy", "gradient", "batch"),
+        choices=("bench", "copy", "gradient", "batch", "variants"),
         default="bench",
         help="copy: five plain 512 MiB device-to-device copies, the known-size reference a "
         "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by; gradient: only the "
@@ -617,6 +689,9 @@ def main() -> int:
         return 0
     if args.mode == "gradient":
         print(json.dumps(gradient_cost(128, 2000)), flush=True)
+        return 0
+    if args.mode == "variants":
+        print(json.dumps(scheme_variants(args.size, args.steps_variants, args.warmup)), flush=True)
         return 0
     if args.mode == "batch":
         print(json.dumps(batched_rooms(args.batch_rooms, args.batch_steps, args.warmup,
@@ -704,6 +779,9 @@ def main() -> int:
             extra["batched_rooms_fp32"] = batched_rooms(args.batch_rooms, args.batch_steps,
                                                         args.warmup, check=not args.skip_check)
             log(f"batched: {extra['batched_rooms_fp32']}")
+        log("scheme variants")
+        extra["scheme_variants"] = scheme_variants(args.size, args.steps_variants, args.warmup)
+        log(f"variants: {extra['scheme_variants']}")
         log("gradient cost")
         extra["wall_gradient_cost_fp32"] = gradient_cost(128, 2000)
         log(f"gradient: {extra['wall_gradient_cost_fp32']}")
