@@ -227,6 +227,9 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
     lib.acfdtd_hip_energy_noalloc.restype = _c.c_int
     lib.acfdtd_hip_energy_noalloc.argtypes = [_c.POINTER(Desc), _ptr, _ptr]
+    lib.acfdtd_hip_fourth_run.restype = _c.c_int
+    lib.acfdtd_hip_fourth_run.argtypes = [_c.POINTER(Desc), _c.c_double, _c.c_double, _c.c_int64,
+                                          _c.c_int64, _c.c_int64, _ptr]
     lib.acfdtd_hip_wave_struct_size.restype = _c.c_size_t
     lib.acfdtd_hip_wave_run.restype = _c.c_int
     lib.acfdtd_hip_wave_run.argtypes = [_c.POINTER(WaveDesc), _c.c_int64, _c.c_int64, _c.c_int64,

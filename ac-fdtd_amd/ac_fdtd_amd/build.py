@@ -52,6 +52,7 @@ SOURCES = (
     "batch2_64.hip",
     "batch2_64_v1.hip",
     "wave.hip",
+    "fourth.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

@@ -531,6 +531,27 @@ def scheme_variants(size: int, steps: int, warmup: int) -> dict:
         "hbm_fraction": round(12 * big.n_cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     del solver
     torch.cuda.empty_cache()
+    # Fourth order in space: rigid walls (images), with the throughput workload's sponge and air.
+    from ac_fdtd_amd.fourth import FourthOrderFDTD
+
+    layered = {"absorbing_layer": config["absorbing_layer"],
+               "air_absorption": config["air_absorption"]}
+    for name, cls, kwargs in (("fourth_order_fp32", FourthOrderFDTD, {}),
+                              ("second_order_two_sweeps_fp32", AcousticFDTD,
+                               {"fused": False, "time_block": 1})):
+        solver = cls(grid, dtype=np.float32, **layered, **kwargs)
+        solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt,
+                                                                   width))
+        for r in receivers:
+            solver.add_pressure_receiver(r)
+        ms = timed_run(solver, steps, warmup)
+        n_relax = solver.coefficients.n_relax
+        out[name] = {"configuration": "rigid walls, 16-cell sponge, air 20 C / 50 %",
+                     "gcells": round(cells / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 5),
+                     "hbm_fraction": round(update_bytes(n_relax, 4) * cells / (ms * 1e-3) / 1e9
+                                           / HBM_PEAK_GBS, 4)}
+        del solver
+        torch.cuda.empty_cache()
     out["pressure_only_over_first_order_speed"] = round(
         out["pressure_only_fp32"]["gcells"] / out["first_order_fp32"]["gcells"], 3)
     out["pressure_only_fp32_vs_fp64_receiver_rel_l2"] = relative_l2(

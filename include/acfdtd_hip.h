@@ -361,6 +361,20 @@ size_t acfdtd_hip_wave_struct_size(void);
 int acfdtd_hip_wave_run(const acfdtd_hip_wave_desc *desc, int64_t step_offset, int64_t n_steps,
                         int64_t recording_column, void *stream);
 
+/* ------------------------------------------------------ fourth order in space --
+ * The staggered scheme with the fourth-order difference D u = c1 (u[i] - u[i-1]) -
+ * c2 (u[i+1] - u[i-2]) (c1 = 9/8, c2 = 1/24 for fourth order; 1, 0 gives the second-order
+ * sweeps' arithmetic) in the velocity and pressure sweeps: the reference's "4th-order-in-space
+ * variant as an option" (PLAN.md:17, :116), which it does not implement. Rigid walls by images
+ * (pressure even, normal velocity odd across each wall face); sponge, air absorption, sources
+ * and receivers as in acfdtd_hip_run. The descriptor is acfdtd_hip_desc with wall_mask 0, no
+ * flags and no tape (refused otherwise); swap buffers and time_block are ignored: each step is
+ * one velocity sweep and one pressure sweep in place, (12 + 2 n_relax) elements per
+ * cell-update. The caller picks dt within the fourth-order limit c dt / dx <= 1 / (sqrt(3)
+ * (c1 + c2)). No reference counterpart. */
+int acfdtd_hip_fourth_run(const acfdtd_hip_desc *desc, double c1, double c2, int64_t step_offset,
+                          int64_t n_steps, int64_t recording_column, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);
 

@@ -95,7 +95,7 @@ class OracleFDTD:
         air_absorption=None,
     ):
         if not 0.0 < courant <= 1.0:
-            raise ValueError(f"courant must be in (0, 1], got {courant}")
+            raise ValueError(f"the Courant number must lie in (0, 1]; {courant} was given")
         self.grid, self.medium, self.courant = grid, medium, courant
         self.dtype = np.dtype(dtype)
         self.walls = walls or WallAdmittances()
@@ -175,9 +175,21 @@ class OracleFDTD:
             self.step()
 
     # -- one step --------------------------------------------------------------------------
+    def _face_difference(self, axis):
+        """``p[i+1] - p[i]`` on the interior faces of ``axis``."""
+        return self.p[_hi(axis)] - self.p[_lo(axis)]
+
+    def _divergence(self):
+        """``((dvx + dvy) + dvz)`` over every cell, wall faces included (scheme.py:317-322)."""
+        v = self.velocity
+        div = v[0][_hi(0)] - v[0][_lo(0)]
+        div += v[1][_hi(1)] - v[1][_lo(1)]
+        div += v[2][_hi(2)] - v[2][_lo(2)]
+        return div
+
     def step(self):
         for a in range(3):
-            self.velocity[a][_inner(a)] -= self.k_vel * (self.p[_hi(a)] - self.p[_lo(a)])
+            self.velocity[a][_inner(a)] -= self.k_vel * self._face_difference(a)
         for a in range(3):
             for where, f in self.v_slabs[a]:
                 self.velocity[a][where] *= f
@@ -194,10 +206,7 @@ class OracleFDTD:
     def _pressure(self):
         flat = self.p.reshape(-1)
         before = flat[self.wall_idx].copy() if self.wall_idx.size else None
-        v = self.velocity
-        div = v[0][_hi(0)] - v[0][_lo(0)]
-        div += v[1][_hi(1)] - v[1][_lo(1)]
-        div += v[2][_hi(2)] - v[2][_lo(2)]
+        div = self._divergence()
         for state, decay, mean, gain in self.relax:
             target = div * gain
             offset = state - target
@@ -218,6 +227,6 @@ class OracleFDTD:
         for a in range(3):
             comp = self.velocity[a]
             kin += 0.5 * m.density * float(np.dot(comp.ravel(), comp.ravel()))
-            grad = (self.p[_hi(a)] - self.p[_lo(a)]) / dx
+            grad = self._face_difference(a) / dx
             coup += float(np.sum(comp[_inner(a)] * grad))
         return dx**3 * (pot + kin - 0.5 * self.dt * coup)

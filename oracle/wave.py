@@ -87,7 +87,7 @@ class OracleWaveFDTD:
     def __init__(self, grid: Grid, medium: Medium = AIR, courant: float = 1.0, dtype=np.float64,
                  walls: WallAdmittances | None = None):
         if not 0.0 < courant <= 1.0:
-            raise ValueError(f"courant must be in (0, 1], got {courant}")
+            raise ValueError(f"the Courant number must lie in (0, 1]; {courant} was given")
         self.grid, self.medium = grid, medium
         self.dtype = np.dtype(dtype)
         t = self.dtype.type
