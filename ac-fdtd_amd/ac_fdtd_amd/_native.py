@@ -24,6 +24,7 @@ __all__ = [
     "STAGED",
     "WaveDesc",
     "GRAD_AIR",
+    "IIRDesc",
     "GRAD_SLOTS",
     "GRAD_SPONGE",
     "GRAD_WALLS",
@@ -170,6 +171,23 @@ class WaveDesc(_c.Structure):
     ]
 
 
+MAX_BRANCHES = 8
+
+
+class IIRDesc(_c.Structure):
+    """Mirror of ``acfdtd_hip_iir_desc`` (frequency-dependent walls)."""
+
+    _fields_ = [
+        ("abi_version", _c.c_uint32),
+        ("struct_size", _c.c_uint32),
+        ("n_branches", _c.c_int32),
+        ("face_mask", _c.c_uint32),
+        ("velocity_scale", _c.c_double),
+        ("coef", _c.c_double * (6 * MAX_BRANCHES * 4)),
+        ("state", _ptr),
+    ]
+
+
 _library: ctypes.CDLL | None = None
 
 
@@ -227,6 +245,9 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
     lib.acfdtd_hip_energy_noalloc.restype = _c.c_int
     lib.acfdtd_hip_energy_noalloc.argtypes = [_c.POINTER(Desc), _ptr, _ptr]
+    lib.acfdtd_hip_iir_run.restype = _c.c_int
+    lib.acfdtd_hip_iir_run.argtypes = [_c.POINTER(Desc), _c.POINTER(IIRDesc), _c.c_int64,
+                                       _c.c_int64, _c.c_int64, _ptr]
     lib.acfdtd_hip_fourth_run.restype = _c.c_int
     lib.acfdtd_hip_fourth_run.argtypes = [_c.POINTER(Desc), _c.c_double, _c.c_double, _c.c_int64,
                                           _c.c_int64, _c.c_int64, _ptr]

@@ -53,6 +53,7 @@ SOURCES = (
     "batch2_64_v1.hip",
     "wave.hip",
     "fourth.hip",
+    "iir.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

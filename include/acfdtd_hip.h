@@ -375,6 +375,39 @@ int acfdtd_hip_wave_run(const acfdtd_hip_wave_desc *desc, int64_t step_offset, i
 int acfdtd_hip_fourth_run(const acfdtd_hip_desc *desc, double c1, double c2, int64_t step_offset,
                           int64_t n_steps, int64_t recording_column, void *stream);
 
+/* ------------------------------------------------- frequency-dependent (IIR) walls --
+ * Walls whose normalised admittance is a conductance plus mass-spring-damper branches, each
+ * branch a biquad from the bilinear transform: the "frequency-dependent impedance via an IIR
+ * filter fitted to octave-band absorption data" the reference plans (PLAN.md:21-24) and does
+ * not implement. The walls' instantaneous admittance sits in desc's wall table as for a real
+ * admittance; after each step every face cell of a frequency-dependent face (bit f of
+ * face_mask, faces in the order x-low, x-high, y-low, y-high, z-low, z-high) takes its step's
+ * mean pressure x = (f_u T) * 0.5, with T the boundary tape value p_tmp + p_old and f_u the
+ * cell's 1 / (1 + a), runs each branch b (TDF-II: y = b0 x + s1; s1 <- s2 - a1 y;
+ * s2 <- b2 x - a2 y; coefficients coef[f][b] = {b0, b2, a1, a2}) and writes
+ * (sum_b s1) * velocity_scale on its wall face (negated on low faces), which the next
+ * pressure sweep reads as the wall's outward velocity. No reference counterpart. */
+#define ACFDTD_HIP_MAX_BRANCHES 8
+typedef struct acfdtd_hip_iir_desc {
+    uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
+    uint32_t struct_size; /* sizeof(acfdtd_hip_iir_desc) */
+    int32_t n_branches;   /* 0 .. ACFDTD_HIP_MAX_BRANCHES, the same count on every face */
+    uint32_t face_mask;
+    double velocity_scale; /* 1 / (rho c), working precision */
+    double coef[6][ACFDTD_HIP_MAX_BRANCHES][4]; /* working precision */
+    /* Branch states (device, working precision, zero at the start): element e of a tape row
+     * (the face-plane layout of acfdtd_hip_desc::wall_tape) owns 2 * n_branches values at
+     * state[2 * n_branches * e]. */
+    void *state;
+} acfdtd_hip_iir_desc;
+
+/* Advance n_steps: per step the velocity sweep, the pressure sweep writing the tape, and the
+ * wall update. desc needs a wall tape of at least one row (used as scratch: the call writes
+ * row 0 every step), no sponge, and the wall table of the instantaneous admittances. */
+int acfdtd_hip_iir_run(const acfdtd_hip_desc *desc, const acfdtd_hip_iir_desc *iir,
+                       int64_t step_offset, int64_t n_steps, int64_t recording_column,
+                       void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);
 

@@ -217,6 +217,8 @@ class OracleFDTD:
             self.p += div * self.relax_scale
         self.p -= div * self.k_pres
         if before is not None:
+            # p_tmp + p_old on the wall cells, before the closed form (the boundary tape).
+            self.boundary_sum = flat[self.wall_idx] + before
             flat[self.wall_idx] = flat[self.wall_idx] * self.wall_fu - before * self.wall_fp
 
     # -- diagnostics ------------------------------------------------------------------------
