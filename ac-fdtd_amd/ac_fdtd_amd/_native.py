@@ -29,6 +29,7 @@ __all__ = [
     "GRAD_SPONGE",
     "GRAD_WALLS",
     "MAX_RELAXATION",
+    "PMLDesc",
     "check",
     "library",
 ]
@@ -188,6 +189,21 @@ class IIRDesc(_c.Structure):
     ]
 
 
+class PMLDesc(_c.Structure):
+    """Mirror of ``acfdtd_hip_pml_desc`` (split-field perfectly matched layer)."""
+
+    _fields_ = [
+        ("abi_version", _c.c_uint32),
+        ("struct_size", _c.c_uint32),
+        ("cell_a", _ptr * 3),
+        ("cell_b", _ptr * 3),
+        ("face_a", _ptr * 3),
+        ("face_b", _ptr * 3),
+        ("p_y", _ptr),
+        ("p_z", _ptr),
+    ]
+
+
 _library: ctypes.CDLL | None = None
 
 
@@ -245,6 +261,9 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_energy.argtypes = [_c.POINTER(Desc), _ptr, _ptr, _ptr]
     lib.acfdtd_hip_energy_noalloc.restype = _c.c_int
     lib.acfdtd_hip_energy_noalloc.argtypes = [_c.POINTER(Desc), _ptr, _ptr]
+    lib.acfdtd_hip_pml_run.restype = _c.c_int
+    lib.acfdtd_hip_pml_run.argtypes = [_c.POINTER(Desc), _c.POINTER(PMLDesc), _c.c_int64,
+                                       _c.c_int64, _c.c_int64, _ptr]
     lib.acfdtd_hip_iir_run.restype = _c.c_int
     lib.acfdtd_hip_iir_run.argtypes = [_c.POINTER(Desc), _c.POINTER(IIRDesc), _c.c_int64,
                                        _c.c_int64, _c.c_int64, _ptr]

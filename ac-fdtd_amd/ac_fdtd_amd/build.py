@@ -54,6 +54,7 @@ SOURCES = (
     "wave.hip",
     "fourth.hip",
     "iir.hip",
+    "pml.hip",
 )
 ARCH = os.environ.get("ACFDTD_HIP_ARCH", "gfx950")
 

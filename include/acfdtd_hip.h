@@ -408,6 +408,34 @@ int acfdtd_hip_iir_run(const acfdtd_hip_desc *desc, const acfdtd_hip_iir_desc *i
                        int64_t step_offset, int64_t n_steps, int64_t recording_column,
                        void *stream);
 
+/* ------------------------------------------------------ split-field perfectly matched layer --
+ * Berenger's split-field layer for the first-order scheme: the "true PML held in reserve"
+ * the reference plans (PLAN.md:21-24) and contrasts with its sponge (boundaries.py:42-46),
+ * not implemented there. Each field is damped only along its own axis, the pressure split in
+ * p = p_x + p_y + p_z inside the layer; the damping is centred in time, with per-axis
+ * coefficients A = (1 - sigma dt / 2) / (1 + sigma dt / 2) and B = 1 / (1 + sigma dt / 2) at
+ * cell centres and on faces. Per step:
+ *   interior faces: v_a = A_f v_a - B_f (vel (p[i] - p[i-1]));
+ *   cells with A_c = 1 on all three axes: p = p - ((Dx + Dy) + Dz) pres, Da = v_a[hi] - v_a[lo];
+ *   other cells: p_x = (p - p_y) - p_z; p_a = A_c,a p_a - B_c,a (pres Da) for each axis;
+ *                p = (p_x + p_y) + p_z (p_y and p_z stored);
+ *   then sources and receivers as acfdtd_hip_run. desc must have rigid walls (wall_mask 0),
+ * no relaxation states, no sponge, no flags and no tape; swap buffers and time_block are
+ * ignored (one velocity and one pressure sweep in place per step). No reference counterpart. */
+typedef struct acfdtd_hip_pml_desc {
+    uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
+    uint32_t struct_size; /* sizeof(acfdtd_hip_pml_desc) */
+    /* Device, working precision: cell_* have n_a entries, face_* n_a + 1. */
+    const void *cell_a[3], *cell_b[3];
+    const void *face_a[3], *face_b[3];
+    /* Split parts of the pressure in the layer (device, nx*ny*nz each, zero at the start). */
+    void *p_y, *p_z;
+} acfdtd_hip_pml_desc;
+
+int acfdtd_hip_pml_run(const acfdtd_hip_desc *desc, const acfdtd_hip_pml_desc *pml,
+                       int64_t step_offset, int64_t n_steps, int64_t recording_column,
+                       void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *acfdtd_hip_last_error(void);
 
