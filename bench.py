@@ -559,6 +559,70 @@ def scheme_variants(size: int, steps: int, warmup: int) -> dict:
     return out
 
 
+def boundary_variants(size: int, steps: int, warmup: int) -> dict:
+    """The two boundaries the reference plans beyond its sponge and real admittance: the
+    split-field PML's reflection against the sponge's (a 96^3 room lined with 12 cells on all
+    faces, three receivers from the centre to a corner, residual against a free domain), and
+    the fp32 throughput at ``size``^3 of the PML (16-cell layer) and of walls with
+    frequency-dependent admittance fitted to a porous absorber's octave table."""
+    import torch
+
+    from ac_fdtd_amd import AbsorbingLayer, Grid
+    from ac_fdtd_amd.iir_walls import IIRWallFDTD
+    from ac_fdtd_amd.impedance import FrequencyDependentWalls
+    from ac_fdtd_amd.pml import PMLFDTD
+    from ac_fdtd_amd.scheme import AcousticFDTD
+    from ac_fdtd_amd.sources import gaussian_derivative_pulse
+
+    out = {}
+    n, depth, run_steps, dx = 96, 12, 160, 0.01
+    pulse = np.exp(-((np.arange(run_steps) - 30.0) / 8.0) ** 2)
+    shift = run_steps * dx
+    points = [(0.5 * n, 0.5 * n, 0.5 * n), (depth + 4, 0.5 * n, 0.5 * n),
+              (depth + 4, depth + 4, depth + 4)]
+
+    def record(solver, offset):
+        solver.add_volume_source(tuple(c * dx + offset for c in (0.5 * n, 0.4 * n, 0.45 * n)),
+                                 pulse)
+        for point in points:
+            solver.add_pressure_receiver(tuple(c * dx + offset for c in point))
+        solver.run(run_steps)
+        return solver.recorded_pressure
+
+    side = n + 2 * run_steps
+    free = record(AcousticFDTD(Grid(dx=dx, nx=side, ny=side, nz=side), time_block=1), shift)
+    torch.cuda.empty_cache()
+    grid = Grid(dx=dx, nx=n, ny=n, nz=n)
+    layer = AbsorbingLayer(thickness=depth)
+    for name, solver in (("pml", PMLFDTD(grid, layer=layer)),
+                         ("sponge", AcousticFDTD(grid, absorbing_layer=layer))):
+        recorded = record(solver, 0.0)
+        out[f"{name}_residual_db_centre_face_corner"] = [
+            round(20 * math.log10(np.linalg.norm(recorded[r] - free[r]) / np.linalg.norm(free[r])), 1)
+            for r in range(3)]
+    del free
+    torch.cuda.empty_cache()
+
+    big, config, source, receivers, width = workload(size, steps)
+    porous = {125: 0.1, 250: 0.2, 500: 0.45, 1000: 0.7, 2000: 0.8, 4000: 0.85}
+    for name, solver in (
+        ("pml_fp32", PMLFDTD(big, dtype=np.float32, layer=AbsorbingLayer(thickness=SPONGE))),
+        ("iir_walls_fp32", IIRWallFDTD(big, dtype=np.float32,
+                                       walls=FrequencyDependentWalls.uniform(porous),
+                                       air_absorption=config["air_absorption"])),
+    ):
+        solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt,
+                                                                   width))
+        for r in receivers:
+            solver.add_pressure_receiver(r)
+        ms = timed_run(solver, steps, warmup)
+        out[name] = {"grid": [size] * 3, "gcells": round(big.n_cells / (ms * 1e-3) / 1e9, 3),
+                     "ms_per_step": round(ms, 5)}
+        del solver
+        torch.cuda.empty_cache()
+    return out
+
+
 def stream_ceiling(gib: float = 1.0, copies: int = 20) -> float:
     """GB/s (read + write) of the library's 16-byte-per-lane copy over ``gib`` GiB, four times
     the 256 MB MALL: the HBM bandwidth a plain streaming kernel reaches here."""
@@ -712,7 +776,9 @@ def main() -> int:
         print(json.dumps(gradient_cost(128, 2000)), flush=True)
         return 0
     if args.mode == "variants":
-        print(json.dumps(scheme_variants(args.size, args.steps_variants, args.warmup)), flush=True)
+        print(json.dumps({"schemes": scheme_variants(args.size, args.steps_variants, args.warmup),
+                          "boundaries": boundary_variants(args.size, args.steps_variants,
+                                                          args.warmup)}), flush=True)
         return 0
     if args.mode == "batch":
         print(json.dumps(batched_rooms(args.batch_rooms, args.batch_steps, args.warmup,
@@ -803,6 +869,9 @@ def main() -> int:
         log("scheme variants")
         extra["scheme_variants"] = scheme_variants(args.size, args.steps_variants, args.warmup)
         log(f"variants: {extra['scheme_variants']}")
+        log("boundary variants")
+        extra["boundary_variants"] = boundary_variants(args.size, args.steps_variants, args.warmup)
+        log(f"boundaries: {extra['boundary_variants']}")
         log("gradient cost")
         extra["wall_gradient_cost_fp32"] = gradient_cost(128, 2000)
         log(f"gradient: {extra['wall_gradient_cost_fp32']}")
