@@ -12,6 +12,8 @@
   the direct sound by then).
 * The large room: 512^3 with the sponge on two faces, 32 receivers, fp32 within 1e-5 of fp64
   over 1500 steps.
+* The reference's normal-incidence reflection and ISO duct attenuation checks, on the HIP
+  kernel (the CPU oracle passes them in ``tests/test_oracle.py``).
 
 The fp64 single-room kernel these compare against is bit-for-bit the oracle
 (``tests/test_gpu_sweeps.py``, ``tests/test_oracle_variants.py``).
@@ -115,3 +117,29 @@ def test_large_room_512_fp32_within_1e5_of_fp64():
     print(f"512^3 large room, {steps} steps, 32 receivers: fp32 relative L2 {error:.3e}",
           flush=True)
     assert error <= 1e-5
+
+
+def _hip_factory(grid, **kwargs):
+    from ac_fdtd_amd.scheme import AcousticFDTD
+
+    return AcousticFDTD(grid, **kwargs)
+
+
+@pytest.mark.parametrize(("xi", "expected"), [(0.0, 1.0), (0.25, 0.6), (0.5, 1 / 3), (1.0, 0.0)])
+def test_normal_incidence_reflection_on_the_hip_kernel(xi, expected):
+    """The reference's reflection check (``tests/test_boundaries.py:113-144``) on the HIP time
+    loop: a pulse down a one-cell duct off a wall of admittance xi reflects (1 - xi)/(1 + xi)
+    within 0.03."""
+    from test_oracle import _reflection
+
+    assert _reflection(_hip_factory, xi) == pytest.approx(expected, abs=0.03)
+
+
+@pytest.mark.parametrize("frequency", [2000.0, 4000.0])
+def test_iso_duct_attenuation_on_the_hip_kernel(frequency):
+    """The reference's ISO 9613-1 check (``tests/test_air.py:91-124``) on the HIP time loop: a
+    pulse down a 3000-cell duct, attenuation between two stations within 5 % of the standard."""
+    from test_oracle import _iso_duct
+
+    measured, expected = _iso_duct(_hip_factory, frequency)
+    assert measured == pytest.approx(expected, rel=0.05)
