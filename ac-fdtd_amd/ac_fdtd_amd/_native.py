@@ -169,6 +169,10 @@ class WaveDesc(_c.Structure):
         ("receiver_cell", _ptr),
         ("recording", _ptr),
         ("recording_stride", _c.c_int64),
+        ("time_block", _c.c_int32),
+        ("reserved", _c.c_int32),
+        ("p_swap", _ptr),
+        ("p_prev_swap", _ptr),
     ]
 
 

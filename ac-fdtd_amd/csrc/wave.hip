@@ -181,6 +181,277 @@ __global__ __launch_bounds__(512) void wave_step(const WaveArgs<real> a, const r
     }
 }
 
+// Two steps per sweep. A workgroup owns `rows` rows (or a segment of `seg` lane groups of them)
+// and also computes p^{n+1} on one halo row each side in y (and one halo lane group each side
+// in z for segments), so that the second step at every owned cell finds its neighbours' p^{n+1}
+// in LDS. Marching along x, iteration s
+//   A. computes p^{n+1} at plane s (the halo planes i0 - 1 and i1 included) from p^n of plane
+//      s in LDS (in-plane neighbours; a lane on the block's edge takes its outside neighbour
+//      from global memory) and the lane's own p^n of planes s - 1, s, s + 1 in registers;
+//   B. publishes it in LDS and issues the loads of p^{n-1} and the outside neighbour of plane
+//      s + 2 (used two iterations later);
+//   C. computes p^{n+2} at plane s - 1 from that plane's p^{n+1} in LDS and the lane's own
+//      p^{n+1} of planes s - 2, s - 1, s;
+//   D. publishes its p^n of plane s + 1 in LDS, issues the load of p^n of plane s + 3 (first
+//      used as an x neighbour two iterations later); one barrier.
+// Two slots for each field and one barrier per plane suffice: a slot written in iteration s
+// was last read in iteration s - 1, before the barrier that ends it, and is read next in
+// iteration s + 1, after the barrier that ends s. Each p^{n+2} and owned p^{n+1} is written
+// once; p^n and p^{n-1} are read once per workgroup (the halo rows and the two halo planes a
+// second time, mostly from L2): 4 elements per two cell-updates.
+struct Wave2Tiling {
+    int vec;
+    int groups;  // lane groups per row, nz / vec
+    int rows;    // owned rows per workgroup
+    int pitch;   // lanes per block row: a whole row, or a segment with a halo lane each side
+    int zhalo;   // 0 (whole rows) or 1 (segments)
+    int seg;     // owned lane groups per block row
+    int nseg;
+    int threads;  // (rows + 2) * pitch
+    int blocks_per_plane;
+    int chunk;
+    int nchunk;
+    int tiles;
+    int launched;
+};
+
+template <typename real>
+struct Wave2Args {
+    Wave2Tiling t;
+    WaveArgs<real> w;
+};
+
+template <typename real, int V>
+__device__ __forceinline__ void add_sources(const WaveArgs<real> &a, Cells<real, V> &out, int i,
+                                            int jk0, int64_t step) {
+    const real zero = real(0);
+    for (int s = 0; s < a.n_sources; ++s) {
+        if (a.source_cell[2 * s] != i) continue;
+        const int at = a.source_cell[2 * s + 1] - jk0;
+        if (at < 0 || at >= V) continue;
+        const int64_t length = a.source_length[s];
+        const real *signal = a.source_signal + (int64_t)s * a.source_stride;
+        const real now = step < length ? a.src * signal[step] : zero;
+        const real before = (step >= 1 && step - 1 < length) ? a.src * signal[step - 1] : zero;
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (m == at) out.c[m] = out.c[m] + (now - before);
+    }
+}
+
+// Receivers read back the lane's own cells just stored at `cells` (picking one of the lane's
+// registers by a run-time index would put them in scratch memory).
+template <typename real, int V>
+__device__ __forceinline__ void record(const WaveArgs<real> &a, const real *cells, int i, int jk0,
+                                       int64_t column) {
+    for (int r = 0; r < a.n_receivers; ++r) {
+        if (a.receiver_cell[2 * r] != i) continue;
+        const int at = a.receiver_cell[2 * r + 1] - jk0;
+        if (at < 0 || at >= V) continue;
+        a.recording[(int64_t)r * a.recording_stride + column] = cells[at];
+    }
+}
+
+template <typename real, int V, bool WALLS>
+__global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
+                                                  const real *__restrict__ p,
+                                                  const real *__restrict__ p_prev,
+                                                  real *__restrict__ q2, real *__restrict__ q1,
+                                                  const int64_t step, const int64_t column) {
+    __shared__ Cells<real, V> pn_slot[2][512];
+    __shared__ Cells<real, V> p1_slot[2][512];
+    const Wave2Tiling &t = args.t;
+    const WaveArgs<real> &a = args.w;
+    unsigned b = blockIdx.x;
+    b = (b & 7u) * ((unsigned)t.launched >> 3) + (b >> 3);
+    if (b >= (unsigned)t.tiles) return;  // the whole workgroup leaves together
+    const int blk = (int)(b % (unsigned)t.blocks_per_plane);
+    const int xc = (int)(b / (unsigned)t.blocks_per_plane);
+    const int rb = blk / t.nseg, sg = blk % t.nseg;
+    const int tid = threadIdx.x;
+    const int br = tid / t.pitch, pos = tid % t.pitch;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int j = rb * t.rows - 1 + br;
+    const int kg = sg * t.seg - t.zhalo + pos;
+    const bool valid = j >= 0 && j < ny && kg >= 0 && kg < t.groups;
+    const bool y_edge = br == 0 || br == t.rows + 1;
+    const bool z_edge = pos < t.zhalo || pos >= t.zhalo + t.seg;
+    // The block's corner lanes only publish p^n: nobody reads their p^{n+1}.
+    const bool computes = valid && !(y_edge && z_edge);
+    const bool owner = valid && !y_edge && !z_edge;
+    const int k0 = kg * V;
+    const int i0 = xc * t.chunk;
+    const int i1 = min(i0 + t.chunk, nx);
+    const int s0 = i0 > 0 ? i0 - 1 : 0;
+    const int last_pn = min(i1 + 1, nx - 1);  // the last plane of p^n the march reads
+    const int last_p1 = min(i1, nx - 1);      // the last plane of step 1
+    const int64_t plane = (int64_t)ny * nz;
+    const int jk0 = j * nz + k0;
+    const int64_t row = valid ? (int64_t)jk0 : 0;
+    const bool has_ylo = j > 0, has_yhi = j + 1 < ny;
+    const bool has_zlo = k0 > 0, has_zhi = k0 + V < nz;
+    // In-plane neighbours in LDS (the lane itself where there is none, or it is outside).
+    const int ys = has_ylo && br > 0 ? tid - t.pitch : tid;
+    const int yn = has_yhi && br < t.rows + 1 ? tid + t.pitch : tid;
+    const int zb = has_zlo && pos > 0 ? tid - 1 : tid;
+    const int za = has_zhi && pos < t.pitch - 1 ? tid + 1 : tid;
+    // A computing lane has at most one neighbour outside the block: a row (y edge) or one cell
+    // (z edge of a segment).
+    const bool out_south = computes && br == 0 && has_ylo;
+    const bool out_north = computes && br == t.rows + 1 && has_yhi;
+    const bool out_before = computes && pos == 0 && t.zhalo && has_zlo;
+    const bool out_after = computes && pos == t.pitch - 1 && t.zhalo && has_zhi;
+    // Loaded as the whole neighbouring row or lane group (aligned), the lane's own otherwise.
+    const int64_t out_at = out_south   ? row - nz
+                           : out_north ? row + nz
+                           : out_before ? row - V
+                           : out_after  ? row + V
+                                        : row;
+    const int code_y = axis_class(j, ny) << 2;
+    const real zero = real(0), lam2 = a.courant2;
+    // The wall table in LDS and the lane's factors on planes inside the x walls in registers:
+    // a global load in the march would make every wait for it also wait for the loads issued
+    // ahead of it (loads complete in order).
+    __shared__ real walls_lds[WALLS ? 128 : 1];
+    if (WALLS)
+        for (int e = tid; e < 128; e += t.threads) walls_lds[e] = a.wall_table[e];
+    real f_in[V], keep_in[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const int code = code_y | (axis_class(k0 + m, nz) << 4);
+        f_in[m] = WALLS && valid ? a.wall_table[code] : real(1);
+        keep_in[m] = WALLS && valid ? a.wall_table[64 + code] : real(1);
+    }
+    // One cell-update at plane x; the table is read (from LDS) only on the two x-wall planes.
+    auto cell = [&](int x, int m, real c, real old, real lap) {
+        if (!WALLS) return (c * real(2) - old) + lam2 * lap;
+        real f = f_in[m], keep = keep_in[m];
+        if (x == 0 || x == nx - 1) {
+            const int code = axis_class(x, nx) | code_y | (axis_class(k0 + m, nz) << 4);
+            f = walls_lds[code];
+            keep = walls_lds[64 + code];
+        }
+        return ((c * real(2) - keep * old) + lam2 * lap) * f;
+    };
+
+    auto at_plane = [&](int x) { return (int64_t)min(x, nx - 1) * plane; };
+
+    // Whether a source or receiver lies in this lane's column of cells (any plane).
+    bool mine = false;
+    if (valid) {
+        for (int e = 0; e < a.n_sources; ++e) {
+            const int at = a.source_cell[2 * e + 1] - jk0;
+            mine = mine || (at >= 0 && at < V);
+        }
+        for (int e = 0; e < a.n_receivers; ++e) {
+            const int at = a.receiver_cell[2 * e + 1] - jk0;
+            mine = mine || (at >= 0 && at < V);
+        }
+    }
+
+    // Register rings indexed by u = s - s0: p^n of plane s0 + u in pn[u mod 4] (planes s - 1 ..
+    // s + 2 live in iteration s), p^{n-1} and the outside neighbour of plane s0 + u in
+    // old[u mod 2] and out[u mod 2]. The march is unrolled by four so every index is a constant
+    // and a load in flight is never copied between registers (a copy would wait for it). Every
+    // lane issues every load (clamped to the grid), so the waits count the same on all paths.
+    Cells<real, V> pn[4], old[2], out[2];
+    pn[3] = load_cells<real, V>(p, at_plane(s0 > 0 ? s0 - 1 : 0) + row);
+    pn[0] = load_cells<real, V>(p, at_plane(s0) + row);
+    pn[1] = load_cells<real, V>(p, at_plane(s0 + 1) + row);
+    pn[2] = load_cells<real, V>(p, at_plane(s0 + 2) + row);
+    old[0] = load_cells<real, V>(p_prev, at_plane(s0) + row);
+    old[1] = load_cells<real, V>(p_prev, at_plane(s0 + 1) + row);
+    out[0] = load_cells<real, V>(p, at_plane(s0) + out_at);
+    out[1] = load_cells<real, V>(p, at_plane(s0 + 1) + out_at);
+    if (valid) pn_slot[0][tid] = pn[0];
+    __syncthreads();
+
+    Cells<real, V> p1_lo{}, p1_mid{};  // the lane's p^{n+1} of planes s - 2, s - 1
+    for (int u0 = 0;; u0 += 4) {
+#pragma unroll
+        for (int ph = 0; ph < 4; ++ph) {
+            const int s = s0 + u0 + ph;
+            if (s > i1) return;  // uniform: the whole workgroup leaves after the same barrier
+            const int cur = ph & 1;
+            const Cells<real, V> &pn_m1 = pn[(ph + 3) & 3];
+            const Cells<real, V> &pn_0 = pn[ph];
+            const Cells<real, V> &pn_p1 = pn[(ph + 1) & 3];
+            Cells<real, V> p1{};
+            if (s < nx && computes) {
+                // A: step 1 at plane s.
+                const int64_t base = (int64_t)s * plane;
+                const bool has_xhi = s + 1 < nx;
+                const Cells<real, V> &o = out[ph & 1];
+                const Cells<real, V> south = out_south ? o : pn_slot[cur][ys];
+                const Cells<real, V> north = out_north ? o : pn_slot[cur][yn];
+                const real z_before = out_before ? o.c[V - 1] : pn_slot[cur][zb].c[V - 1];
+                const real z_after = out_after ? o.c[0] : pn_slot[cur][za].c[0];
+#pragma unroll
+                for (int m = 0; m < V; ++m) {
+                    const real c = pn_0.c[m];
+                    const real fx_hi = has_xhi ? pn_p1.c[m] - c : zero;
+                    const real fx_lo = s > 0 ? c - pn_m1.c[m] : zero;
+                    const real fy_hi = has_yhi ? north.c[m] - c : zero;
+                    const real fy_lo = has_ylo ? c - south.c[m] : zero;
+                    const real next_z = m + 1 < V ? pn_0.c[m + 1] : z_after;
+                    const real prev_z = m > 0 ? pn_0.c[m - 1] : z_before;
+                    const real fz_hi = (m + 1 < V || has_zhi) ? next_z - c : zero;
+                    const real fz_lo = (m > 0 || has_zlo) ? c - prev_z : zero;
+                    const real lap = ((fx_hi - fx_lo) + (fy_hi - fy_lo)) + (fz_hi - fz_lo);
+                    p1.c[m] = cell(s, m, c, old[ph & 1].c[m], lap);
+                }
+                // Halo lanes add the sources too: their p^{n+1} feeds the owners' second step.
+                if (mine) add_sources<real, V>(a, p1, s, jk0, step);
+                if (owner && s >= i0 && s < i1) {
+                    *reinterpret_cast<Cells<real, V> *>(q1 + base + row) = p1;
+                    if (mine) record<real, V>(a, q1 + base + row, s, jk0, column);
+                }
+                // B: publish it.
+                p1_slot[cur][tid] = p1;
+            }
+            // B: p^{n-1} and the outside neighbour of plane s + 2, into the slots just freed.
+            old[ph & 1] = load_cells<real, V>(p_prev, at_plane(s + 2) + row);
+            out[ph & 1] = load_cells<real, V>(p, at_plane(s + 2) + out_at);
+            const int r = s - 1;
+            if (r >= i0 && owner) {
+                // C: step 2 at plane r.
+                const int prev = cur ^ 1;
+                const Cells<real, V> south = p1_slot[prev][ys];
+                const Cells<real, V> north = p1_slot[prev][yn];
+                const real z_before = p1_slot[prev][zb].c[V - 1];
+                const real z_after = p1_slot[prev][za].c[0];
+                const bool has_xhi = s < nx;
+                Cells<real, V> next;
+#pragma unroll
+                for (int m = 0; m < V; ++m) {
+                    const real c = p1_mid.c[m];
+                    const real fx_hi = has_xhi ? p1.c[m] - c : zero;
+                    const real fx_lo = r > 0 ? c - p1_lo.c[m] : zero;
+                    const real fy_hi = has_yhi ? north.c[m] - c : zero;
+                    const real fy_lo = has_ylo ? c - south.c[m] : zero;
+                    const real next_z = m + 1 < V ? p1_mid.c[m + 1] : z_after;
+                    const real prev_z = m > 0 ? p1_mid.c[m - 1] : z_before;
+                    const real fz_hi = (m + 1 < V || has_zhi) ? next_z - c : zero;
+                    const real fz_lo = (m > 0 || has_zlo) ? c - prev_z : zero;
+                    const real lap = ((fx_hi - fx_lo) + (fy_hi - fy_lo)) + (fz_hi - fz_lo);
+                    next.c[m] = cell(r, m, c, pn_m1.c[m], lap);
+                }
+                if (mine) add_sources<real, V>(a, next, r, jk0, step + 1);
+                real *const cells = q2 + (int64_t)r * plane + row;
+                *reinterpret_cast<Cells<real, V> *>(cells) = next;
+                if (mine) record<real, V>(a, cells, r, jk0, column + 1);
+            }
+            // D: p^n of plane s + 1 for the next step 1; p^n of plane s + 3 into the slot of
+            // plane s - 1, whose last use was step 2 above.
+            if (valid && s + 1 < nx) pn_slot[cur ^ 1][tid] = pn_p1;
+            pn[(ph + 3) & 3] = load_cells<real, V>(p, at_plane(s + 3) + row);
+            __syncthreads();
+            p1_lo = p1_mid;
+            p1_mid = p1;
+        }
+    }
+}
+
 int wave_vec(const acfdtd_hip_wave_desc &d) {
     const int widest = d.real_bytes == 4 ? 4 : 2;
     for (int v = widest; v > 1; v /= 2)
@@ -205,6 +476,46 @@ WaveTiling wave_tiling(const acfdtd_hip_wave_desc &d) {
         const int wanted = ceil_div(8192, t.blocks_per_plane);
         chunk = ceil_div(d.nx, wanted > 0 ? wanted : 1);
         if (chunk < 4) chunk = 4;
+    }
+    t.chunk = chunk;
+    t.nchunk = ceil_div(d.nx, chunk);
+    t.tiles = t.blocks_per_plane * t.nchunk;
+    t.launched = ceil_div(t.tiles, 8) * 8;
+    return t;
+}
+
+Wave2Tiling wave2_tiling(const acfdtd_hip_wave_desc &d) {
+    Wave2Tiling t{};
+    t.vec = wave_vec(d);
+    t.groups = d.nz / t.vec;
+    const int want = d.tile_j > 0 ? 2 * d.tile_j : 6;
+    if ((want + 2) * t.groups <= 512) {
+        // Whole rows; by default as many as fit 512 lanes (no more than the grid has).
+        t.pitch = t.seg = t.groups;
+        t.zhalo = 0;
+        t.nseg = 1;
+        t.rows = want;
+        if (d.tile_j <= 0) {
+            const int fit = 512 / t.groups - 2;
+            if (fit > t.rows) t.rows = fit;
+            if (t.rows > d.ny) t.rows = d.ny;
+        }
+    } else {
+        // Row segments of equal width with one halo lane group each side.
+        const int widest = 512 / (want + 2) - 2;
+        t.nseg = ceil_div(t.groups, widest);
+        t.seg = ceil_div(t.groups, t.nseg);
+        t.pitch = t.seg + 2;
+        t.zhalo = 1;
+        t.rows = d.tile_j > 0 ? want : 512 / t.pitch - 2;
+    }
+    t.threads = (t.rows + 2) * t.pitch;
+    t.blocks_per_plane = ceil_div(d.ny, t.rows) * t.nseg;
+    int chunk = d.chunk_x;
+    if (chunk <= 0) {
+        const int wanted = ceil_div(2048, t.blocks_per_plane);
+        chunk = ceil_div(d.nx, wanted > 0 ? wanted : 1);
+        if (chunk < 8) chunk = 8;
     }
     t.chunk = chunk;
     t.nchunk = ceil_div(d.nx, chunk);
@@ -242,6 +553,17 @@ int validate_wave(const acfdtd_hip_wave_desc *d) {
     const uintptr_t align = (uintptr_t)wave_vec(*d) * d->real_bytes;
     if (reinterpret_cast<uintptr_t>(d->p) % align || reinterpret_cast<uintptr_t>(d->p_prev) % align)
         return fail(ACFDTD_HIP_EINVAL, "fields must be aligned to a lane's cells");
+    if (d->time_block < 0 || d->time_block > 2) return fail(ACFDTD_HIP_EINVAL, "time_block must be 0, 1 or 2");
+    if (d->time_block == 2) {
+        const void *f[4] = {d->p, d->p_prev, d->p_swap, d->p_prev_swap};
+        if (!f[2] || !f[3]) return fail(ACFDTD_HIP_EINVAL, "time_block 2 needs both swap buffers");
+        for (int x = 0; x < 4; ++x)
+            for (int y = x + 1; y < 4; ++y)
+                if (f[x] == f[y])
+                    return fail(ACFDTD_HIP_EINVAL, "p, p_prev and the swap buffers must be distinct");
+        if (reinterpret_cast<uintptr_t>(f[2]) % align || reinterpret_cast<uintptr_t>(f[3]) % align)
+            return fail(ACFDTD_HIP_EINVAL, "swap buffers must be aligned to a lane's cells");
+    }
     return ACFDTD_HIP_OK;
 }
 
@@ -254,6 +576,17 @@ void launch_wave_step(const WaveArgs<real> &a, bool walls, const real *p, real *
     else
         hipLaunchKernelGGL((wave_step<real, V, false>), dim3(a.t.launched), dim3(a.t.threads), 0, s,
                            a, p, q, step, column);
+}
+
+template <typename real, int V>
+void launch_wave_step2(const Wave2Args<real> &a, bool walls, const real *p, const real *p_prev,
+                       real *q2, real *q1, int64_t step, int64_t column, hipStream_t s) {
+    if (walls)
+        hipLaunchKernelGGL((wave_step2<real, V, true>), dim3(a.t.launched), dim3(a.t.threads), 0, s,
+                           a, p, p_prev, q2, q1, step, column);
+    else
+        hipLaunchKernelGGL((wave_step2<real, V, false>), dim3(a.t.launched), dim3(a.t.threads), 0,
+                           s, a, p, p_prev, q2, q1, step, column);
 }
 
 template <typename real>
@@ -279,7 +612,31 @@ int wave_run(const acfdtd_hip_wave_desc &d, int64_t step_offset, int64_t n_steps
     const bool walls = d.wall_mask != 0;
     real *current = static_cast<real *>(d.p);
     real *previous = static_cast<real *>(d.p_prev);
-    for (int64_t n = 0; n < n_steps; ++n) {
+    int64_t n = 0;
+    if (d.time_block == 2) {
+        // Pairs of two-step sweeps, (p, p_prev) -> (p_swap, p_prev_swap) -> (p, p_prev): after
+        // each four steps the newest field is back in p, as after four single steps.
+        Wave2Args<real> a2{};
+        a2.t = wave2_tiling(d);
+        a2.w = a;
+        real *swap[2] = {static_cast<real *>(d.p_swap), static_cast<real *>(d.p_prev_swap)};
+        real *home[2] = {current, previous};
+        const int64_t paired = n_steps / 4 * 4;
+        for (; n < paired; n += 2) {
+            real *const *from = (n % 4 == 0) ? home : swap;
+            real *const *to = (n % 4 == 0) ? swap : home;
+            const int64_t step = step_offset + n, col = column + n;
+            if (a2.t.vec == 4) {
+                if constexpr (sizeof(real) == 4)
+                    launch_wave_step2<real, 4>(a2, walls, from[0], from[1], to[0], to[1], step, col, s);
+            } else if (a2.t.vec == 2) {
+                launch_wave_step2<real, 2>(a2, walls, from[0], from[1], to[0], to[1], step, col, s);
+            } else {
+                launch_wave_step2<real, 1>(a2, walls, from[0], from[1], to[0], to[1], step, col, s);
+            }
+        }
+    }
+    for (; n < n_steps; ++n) {
         const int64_t step = step_offset + n, col = column + n;
         if (a.t.vec == 4) {
             if constexpr (sizeof(real) == 4)

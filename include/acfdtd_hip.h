@@ -353,6 +353,17 @@ typedef struct acfdtd_hip_wave_desc {
     const int32_t *receiver_cell;
     void *recording;
     int64_t recording_stride;
+    /* Two steps per sweep (time_block = 2, both swap buffers given, distinct, same shape):
+     * a sweep reads (p^n, p^{n-1}) from one pair and writes (p^{n+2}, p^{n+1}) to the other,
+     * 4 elements per two cell-updates instead of 6; sweeps run in pairs, p / p_prev ->
+     * p_swap / p_prev_swap -> p / p_prev, for the leading multiple of four steps, the rest one
+     * step per sweep. The swap pair is scratch; the roles of p and p_prev on return follow the
+     * rule above, and the results are the same bits as one step per sweep. A workgroup owns
+     * whole rows, or equal row segments when a row is too long, plus one halo row each side
+     * (tile_j > 0: 2 * tile_j owned rows; 0: as many as fill 512 lanes). */
+    int32_t time_block; /* 0 or 1: one step per sweep; 2: two */
+    int32_t reserved;
+    void *p_swap, *p_prev_swap;
 } acfdtd_hip_wave_desc;
 
 size_t acfdtd_hip_wave_struct_size(void);

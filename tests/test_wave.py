@@ -126,6 +126,10 @@ def test_the_wave_descriptor_is_checked_before_any_launch():
         ({"wall_mask": 3}, "wall table"),
         ({"n_sources": 1}, "source arrays"),
         ({"tile_j": 9}, "tile_j"),
+        ({"time_block": 3}, "time_block"),
+        ({"time_block": 2, "p_swap": 12288}, "both swap buffers"),
+        ({"time_block": 2, "p_swap": 12288, "p_prev_swap": 4096}, "distinct"),
+        ({"time_block": 2, "p_swap": 12288, "p_prev_swap": 16388}, "aligned"),
     ):
         d = desc(**overrides)
         code = -2 if message == "bytes" else -1
@@ -167,28 +171,56 @@ def _assert_same(solver, oracle):
 
 @pytest.mark.gpu
 @pytest.mark.usefixtures("require_gpu")
+@pytest.mark.parametrize("time_block", [1, 2])
 @pytest.mark.parametrize("walls", [WallAdmittances(), WALLS], ids=["rigid", "walls"])
 @pytest.mark.parametrize("nz", [16, 14, 15])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["fp32", "fp64"])
-def test_hip_wave_is_bitwise_the_oracle(dtype, nz, walls):
+def test_hip_wave_is_bitwise_the_oracle(dtype, nz, walls, time_block):
+    """Odd and even step counts swap the fields' roles; with two steps per sweep, 37 steps are
+    nine pairs of sweeps and one single step, 40 steps ten pairs."""
     grid = Grid(dx=0.03, nx=20, ny=17, nz=nz)
-    solver, oracle = _pair(grid, dtype, walls)
-    for n_steps in (37, 1, 40):  # odd and even counts swap the fields' roles
+    solver, oracle = _pair(grid, dtype, walls, time_block=time_block)
+    for n_steps in (37, 1, 40, 2, 3):
         solver.run(n_steps)
         oracle.run(n_steps)
+        _assert_same(solver, oracle)
+
+
+@pytest.mark.gpu
+@pytest.mark.usefixtures("require_gpu")
+@pytest.mark.parametrize("time_block", [1, 2])
+@pytest.mark.parametrize("tile_j,chunk_x", [(1, 1), (2, 5), (8, 3), (4, 0)])
+def test_hip_wave_tiles_do_not_change_the_answer(tile_j, chunk_x, time_block):
+    """Several workgroups per plane and several marches along x, so workgroup edges fall inside
+    the grid (rows of 64 lane groups of 4 cells: one step per sweep splits them between
+    wavefronts; two steps per sweep own 2 or 4 whole rows with tile_j 1 or 2, and three row
+    segments of 22 lane groups with tile_j 4 or 8)."""
+    grid = Grid(dx=0.02, nx=23, ny=19, nz=256)
+    solver, oracle = _pair(grid, np.float32, WALLS, seed=5, tile_j=tile_j, chunk_x=chunk_x,
+                           time_block=time_block)
+    solver.run(14)
+    oracle.run(14)
     _assert_same(solver, oracle)
 
 
 @pytest.mark.gpu
 @pytest.mark.usefixtures("require_gpu")
-@pytest.mark.parametrize("tile_j,chunk_x", [(1, 1), (2, 5), (8, 3), (4, 0)])
-def test_hip_wave_tiles_do_not_change_the_answer(tile_j, chunk_x):
-    """Several workgroups per plane and several marches along x, so workgroup edges fall inside
-    the grid (rows of 64 lane groups of 4 cells, split between wavefronts)."""
-    grid = Grid(dx=0.02, nx=23, ny=19, nz=256)
-    solver, oracle = _pair(grid, np.float32, WALLS, seed=5, tile_j=tile_j, chunk_x=chunk_x)
-    solver.run(12)
-    oracle.run(12)
+@pytest.mark.parametrize("shape,dtype,tile_j,chunk_x", [
+    ((10, 5, 130), np.float64, 8, 4),   # 65 lane groups of 2: three segments, 16 rows
+    ((7, 11, 61), np.float32, 8, 3),    # 61 single cells: three segments
+    ((9, 7, 12), np.float32, 0, 0),     # 3 groups of 4, all rows in one workgroup
+    ((5, 1, 4), np.float32, 0, 0),      # one row of one group
+    ((1, 6, 7), np.float64, 1, 1),      # one plane
+    ((17, 40, 16), np.float64, 0, 5),   # 8 groups: 62 rows per workgroup, ragged last block
+], ids=["seg-fp64", "seg-odd", "one-block", "one-row", "one-plane", "ragged"])
+def test_hip_wave_two_step_sweep_shapes(shape, dtype, tile_j, chunk_x):
+    """Two steps per sweep on whole rows and row segments, single rows and planes."""
+    grid = Grid(dx=0.03, nx=shape[0], ny=shape[1], nz=shape[2])
+    solver, oracle = _pair(grid, dtype, WALLS, seed=7, tile_j=tile_j, chunk_x=chunk_x,
+                           time_block=2)
+    for n_steps in (8, 6):
+        solver.run(n_steps)
+        oracle.run(n_steps)
     _assert_same(solver, oracle)
 
 
