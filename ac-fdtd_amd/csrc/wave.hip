@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "acfdtd_hip.h"
 #include "sweeps.hpp"
@@ -252,14 +253,21 @@ __device__ __forceinline__ void record(const WaveArgs<real> &a, const real *cell
     }
 }
 
+// LDS layout of one plane of a field: lane groups at index tid + pitch (a block row of room
+// above and below for the outside rows), then the outside cells of segment edges by block row.
+constexpr int kWaveRoom = 768;  // (rows + 4) * pitch <= 512 + 2 * pitch, pitch <= 128
+constexpr int kWaveEdges = 256; // rows + 2 <= 170 when segments are used
+
 template <typename real, int V, bool WALLS>
 __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
                                                   const real *__restrict__ p,
                                                   const real *__restrict__ p_prev,
                                                   real *__restrict__ q2, real *__restrict__ q1,
                                                   const int64_t step, const int64_t column) {
-    __shared__ Cells<real, V> pn_slot[2][512];
-    __shared__ Cells<real, V> p1_slot[2][512];
+    constexpr int kCells = kWaveRoom * V;
+    __shared__ alignas(16) real pn_lds[2][kCells + 2 * kWaveEdges];
+    __shared__ alignas(16) real p1_lds[2][kCells];
+    __shared__ real walls_lds[WALLS ? 128 : 1];
     const Wave2Tiling &t = args.t;
     const WaveArgs<real> &a = args.w;
     unsigned b = blockIdx.x;
@@ -283,20 +291,13 @@ __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
     const int i0 = xc * t.chunk;
     const int i1 = min(i0 + t.chunk, nx);
     const int s0 = i0 > 0 ? i0 - 1 : 0;
-    const int last_pn = min(i1 + 1, nx - 1);  // the last plane of p^n the march reads
-    const int last_p1 = min(i1, nx - 1);      // the last plane of step 1
     const int64_t plane = (int64_t)ny * nz;
     const int jk0 = j * nz + k0;
     const int64_t row = valid ? (int64_t)jk0 : 0;
     const bool has_ylo = j > 0, has_yhi = j + 1 < ny;
     const bool has_zlo = k0 > 0, has_zhi = k0 + V < nz;
-    // In-plane neighbours in LDS (the lane itself where there is none, or it is outside).
-    const int ys = has_ylo && br > 0 ? tid - t.pitch : tid;
-    const int yn = has_yhi && br < t.rows + 1 ? tid + t.pitch : tid;
-    const int zb = has_zlo && pos > 0 ? tid - 1 : tid;
-    const int za = has_zhi && pos < t.pitch - 1 ? tid + 1 : tid;
     // A computing lane has at most one neighbour outside the block: a row (y edge) or one cell
-    // (z edge of a segment).
+    // (z edge of a segment). It loads it and publishes it in LDS with its own p^n.
     const bool out_south = computes && br == 0 && has_ylo;
     const bool out_north = computes && br == t.rows + 1 && has_yhi;
     const bool out_before = computes && pos == 0 && t.zhalo && has_zlo;
@@ -307,12 +308,19 @@ __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
                            : out_before ? row - V
                            : out_after  ? row + V
                                         : row;
+    // Neighbour addresses in a plane's LDS (in elements). A missing neighbour (a wall) is the
+    // cell itself, so its difference is c - c = +0, the wall's zero flux, with no select.
+    const int own = (tid + t.pitch) * V;
+    const int a_ylo = has_ylo ? own - t.pitch * V : own;
+    const int a_yhi = has_yhi ? own + t.pitch * V : own;
+    const int a_zlo = !has_zlo ? own : pos > 0 ? own - 1 : kCells + br;
+    const int a_zhi = !has_zhi ? own + V - 1 : pos < t.pitch - 1 ? own + V : kCells + kWaveEdges + br;
+    const int a_out = out_south ? a_ylo : out_north ? a_yhi : out_before ? a_zlo : a_zhi;
     const int code_y = axis_class(j, ny) << 2;
     const real zero = real(0), lam2 = a.courant2;
     // The wall table in LDS and the lane's factors on planes inside the x walls in registers:
     // a global load in the march would make every wait for it also wait for the loads issued
     // ahead of it (loads complete in order).
-    __shared__ real walls_lds[WALLS ? 128 : 1];
     if (WALLS)
         for (int e = tid; e < 128; e += t.threads) walls_lds[e] = a.wall_table[e];
     real f_in[V], keep_in[V];
@@ -322,16 +330,50 @@ __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
         f_in[m] = WALLS && valid ? a.wall_table[code] : real(1);
         keep_in[m] = WALLS && valid ? a.wall_table[64 + code] : real(1);
     }
-    // One cell-update at plane x; the table is read (from LDS) only on the two x-wall planes.
-    auto cell = [&](int x, int m, real c, real old, real lap) {
-        if (!WALLS) return (c * real(2) - old) + lam2 * lap;
-        real f = f_in[m], keep = keep_in[m];
-        if (x == 0 || x == nx - 1) {
-            const int code = axis_class(x, nx) | code_y | (axis_class(k0 + m, nz) << 4);
-            f = walls_lds[code];
-            keep = walls_lds[64 + code];
+
+    // One step at plane x for the lane's V cells: `mid` is the field there, `lo` and `hi` at
+    // planes x - 1 and x + 1, `lds` the plane in LDS. XIN: x is inside the x walls.
+    auto advance = [&](auto xin, const Cells<real, V> &lo, const Cells<real, V> &mid,
+                       const Cells<real, V> &hi, const real *lds, const Cells<real, V> &old,
+                       int x) {
+        constexpr bool XIN = decltype(xin)::value;
+        const Cells<real, V> south = *reinterpret_cast<const Cells<real, V> *>(lds + a_ylo);
+        const Cells<real, V> north = *reinterpret_cast<const Cells<real, V> *>(lds + a_yhi);
+        const real z_before = lds[a_zlo];
+        const real z_after = lds[a_zhi];
+        Cells<real, V> res;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const real c = mid.c[m];
+            const real fx_hi = (XIN || x + 1 < nx) ? hi.c[m] - c : zero;
+            const real fx_lo = (XIN || x > 0) ? c - lo.c[m] : zero;
+            const real fy_hi = north.c[m] - c;
+            const real fy_lo = c - south.c[m];
+            const real fz_hi = (m + 1 < V ? mid.c[m + 1] : z_after) - c;
+            const real fz_lo = c - (m > 0 ? mid.c[m - 1] : z_before);
+            const real lap = ((fx_hi - fx_lo) + (fy_hi - fy_lo)) + (fz_hi - fz_lo);
+            if (!WALLS) {
+                res.c[m] = (c * real(2) - old.c[m]) + lam2 * lap;
+            } else {
+                real f = f_in[m], keep = keep_in[m];
+                if (!XIN) {
+                    const int code = axis_class(x, nx) | code_y | (axis_class(k0 + m, nz) << 4);
+                    f = walls_lds[code];
+                    keep = walls_lds[64 + code];
+                }
+                res.c[m] = ((c * real(2) - keep * old.c[m]) + lam2 * lap) * f;
+            }
         }
-        return ((c * real(2) - keep * old) + lam2 * lap) * f;
+        return res;
+    };
+    auto publish_pn = [&](real *lds, const Cells<real, V> &mine_pn, const Cells<real, V> &o) {
+        if (valid) *reinterpret_cast<Cells<real, V> *>(lds + own) = mine_pn;
+        if (out_south || out_north)
+            *reinterpret_cast<Cells<real, V> *>(lds + a_out) = o;
+        else if (out_before)
+            lds[a_out] = o.c[V - 1];
+        else if (out_after)
+            lds[a_out] = o.c[0];
     };
 
     auto at_plane = [&](int x) { return (int64_t)min(x, nx - 1) * plane; };
@@ -363,7 +405,7 @@ __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
     old[1] = load_cells<real, V>(p_prev, at_plane(s0 + 1) + row);
     out[0] = load_cells<real, V>(p, at_plane(s0) + out_at);
     out[1] = load_cells<real, V>(p, at_plane(s0 + 1) + out_at);
-    if (valid) pn_slot[0][tid] = pn[0];
+    publish_pn(pn_lds[0], pn[0], out[0]);
     __syncthreads();
 
     Cells<real, V> p1_lo{}, p1_mid{};  // the lane's p^{n+1} of planes s - 2, s - 1
@@ -379,71 +421,38 @@ __global__ __launch_bounds__(512) void wave_step2(const Wave2Args<real> args,
             Cells<real, V> p1{};
             if (s < nx && computes) {
                 // A: step 1 at plane s.
-                const int64_t base = (int64_t)s * plane;
-                const bool has_xhi = s + 1 < nx;
-                const Cells<real, V> &o = out[ph & 1];
-                const Cells<real, V> south = out_south ? o : pn_slot[cur][ys];
-                const Cells<real, V> north = out_north ? o : pn_slot[cur][yn];
-                const real z_before = out_before ? o.c[V - 1] : pn_slot[cur][zb].c[V - 1];
-                const real z_after = out_after ? o.c[0] : pn_slot[cur][za].c[0];
-#pragma unroll
-                for (int m = 0; m < V; ++m) {
-                    const real c = pn_0.c[m];
-                    const real fx_hi = has_xhi ? pn_p1.c[m] - c : zero;
-                    const real fx_lo = s > 0 ? c - pn_m1.c[m] : zero;
-                    const real fy_hi = has_yhi ? north.c[m] - c : zero;
-                    const real fy_lo = has_ylo ? c - south.c[m] : zero;
-                    const real next_z = m + 1 < V ? pn_0.c[m + 1] : z_after;
-                    const real prev_z = m > 0 ? pn_0.c[m - 1] : z_before;
-                    const real fz_hi = (m + 1 < V || has_zhi) ? next_z - c : zero;
-                    const real fz_lo = (m > 0 || has_zlo) ? c - prev_z : zero;
-                    const real lap = ((fx_hi - fx_lo) + (fy_hi - fy_lo)) + (fz_hi - fz_lo);
-                    p1.c[m] = cell(s, m, c, old[ph & 1].c[m], lap);
-                }
+                p1 = s > 0 && s + 1 < nx
+                         ? advance(std::true_type{}, pn_m1, pn_0, pn_p1, pn_lds[cur], old[ph & 1], s)
+                         : advance(std::false_type{}, pn_m1, pn_0, pn_p1, pn_lds[cur], old[ph & 1], s);
                 // Halo lanes add the sources too: their p^{n+1} feeds the owners' second step.
                 if (mine) add_sources<real, V>(a, p1, s, jk0, step);
                 if (owner && s >= i0 && s < i1) {
-                    *reinterpret_cast<Cells<real, V> *>(q1 + base + row) = p1;
-                    if (mine) record<real, V>(a, q1 + base + row, s, jk0, column);
+                    real *const cells = q1 + (int64_t)s * plane + row;
+                    *reinterpret_cast<Cells<real, V> *>(cells) = p1;
+                    if (mine) record<real, V>(a, cells, s, jk0, column);
                 }
                 // B: publish it.
-                p1_slot[cur][tid] = p1;
+                *reinterpret_cast<Cells<real, V> *>(p1_lds[cur] + own) = p1;
             }
             // B: p^{n-1} and the outside neighbour of plane s + 2, into the slots just freed.
             old[ph & 1] = load_cells<real, V>(p_prev, at_plane(s + 2) + row);
+            const Cells<real, V> out_next = out[(ph + 1) & 1];
             out[ph & 1] = load_cells<real, V>(p, at_plane(s + 2) + out_at);
             const int r = s - 1;
             if (r >= i0 && owner) {
-                // C: step 2 at plane r.
-                const int prev = cur ^ 1;
-                const Cells<real, V> south = p1_slot[prev][ys];
-                const Cells<real, V> north = p1_slot[prev][yn];
-                const real z_before = p1_slot[prev][zb].c[V - 1];
-                const real z_after = p1_slot[prev][za].c[0];
-                const bool has_xhi = s < nx;
-                Cells<real, V> next;
-#pragma unroll
-                for (int m = 0; m < V; ++m) {
-                    const real c = p1_mid.c[m];
-                    const real fx_hi = has_xhi ? p1.c[m] - c : zero;
-                    const real fx_lo = r > 0 ? c - p1_lo.c[m] : zero;
-                    const real fy_hi = has_yhi ? north.c[m] - c : zero;
-                    const real fy_lo = has_ylo ? c - south.c[m] : zero;
-                    const real next_z = m + 1 < V ? p1_mid.c[m + 1] : z_after;
-                    const real prev_z = m > 0 ? p1_mid.c[m - 1] : z_before;
-                    const real fz_hi = (m + 1 < V || has_zhi) ? next_z - c : zero;
-                    const real fz_lo = (m > 0 || has_zlo) ? c - prev_z : zero;
-                    const real lap = ((fx_hi - fx_lo) + (fy_hi - fy_lo)) + (fz_hi - fz_lo);
-                    next.c[m] = cell(r, m, c, pn_m1.c[m], lap);
-                }
+                // C: step 2 at plane r (the x neighbour above is p1 of plane s, absent at nx).
+                Cells<real, V> next =
+                    r > 0 && s < nx
+                        ? advance(std::true_type{}, p1_lo, p1_mid, p1, p1_lds[cur ^ 1], pn_m1, r)
+                        : advance(std::false_type{}, p1_lo, p1_mid, p1, p1_lds[cur ^ 1], pn_m1, r);
                 if (mine) add_sources<real, V>(a, next, r, jk0, step + 1);
                 real *const cells = q2 + (int64_t)r * plane + row;
                 *reinterpret_cast<Cells<real, V> *>(cells) = next;
                 if (mine) record<real, V>(a, cells, r, jk0, column + 1);
             }
-            // D: p^n of plane s + 1 for the next step 1; p^n of plane s + 3 into the slot of
-            // plane s - 1, whose last use was step 2 above.
-            if (valid && s + 1 < nx) pn_slot[cur ^ 1][tid] = pn_p1;
+            // D: p^n (and the outside neighbour) of plane s + 1 for the next step 1; p^n of
+            // plane s + 3 into the slot of plane s - 1, whose last use was step 2 above.
+            if (s + 1 < nx) publish_pn(pn_lds[cur ^ 1], pn_p1, out_next);
             pn[(ph + 3) & 3] = load_cells<real, V>(p, at_plane(s + 3) + row);
             __syncthreads();
             p1_lo = p1_mid;

@@ -486,8 +486,9 @@ def timed_run(solver, steps: int, warmup: int) -> float:
 def scheme_variants(size: int, steps: int, warmup: int) -> dict:
     """The throughput workload's grid, source and receivers with walls of absorption 0.15 and
     neither sponge nor air (what the pressure-only scheme supports): the pressure-only scheme
-    (3 elements per cell-update) in fp32 and fp64 against the first-order scheme on the same
-    configuration."""
+    against the first-order scheme on the same configuration. The pressure-only kernel moves 3
+    elements per cell-update with one step per sweep and 2 with two (halo re-reads not
+    counted); fp32 runs both, fp64 one step per sweep (its default)."""
     import torch
 
     from ac_fdtd_amd import Grid
@@ -501,8 +502,9 @@ def scheme_variants(size: int, steps: int, warmup: int) -> dict:
     cells = grid.n_cells
     recordings = {}
     for name, cls, dtype, kwargs in (
-        ("pressure_only_fp32", PressureOnlyFDTD, np.float32, {}),
-        ("pressure_only_fp64", PressureOnlyFDTD, np.float64, {}),
+        ("pressure_only_fp32", PressureOnlyFDTD, np.float32, {"time_block": 2}),
+        ("pressure_only_fp32_one_step_per_sweep", PressureOnlyFDTD, np.float32, {"time_block": 1}),
+        ("pressure_only_fp64", PressureOnlyFDTD, np.float64, {"time_block": 1}),
         ("first_order_fp32", AcousticFDTD, np.float32, {"autotune": True}),
     ):
         solver = cls(grid, dtype=dtype, **walls, **kwargs)
@@ -513,24 +515,28 @@ def scheme_variants(size: int, steps: int, warmup: int) -> dict:
         ms = timed_run(solver, steps, warmup)
         entry = {"gcells": round(cells / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 5)}
         if cls is PressureOnlyFDTD:
-            itemsize = np.dtype(dtype).itemsize
-            entry["bytes_per_cell_update"] = 3 * itemsize
-            entry["hbm_fraction"] = round(3 * itemsize * cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            moved = (2 if kwargs["time_block"] == 2 else 3) * np.dtype(dtype).itemsize
+            entry["bytes_per_cell_update"] = moved
+            entry["hbm_fraction"] = round(moved * cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         recordings[name] = solver.recorded_pressure.astype(np.float64)
         out[name] = entry
         del solver
         torch.cuda.empty_cache()
     # Beyond the 256 MB last-level cache: two fp32 fields of 512^3 are 1.07 GB.
     big = Grid(dx=DX, nx=2 * size, ny=2 * size, nz=2 * size)
-    solver = PressureOnlyFDTD(big, dtype=np.float32, **walls)
-    solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt, width))
-    ms = timed_run(solver, max(steps // 4, 20), warmup)
-    out["pressure_only_fp32_double_size"] = {
-        "grid": [2 * size] * 3, "gcells": round(big.n_cells / (ms * 1e-3) / 1e9, 3),
-        "ms_per_step": round(ms, 5),
-        "hbm_fraction": round(12 * big.n_cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    del solver
-    torch.cuda.empty_cache()
+    for name, time_block in (("pressure_only_fp32_double_size", 2),
+                             ("pressure_only_fp32_double_size_one_step_per_sweep", 1)):
+        solver = PressureOnlyFDTD(big, dtype=np.float32, time_block=time_block, **walls)
+        solver.add_volume_source(source, gaussian_derivative_pulse(steps + warmup + 8, solver.dt,
+                                                                   width))
+        ms = timed_run(solver, max(steps // 4, 20), warmup)
+        moved = 8 if time_block == 2 else 12
+        out[name] = {
+            "grid": [2 * size] * 3, "gcells": round(big.n_cells / (ms * 1e-3) / 1e9, 3),
+            "ms_per_step": round(ms, 5), "bytes_per_cell_update": moved,
+            "hbm_fraction": round(moved * big.n_cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del solver
+        torch.cuda.empty_cache()
     # Fourth order in space: rigid walls (images), with the throughput workload's sponge and air.
     from ac_fdtd_amd.fourth import FourthOrderFDTD
 
