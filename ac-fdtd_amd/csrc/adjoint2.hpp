@@ -180,8 +180,8 @@ __device__ __forceinline__ void adjoint_cells(
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
     const bool lo_interior = !XWALL || i > 0;
-    const real fxc = LAYER ? a.cell[0][i] : real(1);
-    const real fxf_hi = LAYER ? a.face[0][i + 1] : real(1);
+    const real fxc = LAYER ? uniform_load(a.cell[0] + i) : real(1);
+    const real fxf_hi = LAYER ? uniform_load(a.face[0] + i + 1) : real(1);
 
     // Faces (adjoint_face_sweep).
 #pragma unroll
@@ -335,7 +335,7 @@ __device__ __forceinline__ void adjoint2_march(
         vload<real, V>(ld_in + L.cell(s), ld_a);
         vload<real, V>(lvx_in + L.cell(s), v);
         vload<real, V>(ld_in + L.cell(s > 0 ? s - 1 : s), prev);
-        const real fxf = LAYER ? a.face[0][s] : real(1);
+        const real fxf = LAYER ? uniform_load(a.face[0] + s) : real(1);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = v[e] + ((s > 0 ? prev[e] : real(0)) - ld_a[e]);

@@ -153,8 +153,8 @@ __device__ __forceinline__ void adjoint_march_vec(
         const int64_t c = L.cell(i);
         const bool up_interior = !XWALL || i + 1 < nx;
         const bool lo_interior = !XWALL || i > 0;
-        const real fxc = LAYER ? a.cell[0][i] : real(1);
-        const real fxf_hi = LAYER ? a.face[0][i + 1] : real(1);
+        const real fxc = LAYER ? uniform_load(a.cell[0] + i) : real(1);
+        const real fxf_hi = LAYER ? uniform_load(a.face[0] + i + 1) : real(1);
 
         // Loads, none behind a per-lane branch.
         real lvx_v[V], ld_up[V], lvy_lo_in[V], lvy_hi_in[V], ld_ylo[V], ld_yhi[V], lvz_v[V];
@@ -240,7 +240,7 @@ __device__ __forceinline__ void adjoint_march_vec(
         }
         if (!has_zhi) lvz_out[cz + V] = lvz_f[V];
         if (sponge_face) {
-            const real wxc = g.w_cell[0][i], wxf_hi = g.w_face[0][i + 1];
+            const real wxc = uniform_load(g.w_cell[0] + i), wxf_hi = uniform_load(g.w_face[0] + i + 1);
             real ax[V], ay_lo[V], ay_hi[V], az[V];
             vload<real, V>(io.after_vx + c + L.plane, ax);
             vload<real, V>(io.after_vy + cy, ay_lo);
@@ -314,7 +314,7 @@ __device__ __forceinline__ void adjoint_march_vec(
             }
             const int cx = XWALL ? wall_class(i, nx) : 0;
             real wxc = 0;
-            if (sponge_cell) wxc = g.w_cell[0][i];
+            if (sponge_cell) wxc = uniform_load(g.w_cell[0] + i);
             real lp_out[V], ld_new[V];
 #pragma unroll
             for (int e = 0; e < V; ++e) {

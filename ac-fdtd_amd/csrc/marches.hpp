@@ -391,8 +391,8 @@ __device__ __forceinline__ void fused_march_vec(
         constexpr bool XWALL = decltype(xwall_tag)::value;
         const int64_t c = L.cell(i);
         const bool up_interior = !XWALL || i + 1 < nx;
-        const real fxc = LAYER ? cell_x[i] : real(1);
-        const real fxf_hi = LAYER ? face_x[i + 1] : real(1);
+        const real fxc = LAYER ? uniform_load(cell_x + i) : real(1);
+        const real fxf_hi = LAYER ? uniform_load(face_x + i + 1) : real(1);
 
         // Loads, none behind a branch.
         real vx_v[V], p_up[V], vy_lo_in[V], vy_hi_in[V], q_lo[V], q_hi[V], vz_v[V];

@@ -525,26 +525,6 @@ struct PlaneState {
     real psi[NR > 0 ? NR : 1][V];
 };
 
-// Element-wise copy (a struct assignment becomes a memcpy that keeps both states in scratch
-// memory instead of registers).
-template <typename real, int V, int NR>
-__device__ __forceinline__ void copy_state(PlaneState<real, V, NR> &to,
-                                           const PlaneState<real, V, NR> &from) {
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-        to.p[e] = from.p[e];
-        to.vx_hi[e] = from.vx_hi[e];
-        to.vy_lo[e] = from.vy_lo[e];
-        to.vy_hi[e] = from.vy_hi[e];
-        to.vz[e] = from.vz[e];
-    }
-    to.vz[V] = from.vz[V];
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) to.psi[r][e] = from.psi[r][e];
-}
-
 // One step of the lane's V cells in plane i, the arithmetic of fused_march_vec's plane body:
 // from the pressures p_c (plane i), p_up (plane i + 1), q_lo / q_hi (rows j -/+ 1), pz_lo /
 // pz_hi (cells k0 - 1 and k0 + V), the new velocity on the lower x face, the old velocities on
@@ -562,8 +542,8 @@ __device__ __forceinline__ void advance_cells(
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
-    const real fxc = LAYER ? a.cell[0][i] : real(1);
-    const real fxf_hi = LAYER ? a.face[0][i + 1] : real(1);
+    const real fxc = LAYER ? uniform_load(a.cell[0] + i) : real(1);
+    const real fxf_hi = LAYER ? uniform_load(a.face[0] + i + 1) : real(1);
 
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -774,7 +754,7 @@ __device__ __forceinline__ void fused2_march(
         vload<real, V>(p_in + L.cell(s), pn_c);
         vload<real, V>(vx_in + L.cell(s), v);
         vload<real, V>(p_in + L.cell(s > 0 ? s - 1 : s), prev);
-        const real fxf = LAYER ? a.face[0][s] : real(1);
+        const real fxf = LAYER ? uniform_load(a.face[0] + s) : real(1);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = s > 0 ? v[e] - vel * (pn_c[e] - prev[e]) : v[e];
@@ -1004,15 +984,21 @@ __device__ __forceinline__ void fused2_march(
         vstore<real, V>(vx_out + L.cell(i0), s2_vx_lo);
     }
 
-    for (int i = i0; i < i1; ++i) {
+    // One plane of the march: step 1 of plane i + 1 into `next`, step 2 of plane i from `here`
+    // (the step-1 state of plane i). The march alternates the two states' roles, unrolled by
+    // two, instead of copying `next` into `here` after each plane: a copy into registers that
+    // a load may still target makes the wavefront wait for that load, and with it for every
+    // store issued since (vector memory operations complete in issue order).
+    auto plane = [&](const int i, PlaneState<real, V, NR> &here,
+                     PlaneState<real, V, NR> &next) __attribute__((always_inline)) {
         const bool more = i + 1 < nx;
         if (valid && more) {
-            if (STAGED) step1_staged_at(i + 1, B);
-            else if (PREFETCH) compute1_at(i + 1, ahead, B);
-            else step1_at(i + 1, B);
+            if (STAGED) step1_staged_at(i + 1, next);
+            else if (PREFETCH) compute1_at(i + 1, ahead, next);
+            else step1_at(i + 1, next);
             if (owner && i + 1 < i1 && has_events(c, i + 1))
-                record_cells<real, V>(a, c, i + 1, column, B.p);
-            share(i + 1, B.p);
+                record_cells<real, V>(a, c, i + 1, column, next.p);
+            share(i + 1, next.p);
             // The next iteration's step 1 (plane i + 2), requested now: it travels during the
             // barrier and the second step. The stage is refilled only after this lane's own
             // reads of it are done.
@@ -1029,11 +1015,14 @@ __device__ __forceinline__ void fused2_march(
             // plane states in scratch memory.
             real p_next[V];
 #pragma unroll
-            for (int e = 0; e < V; ++e) p_next[e] = more ? B.p[e] : A.p[e];
-            if (i == 0 || i == nx - 1) step2(i, A, p_next, Wall{});
-            else step2(i, A, p_next, Interior{});
+            for (int e = 0; e < V; ++e) p_next[e] = more ? next.p[e] : here.p[e];
+            if (i == 0 || i == nx - 1) step2(i, here, p_next, Wall{});
+            else step2(i, here, p_next, Interior{});
         }
-        if (valid && more) copy_state(A, B);
+    };
+    for (int i = i0; i < i1; i += 2) {
+        plane(i, A, B);
+        if (i + 1 < i1) plane(i + 1, B, A);
     }
 }
 

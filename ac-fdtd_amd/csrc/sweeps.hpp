@@ -112,6 +112,16 @@ struct Layout {
 // Wall class of index i on an axis of n cells: bit 0 low face, bit 1 high face.
 __device__ inline int wall_class(int i, int n) { return (i == 0 ? 1 : 0) | (i == n - 1 ? 2 : 0); }
 
+// A load of read-only data (never written during the launch) at a wavefront-uniform address,
+// through the scalar data cache: it counts in the scalar memory counter, so a wait for its value
+// does not also wait for the wavefront's vector memory operations in flight (vector loads and
+// stores complete in issue order, so a vector load issued late is only awaited together with
+// everything issued before it, such as the next plane's inputs).
+template <typename T>
+__device__ inline T uniform_load(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 template <typename real>
 struct SweepArgs {
     Tiling tiles;
