@@ -531,7 +531,8 @@ struct PlaneState {
 // the other faces and the relaxation states (advanced in place), produce the new velocities,
 // the new pressure after walls, sponge and sources of `step`, and the boundary tape's value
 // p_tmp + p_old (the pressure before the wall closed form plus the old pressure).
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool XWALL>
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool XWALL,
+          bool PLAINXY = false>
 __device__ __forceinline__ void advance_cells(
     const SweepArgs<real> &a, const LaneConsts<real, V> &c, const int i, const int64_t step,
     const real (&p_c)[V], const real (&p_up)[V], const real (&q_lo)[V], const real (&q_hi)[V],
@@ -542,25 +543,32 @@ __device__ __forceinline__ void advance_cells(
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
-    const real fxc = LAYER ? uniform_load(a.cell[0] + i) : real(1);
-    const real fxf_hi = LAYER ? uniform_load(a.face[0] + i + 1) : real(1);
+    // PLAINXY: every x and y sponge factor of the plane and row is 1, so their products are
+    // skipped (v * 1 is v exactly) and only the z factor is applied.
+    const real fxc = LAYER && !PLAINXY ? uniform_load(a.cell[0] + i) : real(1);
+    const real fxf_hi = LAYER && !PLAINXY ? uniform_load(a.face[0] + i + 1) : real(1);
+    auto damp = [&](const real v, const real fx, const real fy, const real fz)
+                    __attribute__((always_inline)) {
+        if (PLAINXY) return LAYER ? v * fz : v;
+        return damp3<real, LAYER>(v, fx, fy, fz);
+    };
 
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const real wx = up_interior ? vx_v[e] - vel * (p_up[e] - p_c[e]) : vx_v[e];
-        o.vx_hi[e] = damp3<real, LAYER>(wx, fxf_hi, c.fyc, c.fzc[e]);
+        o.vx_hi[e] = damp(wx, fxf_hi, c.fyc, c.fzc[e]);
         const real wl = c.has_ylo ? vy_lo_in[e] - vel * (p_c[e] - q_lo[e]) : vy_lo_in[e];
-        o.vy_lo[e] = damp3<real, LAYER>(wl, fxc, c.fyf_lo, c.fzc[e]);
+        o.vy_lo[e] = damp(wl, fxc, c.fyf_lo, c.fzc[e]);
         const real wh = c.has_yhi ? vy_hi_in[e] - vel * (q_hi[e] - p_c[e]) : vy_hi_in[e];
-        o.vy_hi[e] = damp3<real, LAYER>(wh, fxc, c.fyf_hi, c.fzc[e]);
+        o.vy_hi[e] = damp(wh, fxc, c.fyf_hi, c.fzc[e]);
         real wz;
         if (e > 0) wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
         else wz = c.has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
-        o.vz[e] = damp3<real, LAYER>(wz, fxc, c.fyc, c.fzf[e]);
+        o.vz[e] = damp(wz, fxc, c.fyc, c.fzf[e]);
     }
     {
         const real wz = c.has_zhi ? vz_top - vel * (pz_hi - p_c[V - 1]) : vz_top;
-        o.vz[V] = damp3<real, LAYER>(wz, fxc, c.fyc, c.fzf[V]);
+        o.vz[V] = damp(wz, fxc, c.fyc, c.fzf[V]);
     }
 
     const int cx = XWALL ? wall_class(i, nx) : 0;
@@ -610,7 +618,7 @@ __device__ __forceinline__ void advance_cells(
             }
             if (is_wall) pn = pn * fu - p_old * fp;
         }
-        if (LAYER) pn = ((pn * fxc) * c.fyc) * c.fzc[e];
+        if (LAYER) pn = PLAINXY ? pn * c.fzc[e] : ((pn * fxc) * c.fyc) * c.fzc[e];
         o.p[e] = pn;
     }
     if (has_events(c, i)) {
@@ -738,6 +746,21 @@ __device__ __forceinline__ void fused2_march(
     // and the compiler waits for them inside the loop with a memory count of zero, which on
     // this counter also waits for every store issued before (the boundary tape's).
     wait_memory();
+    // Whether every x and y sponge factor of plane x is 1 for the whole wavefront (rows are
+    // whole wavefronts or less, and outside the sponge most are): the step then skips those
+    // products, which leave a value unchanged (x * 1 is x), and is the same bits with fewer
+    // instructions. Uniform across the wavefront, so it selects a code path, not lanes.
+    const bool lane_row_plain =
+        !LAYER || !valid || (c.fyc == real(1) && c.fyf_lo == real(1) && c.fyf_hi == real(1));
+    const bool wave_rows_plain = LAYER && __ballot(!lane_row_plain) == 0;
+    auto plain_xy = [&](const int x) __attribute__((always_inline)) {
+        if constexpr (!LAYER) {
+            return false;
+        } else {
+            return wave_rows_plain && uniform_load(a.cell[0] + x) == real(1) &&
+                   uniform_load(a.face[0] + x + 1) == real(1);
+        }
+    };
 
     auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
         return lds + ((plane % 3) * T + lane) * V;
@@ -866,9 +889,13 @@ __device__ __forceinline__ void fused2_march(
                         auto xwall_tag) __attribute__((always_inline)) {
         constexpr bool XWALL = decltype(xwall_tag)::value;
         real tape_value[V];
-        advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
-            a, c, s, step, pn_c, in.p_up, in.q_lo, in.q_hi, in.pz_lo, in.pz_hi, s1_vx_lo, in.vx_v,
-            in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value);
+        auto run = [&](auto plain_tag) __attribute__((always_inline)) {
+            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value>(
+                a, c, s, step, pn_c, in.p_up, in.q_lo, in.q_hi, in.pz_lo, in.pz_hi, s1_vx_lo,
+                in.vx_v, in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value);
+        };
+        if (plain_xy(s)) run(Wall{});
+        else run(Interior{});
         if (a.wall_tape && owner && s >= i0 && s < i1)
             tape_cells<real, V, XWALL>(a, c, s, step, t2.tape_vec != 0, tape_value);
 #pragma unroll
@@ -919,9 +946,13 @@ __device__ __forceinline__ void fused2_march(
         for (int e = 0; e < V; ++e) vz_v[e] = A.vz[e];
         PlaneState<real, V, NR> o;
         real tape_value[V];
-        advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL>(
-            a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi, A.vy_lo,
-            A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
+        auto run = [&](auto plain_tag) __attribute__((always_inline)) {
+            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value>(
+                a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi,
+                A.vy_lo, A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
+        };
+        if (plain_xy(i)) run(Wall{});
+        else run(Interior{});
         if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, t2.tape_vec != 0, tape_value);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
