@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void fused2_march(
     if (owner) {
         // v^(n+3/2) on face i0, the lower face of the first cell (the prologue of
         // fused_march_vec with the first step's results as inputs).
-        const real fxf = LAYER ? a.face[0][i0] : real(1);
+        const real fxf = LAYER ? uniform_load(a.face[0] + i0) : real(1);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = i0 > 0 ? v_half_lo[e] - vel * (A.p[e] - p_prev[e]) : v_half_lo[e];
