@@ -96,6 +96,8 @@ class Desc(_c.Structure):
         ("wall_tape_step0", _c.c_int64),
         ("wall_tape_rows", _c.c_int64),
         ("psi_swap", _ptr),
+        ("z_pitch", _c.c_int32),
+        ("reserved", _c.c_int32),
     ]
 
 

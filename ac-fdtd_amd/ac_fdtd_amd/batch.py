@@ -2,26 +2,32 @@
 
 Each room is an ordinary :class:`~ac_fdtd_amd.scheme.AcousticFDTD` (its own grid, walls, air,
 sources, receivers and state tensors); the batch advances them together through the library's
-batch API (``acfdtd_hip_batch_*`` in ``include/acfdtd_hip.h``), one launch per two steps (per
-lane-width group) over a tile table that covers every room. Results are bit-identical to
-running each room alone. :meth:`BatchedRooms.capture` records a step loop into a HIP graph.
-There is no reference counterpart: the reference runs one room at a time.
+batch API (``acfdtd_hip_batch_*`` in ``include/acfdtd_hip.h``), one launch per two steps over a
+tile table that covers every room. Every room runs on the widest lanes of its precision (4 z
+cells fp32, 2 fp64): a room whose nz that width does not divide is advanced in rows padded to a
+multiple of it (the descriptor's ``z_pitch``), in buffers the batch owns, and its state is copied
+into them at the start of :meth:`BatchedRooms.run` and back at the end, so the room's own tensors
+keep the reference layout. Results are bit-identical to running each room alone.
+:meth:`BatchedRooms.capture` records a step loop into a HIP graph. There is no reference
+counterpart: the reference runs one room at a time.
 """
 
 from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _native
 from .scheme import AcousticFDTD
 
-__all__ = ["BatchedRooms", "CapturedBatch", "lane_width"]
+__all__ = ["BatchedRooms", "CapturedBatch", "lane_width", "padded_row"]
 
 
 def lane_width(solver: AcousticFDTD) -> int:
-    """z cells per lane the fused kernels use for this room (4 / 2 / 1 as nz allows)."""
+    """z cells per lane the single-room fused kernels use for this room (4 / 2 / 1 as nz
+    allows)."""
     widest = 4 if solver.np_dtype.itemsize == 4 else 2
     width = widest
     while width > 1 and solver.grid.nz % width:
@@ -29,11 +35,94 @@ def lane_width(solver: AcousticFDTD) -> int:
     return width
 
 
+def padded_row(solver: AcousticFDTD) -> int:
+    """Row length of ``solver``'s fields in a batch: nz rounded up to the widest lane of its
+    precision (4 cells fp32, 2 fp64); nz itself when that width divides it."""
+    widest = 4 if solver.np_dtype.itemsize == 4 else 2
+    return -(-solver.grid.nz // widest) * widest
+
+
+class _PaddedRoom:
+    """A room's state, swap set and cell tables in rows of ``padded_row`` elements (every
+    field, vz included), owned by the batch. The elements past the room's own in each row are
+    padding, which the kernels never let reach a cell."""
+
+    def __init__(self, solver: AcousticFDTD) -> None:
+        self.solver = solver
+        g = solver.grid
+        self.zp = padded_row(solver)
+        opts = solver._opts
+        zp = self.zp
+
+        def fields():
+            return [torch.zeros((g.nx, g.ny, zp), **opts),
+                    torch.zeros((g.nx + 1, g.ny, zp), **opts),
+                    torch.zeros((g.nx, g.ny + 1, zp), **opts),
+                    torch.zeros((g.nx, g.ny, zp), **opts)]
+
+        n_relax = solver.coefficients.n_relax
+        self.state = fields()
+        self.psi = torch.zeros((n_relax, g.nx, g.ny, zp), **opts) if n_relax else None
+        self.swap = fields() if solver._swap is not None else None
+        self.psi_swap = (torch.zeros((n_relax, g.nx, g.ny, zp), **opts)
+                         if solver._psi_swap is not None else None)
+        self._tables: dict[tuple, torch.Tensor] = {}
+
+    def _cells(self, cells) -> torch.Tensor:
+        """(i, j * nz + k) pairs re-addressed to rows of zp: (i, j * zp + k), on the device
+        (kept, so that a table a captured graph's room table points at stays alive)."""
+        key = tuple(cells)
+        if key not in self._tables:
+            nz = self.solver.grid.nz
+            table = np.array([(i, (jk // nz) * self.zp + jk % nz) for i, jk in cells],
+                             dtype=np.int32).ravel()
+            self._tables[key] = torch.as_tensor(table, device=self.solver.device)
+        return self._tables[key]
+
+    def _views(self):
+        """(padded view, room tensor) pairs of the state that crosses a run's boundaries."""
+        s, nz = self.solver, self.solver.grid.nz
+        pairs = [(self.state[0][..., :nz], s.p)]
+        pairs += [(self.state[a + 1][..., : nz + (a == 2)], s.velocity[a]) for a in range(3)]
+        if self.psi is not None:
+            pairs.append((self.psi[..., :nz], s.relaxation_states))
+        return pairs
+
+    def load(self) -> None:
+        """Copy the room's state into the padded rows."""
+        for padded, room in self._views():
+            padded.copy_(room)
+
+    def store(self) -> None:
+        """Copy the padded rows' state back into the room's tensors."""
+        for padded, room in self._views():
+            room.copy_(padded)
+
+    def descriptor(self, recording) -> _native.Desc:
+        s = self.solver
+        d = s._descriptor(recording, tape=False)
+        d.z_pitch = self.zp
+        d.p, d.vx, d.vy, d.vz = (t.data_ptr() for t in self.state)
+        if self.psi is not None:
+            d.psi = self.psi.data_ptr()
+        if self.swap is not None:
+            d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.swap)
+        if self.psi_swap is not None:
+            d.psi_swap = self.psi_swap.data_ptr()
+        if s._sources:
+            d.source_cell = self._cells([(i, jk) for i, jk, _ in s._sources]).data_ptr()
+        if recording is not None and s._receivers:
+            d.receiver_cell = self._cells(s._receivers).data_ptr()
+        return d
+
+
 class _Group:
-    """One library batch handle: rooms that share a lane width."""
+    """One library batch handle over ``solvers``; rooms whose nz the lane width does not
+    divide advance in padded rows (:class:`_PaddedRoom`)."""
 
     def __init__(self, solvers: list[AcousticFDTD], lib) -> None:
         self.solvers = solvers
+        self._padded = [_PaddedRoom(s) if padded_row(s) != s.grid.nz else None for s in solvers]
         self.device = solvers[0].device
         self._lib = lib
         self._handle = None
@@ -59,9 +148,23 @@ class _Group:
 
     def _descriptors(self, recordings):
         array = (_native.Desc * len(self.solvers))()
-        for n, solver in enumerate(self.solvers):
-            array[n] = solver._descriptor(None if recordings is None else recordings[n], tape=False)
+        for n, (solver, padded) in enumerate(zip(self.solvers, self._padded)):
+            recording = None if recordings is None else recordings[n]
+            array[n] = (padded.descriptor(recording) if padded is not None
+                        else solver._descriptor(recording, tape=False))
         return array
+
+    def load(self) -> None:
+        """Copy the padded rooms' state into their padded rows (stream-ordered)."""
+        for padded in self._padded:
+            if padded is not None:
+                padded.load()
+
+    def store(self) -> None:
+        """Copy the padded rooms' state back into their own tensors (stream-ordered)."""
+        for padded in self._padded:
+            if padded is not None:
+                padded.store()
 
     def upload(self, recordings) -> None:
         """Point the device room table at the rooms' current buffers and ``recordings``
@@ -85,7 +188,10 @@ class _Group:
             for s in self.solvers
         ]
         self.upload(recordings)
-        self.launch(step_index, n_steps)
+        with torch.cuda.device(self.device):
+            self.load()
+            self.launch(step_index, n_steps)
+            self.store()
         return recordings
 
     def close(self) -> None:
@@ -121,8 +227,8 @@ class BatchedRooms:
     """Advance ``solvers`` together. They must share device, precision, relaxation count,
     sponge presence, compensation and tiling, use the fused path, and be at the same step.
 
-    Rooms are grouped by lane width (how many z cells each lane takes, set by nz): one launch
-    per group and step, so uneven rooms keep the widest accesses their rows allow.
+    Every room takes the widest lanes of the precision (rooms whose nz it does not divide in
+    padded rows, see :func:`padded_row`), so all rooms advance in one launch per two steps.
     """
 
     def __init__(self, solvers: list[AcousticFDTD]) -> None:
@@ -140,13 +246,9 @@ class BatchedRooms:
         self._tables = None  # the CapturedBatch whose buffers the room tables point at, if any
         self._lib = _native.library()
         _bind(self._lib)
-        by_width: dict[int, list[AcousticFDTD]] = {}
-        for solver in solvers:
-            by_width.setdefault(lane_width(solver), []).append(solver)
         self._groups = []
         try:
-            for group in by_width.values():
-                self._groups.append(_Group(group, self._lib))
+            self._groups.append(_Group(self.solvers, self._lib))
         except Exception:
             self.close()
             raise
@@ -157,7 +259,7 @@ class BatchedRooms:
 
     @property
     def two_step(self) -> list[bool]:
-        """Per lane-width group, whether it advances two steps per sweep."""
+        """Per launch group (one: every room), whether it advances two steps per sweep."""
         return [group.two_step for group in self._groups]
 
     @property
@@ -198,7 +300,7 @@ class CapturedBatch:
     """A step loop of :class:`BatchedRooms` captured into one HIP graph.
 
     The graph holds every launch of ``n_steps`` steps of all groups from the batch's step at
-    capture time. The launches read each group's device room table (the rooms' buffers and
+    capture time, with the copies of padded rooms' state into their padded rows and back. The launches read each group's device room table (the rooms' buffers and
     where their receivers record); the table is pointed at this capture's recordings before
     the capture and again before a replay that follows an eager :meth:`BatchedRooms.run`
     (which re-points it, and synchronises the stream to do so). :meth:`replay` runs the same
@@ -231,7 +333,9 @@ class CapturedBatch:
             capture_stream.wait_stream(torch.cuda.current_stream(device))
             with torch.cuda.graph(self.graph, stream=capture_stream):
                 for group in batch._groups:
+                    group.load()  # padded rooms: their state into the padded rows, and back
                     group.launch(self.step0, n_steps)
+                    group.store()
             torch.cuda.current_stream(device).wait_stream(capture_stream)
 
     def _point_tables(self) -> None:

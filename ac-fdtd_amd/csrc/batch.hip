@@ -5,9 +5,11 @@
 // (one BatchRoom per room, read through wave-uniform scalar loads; its field pointers are
 // declared global, see global_ptr). The lanes of a room flatten
 // its (y, z / V) plane as in the single-room fused kernel, with one V for the whole batch (the
-// widest every room's nz allows). The XCD remap applies to the global tile index, so each XCD
-// owns a contiguous run of tiles of one or a few rooms. Steps go in fused pairs (A -> B -> A);
-// an odd trailing step runs room by room through the two in-place sweeps.
+// widest every room's row length allows; a room given padded rows, z_pitch, takes lanes as
+// wide as the pitch allows, its last lane per row partly padding). The XCD remap applies to the
+// global tile index, so each XCD owns a contiguous run of tiles of one or a few rooms. Steps go
+// in fused pairs (A -> B -> A); an odd trailing step runs room by room through the two in-place
+// sweeps.
 
 #include <hip/hip_runtime.h>
 
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(512) void batch_fused_step(const BatchRoom<real> *_
     SweepArgs<real> a = room.args;
     global_ptrs(a);
     int j, k0;
-    if (!flat_lane(a.tiles, a.ny, a.nz, t.jb, j, k0)) return;
+    if (!flat_lane(a.tiles, a.ny, a.zp, t.jb, j, k0)) return;
     const int i0 = t.xc * a.tiles.chunk;
     const int i1 = min(i0 + a.tiles.chunk, a.nx);
     FusedIO<real> io = reverse ? room.backward : room.forward;
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(512) void batch_fused_step(const BatchRoom<real> *_
     io.vx_out = global_ptr(io.vx_out);
     io.vy_out = global_ptr(io.vy_out);
     io.vz_out = global_ptr(io.vz_out);
-    fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(
+    fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED, true>(
         a, i0, i1, j, k0, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
         io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
         a.face[1], a.cell[2], a.face[2], a.recording);
@@ -109,7 +111,7 @@ BatchKernel<real> pick_batch(int vec, int nr, bool walls, bool layer, bool comp)
     return pick_batch_v<real, 1>(nr, walls, layer, comp);
 }
 
-// The batch's lane width: the widest V that divides every room's nz.
+// The batch's lane width: the widest V that divides every room's row length (nz or z_pitch).
 inline int batch_vec(const acfdtd_hip_desc *rooms, int n_rooms) {
     int vec = 4;
     for (int r = 0; r < n_rooms; ++r) vec = std::min(vec, fused_vec(rooms[r]));
@@ -129,7 +131,7 @@ int batch_validate(const acfdtd_hip_desc *rooms, int32_t n_rooms, int64_t *n_til
     const int vec = batch_vec(rooms, n_rooms);
     int64_t total = 0;
     for (int r = 0; r < n_rooms; ++r) {
-        const int status = validate(&rooms[r]);
+        const int status = validate(&rooms[r], true);
         if (status != ACFDTD_HIP_OK)
             return fail(status, "room " + std::to_string(r) + ": " + g_last_error);
         const acfdtd_hip_desc &a = rooms[0], &b = rooms[r];
@@ -440,9 +442,13 @@ extern "C" int acfdtd_hip_batch_run(acfdtd_hip_batch *batch, int64_t step_offset
                      ? batch_pairs<double>(*batch, step_offset + done, pairs, recording_column + done, s)
                      : batch_pairs<float>(*batch, step_offset + done, pairs, recording_column + done, s);
     if (status != ACFDTD_HIP_OK || rest % 2 == 0) return status;
+    // An odd last step: the two in-place sweeps, room by room (they take padded rows too).
     for (const acfdtd_hip_desc &d : batch->descs) {
-        status = acfdtd_hip_run(&d, step_offset + n_steps - 1, 1, recording_column + n_steps - 1,
-                                stream);
+        status = d.real_bytes == 8
+                     ? inplace_steps<double>(d, step_offset + n_steps - 1, 1,
+                                             recording_column + n_steps - 1, s)
+                     : inplace_steps<float>(d, step_offset + n_steps - 1, 1,
+                                            recording_column + n_steps - 1, s);
         if (status != ACFDTD_HIP_OK) return status;
     }
     return ACFDTD_HIP_OK;

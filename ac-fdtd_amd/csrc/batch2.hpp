@@ -1,7 +1,9 @@
 // Batched rooms, two steps per sweep: the two-step sweep's workgroup (fused2_block) on a table of
 // (room, tile) entries that covers every room, one launch per two steps. Each room keeps its own
-// two-step tiling (its rows of nz / V lane groups, whole or in segments); the lane width V is
-// the batch's. Templates only; batch2*.hip instantiate them per precision and lane width.
+// two-step tiling (its rows of row-length / V lane groups, whole or in segments); the lane width
+// V is the batch's, and rooms whose nz it does not divide come with padded rows (z_pitch), so
+// the kernels take padded rows. Templates only; batch2*.hip instantiate them per precision and
+// lane width.
 
 #pragma once
 
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void batch
     Fused2IO<real> io = reverse ? room.backward : room.forward;
     global_ptrs(args);
     global_ptrs(io);
-    fused2_block<real, V, NR, WALLS, LAYER, COMPENSATED, false>(
+    fused2_block<real, V, NR, WALLS, LAYER, COMPENSATED, false, true>(
         args, io, room.t2, (unsigned)tile.tile, step, column, lds, src_lds, len_lds, now_lds,
         rx_lds, stage);
 }

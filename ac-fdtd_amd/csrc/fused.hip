@@ -19,7 +19,7 @@ __global__ __launch_bounds__(512) void fused_step_vec(const SweepArgs<real> a,
                                                       const int64_t column) {
     int kb, jb, xc, j, k0;
     if (!decode_tile(a.tiles, kb, jb, xc)) return;
-    if (!flat_lane(a.tiles, a.ny, a.nz, jb, j, k0)) return;
+    if (!flat_lane(a.tiles, a.ny, a.zp, jb, j, k0)) return;
     const int i0 = xc * a.tiles.chunk;
     const int i1 = min(i0 + a.tiles.chunk, a.nx);
     fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(

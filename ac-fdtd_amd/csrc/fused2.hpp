@@ -29,7 +29,8 @@ namespace acfdtd {
 // barrier per plane the whole workgroup waits for those lanes; in LDS their reads wait for
 // nothing else, while a read from global memory waits for every store the lane issued before
 // it); and, when STAGED, each lane's slots for the next plane's first-step inputs.
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED,
+          bool PADDED = false>
 __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fused2IO<real> &io,
                                              const Tiling2 &t2, const unsigned b,
                                              const int64_t step, const int64_t column,
@@ -65,7 +66,7 @@ __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fus
     }
     for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
     lds_barrier();
-    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED>(
+    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED, PADDED>(
         a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
         io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
         len_lds, now_lds, rx_lds, stage);

@@ -31,7 +31,7 @@ __device__ __forceinline__ void velocity_march(
     real *__restrict__ vz, const real *__restrict__ cell_x, const real *__restrict__ face_x,
     const real *__restrict__ cell_y, const real *__restrict__ face_y,
     const real *__restrict__ cell_z, const real *__restrict__ face_z) {
-    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const Layout L = room_layout(a, j, k);
     const real vel = a.vel;
     real fyc = 1, fzc = 1, fyf = 1, fzf = 1;
     if (LAYER) {
@@ -60,7 +60,7 @@ __device__ __forceinline__ void velocity_march(
         const int64_t cy = L.face_y(i);
         if (LAYER || j > 0) {
             real v = vy[cy];
-            if (j > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p[c - a.nz]);
+            if (j > 0) v = face_update<real, FUSED_FMA>(v, vel, pc - p[c - a.zp]);
             if (LAYER) v = ((v * fxc) * fyf) * fzc;
             vy[cy] = v;
         }
@@ -79,7 +79,7 @@ __device__ __forceinline__ void velocity_march(
                 vx[t] = ((vx[t] * face_x[a.nx]) * fyc) * fzc;
             }
             if (j == a.ny - 1) {
-                const int64_t t = cy + a.nz;
+                const int64_t t = cy + a.zp;
                 vy[t] = ((vy[t] * fxc) * face_y[a.ny]) * fzc;
             }
             if (k == a.nz - 1) {
@@ -100,9 +100,9 @@ __device__ __forceinline__ void pressure_march(
     const real *__restrict__ wall_table, const real *__restrict__ cell_x,
     const real *__restrict__ cell_y, const real *__restrict__ cell_z,
     real *__restrict__ recording) {
-    const Layout L(a.nx, a.ny, a.nz, j, k);
+    const Layout L = room_layout(a, j, k);
     const int64_t n_cells = L.plane * a.nx;
-    const int jk = j * a.nz + k;
+    const int jk = j * a.zp + k;
     const int code_yz = WALLS ? (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4) : 0;
     real fyc = 1, fzc = 1;
     if (LAYER) {
@@ -119,7 +119,7 @@ __device__ __forceinline__ void pressure_march(
         vx_lo_next = vx_hi;
         const int64_t cy = L.face_y(i);
         const int64_t cz = L.face_z(i);
-        const real div = ((vx_hi - vx_lo) + (vy[cy + a.nz] - vy[cy])) + (vz[cz + 1] - vz[cz]);
+        const real div = ((vx_hi - vx_lo) + (vy[cy + a.zp] - vy[cy])) + (vz[cz + 1] - vz[cz]);
 
         const real p_old = p[c];
         real pn;
@@ -283,8 +283,8 @@ __device__ __forceinline__ void lane_add(real (&x)[V], const int e, const real a
 // Lane position in the flattened (y, z / V) plane used by the V-wide fused kernels: consecutive
 // lanes take consecutive V-cell groups of a row and run on into the next row, so no lane idles
 // whatever nz is. Returns false for lanes past the plane.
-__device__ inline bool flat_lane(const Tiling &t, int ny, int nz, int jb, int &j, int &k0) {
-    const int groups = nz / t.vec;
+__device__ inline bool flat_lane(const Tiling &t, int ny, int zp, int jb, int &j, int &k0) {
+    const int groups = zp / t.vec;
     const int64_t flat = (int64_t)jb * (64 * t.tile_j) + threadIdx.y * 64 + threadIdx.x;
     if (flat >= (int64_t)ny * groups) return false;
     j = (int)(flat / groups);
@@ -292,7 +292,14 @@ __device__ inline bool flat_lane(const Tiling &t, int ny, int nz, int jb, int &j
     return true;
 }
 
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+// PADDED: the room's rows may be padded (z_pitch), so the last lane of a row may hold fewer
+// than V of the room's cells: its cells from ztop = nz - k0 on are padding, and its face ztop is
+// the room's upper z wall. That face keeps its input value (damped, as every wall face), the
+// sponge factors of padding cells are read clamped to the room's profiles, and nothing is
+// stored past the lane's own V elements. Padding cells compute values that reach no cell of the
+// room: the only face between them and the room is the wall face, which reads no pressure.
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED,
+          bool PADDED = false>
 __device__ __forceinline__ void fused_march_vec(
     const SweepArgs<real> &a, const int i0, const int i1, const int j, const int k0,
     const int64_t step, const int64_t column, const real *__restrict__ p_in,
@@ -303,17 +310,22 @@ __device__ __forceinline__ void fused_march_vec(
     const real *__restrict__ face_x, const real *__restrict__ cell_y,
     const real *__restrict__ face_y, const real *__restrict__ cell_z,
     const real *__restrict__ face_z, real *__restrict__ recording) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const Layout L(nx, ny, nz, j, k0);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, zp = a.zp;
+    const Layout L = room_layout(a, j, k0);
     const int64_t n_cells = L.plane * nx;
-    const int jk0 = j * nz + k0;
+    const int jk0 = j * zp + k0;
     const real vel = a.vel;
 
     // Neighbours that exist, and clamped offsets to read in their place when they do not.
     const bool has_ylo = j > 0, has_yhi = j + 1 < ny;
     const bool has_zlo = k0 > 0, has_zhi = k0 + V < nz;
-    const int64_t dy_lo = has_ylo ? nz : 0, dy_hi = has_yhi ? nz : 0;
+    const int64_t dy_lo = has_ylo ? zp : 0, dy_hi = has_yhi ? zp : 0;
     const int64_t dz_lo = has_zlo ? 1 : 0, dz_hi = has_zhi ? V : V - 1;
+    // PADDED: the lane's first padding cell (its upper wall face), V or more when it has none;
+    // the upper wall face past the lane (vz_top) is then read only in lanes without padding.
+    const int ztop = nz - k0;
+    const bool partial = PADDED && ztop < V;
+    const int64_t dz_top = partial ? V - 1 : V;
 
     real fyc = 1, fyf_lo = 1, fyf_hi = 1, fzc[V], fzf[V + 1];
 #pragma unroll
@@ -325,9 +337,9 @@ __device__ __forceinline__ void fused_march_vec(
         fyf_lo = face_y[j];
         fyf_hi = face_y[j + 1];
 #pragma unroll
-        for (int e = 0; e < V; ++e) fzc[e] = cell_z[k0 + e];
+        for (int e = 0; e < V; ++e) fzc[e] = cell_z[PADDED ? min(k0 + e, nz - 1) : k0 + e];
 #pragma unroll
-        for (int e = 0; e <= V; ++e) fzf[e] = face_z[k0 + e];
+        for (int e = 0; e <= V; ++e) fzf[e] = face_z[PADDED ? min(k0 + e, nz) : k0 + e];
     }
 
     // Wall classes: y and z part per cell; the factors of interior planes (no x wall).
@@ -400,12 +412,12 @@ __device__ __forceinline__ void fused_march_vec(
         vload<real, V>(p_in + c + (up_interior ? L.plane : 0), p_up);
         const int64_t cy = L.face_y(i);
         vload<real, V>(vy_in + cy, vy_lo_in);
-        vload<real, V>(vy_in + cy + nz, vy_hi_in);
+        vload<real, V>(vy_in + cy + zp, vy_hi_in);
         vload<real, V>(p_in + c - dy_lo, q_lo);
         vload<real, V>(p_in + c + dy_hi, q_hi);
         const int64_t cz = L.face_z(i);
         vload<real, V>(vz_in + cz, vz_v);
-        const real vz_top = vz_in[cz + V];
+        const real vz_top = vz_in[cz + dz_top];
         const real pz_lo = p_in[c - dz_lo];
         const real pz_hi = p_in[c + dz_hi];
         real states[NR > 0 ? NR : 1][V];
@@ -423,8 +435,12 @@ __device__ __forceinline__ void fused_march_vec(
             const real wh = has_yhi ? vy_hi_in[e] - vel * (q_hi[e] - p_c[e]) : vy_hi_in[e];
             vy_hi[e] = damp3<real, LAYER>(wh, fxc, fyf_hi, fzc[e]);
             real wz;
-            if (e > 0) wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
-            else wz = has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+            if (e > 0) {
+                wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
+                if (PADDED && e == ztop) wz = vz_v[e];  // the upper wall face
+            } else {
+                wz = has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+            }
             vz_f[e] = damp3<real, LAYER>(wz, fxc, fyc, fzf[e]);
         }
         {
@@ -433,14 +449,14 @@ __device__ __forceinline__ void fused_march_vec(
         }
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + c + L.plane, vx_hi);
         vstore<real, V>(vy_out + cy, vy_lo);
-        if (!has_yhi) vstore<real, V>(vy_out + cy + nz, vy_hi);
+        if (!has_yhi) vstore<real, V>(vy_out + cy + zp, vy_hi);
         {
             real lower[V];
 #pragma unroll
             for (int e = 0; e < V; ++e) lower[e] = vz_f[e];
             vstore<real, V>(vz_out + cz, lower);
         }
-        if (!has_zhi) vz_out[cz + V] = vz_f[V];
+        if (!has_zhi && !partial) vz_out[cz + V] = vz_f[V];
 
         // Pressure with the relaxation states, walls and sponge.
         const int cx = XWALL ? wall_class(i, nx) : 0;
@@ -572,6 +588,12 @@ FusedIO<real> fused_io(const acfdtd_hip_desc &d, bool reverse) {
 inline bool has_swap(const acfdtd_hip_desc &d) {
     return d.p_swap && d.vx_swap && d.vy_swap && d.vz_swap;
 }
+
+// Launch n_steps steps as the two in-place sweeps each (velocity, then pressure); the state
+// stays in the descriptor's fields. Takes padded rows. Defined in sweeps.hip.
+template <typename real>
+int inplace_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
+                  hipStream_t stream);
 
 // Launch the fused step for the leading even number of n_steps (pairs A -> B -> A); returns the
 // number of steps launched. Defined in fused.hip.

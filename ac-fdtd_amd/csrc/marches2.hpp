@@ -391,9 +391,9 @@ inline int default_rows(const int groups, const void *kernel) {
 // not a multiple of the lane width or no segment fits.
 inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t,
                          const void *kernel = nullptr) {
-    if (d.nz % vec != 0) return false;
+    if (row_cells(d) % vec != 0) return false;
     t.vec = vec;
-    t.groups = d.nz / vec;
+    t.groups = row_cells(d) / vec;
     int rows = d.tile_j > 0 ? 2 * d.tile_j : 0;
     if (rows == 0 && kernel != nullptr) rows = default_rows(t.groups, kernel);
     if (rows == 0) {
@@ -430,15 +430,19 @@ struct LaneConsts {
     const real *src_now;  // the sources' samples of the sweep's two steps, two per source
     int ev_lo, ev_hi;     // planes of the lane's sources and receivers (empty: lo > hi)
     int j, k0, jk0;
+    int ztop;      // the lane's first padding cell, its upper wall face (padded rows; else >= V)
+    bool partial;  // padded rows: the lane holds padding (ztop < V)
     bool has_ylo, has_yhi, has_zlo, has_zhi, yz_boundary, special;
-    int64_t dy_lo, dy_hi, dz_lo, dz_hi;
+    int64_t dy_lo, dy_hi, dz_lo, dz_hi, dz_top;
     real fyc, fyf_lo, fyf_hi, fzc[V], fzf[V + 1];
     int code_yz[V];
     bool wall_in[V];
     real fu_in[V], fp_in[V];
 };
 
-template <typename real, int V, bool WALLS, bool LAYER>
+// PADDED: rows may be padded (see fused_march_vec): a lane may end in padding cells, whose
+// sponge factors are read clamped to the room's profiles.
+template <typename real, int V, bool WALLS, bool LAYER, bool PADDED = false>
 __device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int j, const int k0,
                                             const int32_t *src_cell, const int64_t *src_len,
                                             const real *src_now, const int32_t *rx_cell,
@@ -450,15 +454,18 @@ __device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int 
     c.rx_cell = rx_cell;
     c.j = j;
     c.k0 = k0;
-    c.jk0 = j * nz + k0;
+    c.jk0 = j * a.zp + k0;
+    c.ztop = nz - k0;
+    c.partial = PADDED && c.ztop < V;
     c.has_ylo = j > 0;
     c.has_yhi = j + 1 < ny;
     c.has_zlo = k0 > 0;
     c.has_zhi = k0 + V < nz;
-    c.dy_lo = c.has_ylo ? nz : 0;
-    c.dy_hi = c.has_yhi ? nz : 0;
+    c.dy_lo = c.has_ylo ? a.zp : 0;
+    c.dy_hi = c.has_yhi ? a.zp : 0;
     c.dz_lo = c.has_zlo ? 1 : 0;
     c.dz_hi = c.has_zhi ? V : V - 1;
+    c.dz_top = c.partial ? V - 1 : V;
     c.fyc = 1;
     c.fyf_lo = 1;
     c.fyf_hi = 1;
@@ -471,9 +478,9 @@ __device__ __forceinline__ void lane_consts(const SweepArgs<real> &a, const int 
         c.fyf_lo = a.face[1][j];
         c.fyf_hi = a.face[1][j + 1];
 #pragma unroll
-        for (int e = 0; e < V; ++e) c.fzc[e] = a.cell[2][k0 + e];
+        for (int e = 0; e < V; ++e) c.fzc[e] = a.cell[2][PADDED ? min(k0 + e, nz - 1) : k0 + e];
 #pragma unroll
-        for (int e = 0; e <= V; ++e) c.fzf[e] = a.face[2][k0 + e];
+        for (int e = 0; e <= V; ++e) c.fzf[e] = a.face[2][PADDED ? min(k0 + e, nz) : k0 + e];
     }
     c.yz_boundary = !c.has_ylo || !c.has_yhi || !c.has_zlo || !c.has_zhi;
 #pragma unroll
@@ -532,7 +539,7 @@ struct PlaneState {
 // the new pressure after walls, sponge and sources of `step`, and the boundary tape's value
 // p_tmp + p_old (the pressure before the wall closed form plus the old pressure).
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool XWALL,
-          bool PLAINXY = false>
+          bool PLAINXY = false, bool PADDED = false>
 __device__ __forceinline__ void advance_cells(
     const SweepArgs<real> &a, const LaneConsts<real, V> &c, const int i, const int64_t step,
     const real (&p_c)[V], const real (&p_up)[V], const real (&q_lo)[V], const real (&q_hi)[V],
@@ -562,8 +569,12 @@ __device__ __forceinline__ void advance_cells(
         const real wh = c.has_yhi ? vy_hi_in[e] - vel * (q_hi[e] - p_c[e]) : vy_hi_in[e];
         o.vy_hi[e] = damp(wh, fxc, c.fyf_hi, c.fzc[e]);
         real wz;
-        if (e > 0) wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
-        else wz = c.has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+        if (e > 0) {
+            wz = vz_v[e] - vel * (p_c[e] - p_c[e - 1]);
+            if (PADDED && e == c.ztop) wz = vz_v[e];  // padded rows: the upper wall face
+        } else {
+            wz = c.has_zlo ? vz_v[0] - vel * (p_c[0] - pz_lo) : vz_v[0];
+        }
         o.vz[e] = damp(wz, fxc, c.fyc, c.fzf[e]);
     }
     {
@@ -722,7 +733,9 @@ constexpr int stage_bytes() {
     return staged_fields<NR>() * kStageField + staged_words<real, V>() * kStageWord;
 }
 
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED>
+// PADDED: rows may be padded (z_pitch), see fused_march_vec; not with STAGED or a tape.
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED,
+          bool PADDED = false>
 __device__ __forceinline__ void fused2_march(
     const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
     const int k0, const bool valid, const bool owner, const int t, const int64_t step,
@@ -734,13 +747,15 @@ __device__ __forceinline__ void fused2_march(
     const real *src_now, const int32_t *rx_cell, unsigned char *__restrict__ stage) {
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
-    const int nx = a.nx, nz = a.nz;
-    const Layout L(nx, a.ny, nz, j, k0);
+    static_assert(!(PADDED && STAGED), "the staged sweep takes unpadded rows");
+    const int nx = a.nx, zp = a.zp;  // zp: the row length (the room's nz unless padded)
+    const Layout L = room_layout(a, j, k0);
     const int64_t n_cells = L.plane * nx;
     const real vel = a.vel;
     const int G = t2.pitch, T = t2.threads;  // G: lanes between y neighbours
     LaneConsts<real, V> c;
-    if (valid) lane_consts<real, V, WALLS, LAYER>(a, j, k0, src_cell, src_len, src_now, rx_cell, c);
+    if (valid)
+        lane_consts<real, V, WALLS, LAYER, PADDED>(a, j, k0, src_cell, src_len, src_now, rx_cell, c);
     // The lane constants' loads complete here, on every path into the march. Lanes that skip
     // the prologue would otherwise reach the loop with those loads counted as outstanding,
     // and the compiler waits for them inside the loop with a memory count of zero, which on
@@ -793,12 +808,12 @@ __device__ __forceinline__ void fused2_march(
         vload<real, V>(p_in + cc + (up_interior ? L.plane : 0), in.p_up);
         const int64_t cy = L.face_y(s);
         vload<real, V>(vy_in + cy, in.vy_lo);
-        vload<real, V>(vy_in + cy + nz, in.vy_hi);
+        vload<real, V>(vy_in + cy + zp, in.vy_hi);
         vload<real, V>(p_in + cc - c.dy_lo, in.q_lo);
         vload<real, V>(p_in + cc + c.dy_hi, in.q_hi);
         const int64_t cz = L.face_z(s);
         vload<real, V>(vz_in + cz, in.vz_v);
-        in.vz_top = vz_in[cz + V];
+        in.vz_top = vz_in[cz + (PADDED ? c.dz_top : V)];
         in.pz_lo = p_in[cc - c.dz_lo];
         in.pz_hi = p_in[cc + c.dz_hi];
 #pragma unroll
@@ -822,7 +837,7 @@ __device__ __forceinline__ void fused2_march(
             load_to_lds16(p_in + cc + (up_interior ? L.plane : 0), base + kStageField);
             const int64_t cy = L.face_y(s);
             load_to_lds16(vy_in + cy, base + 2 * kStageField);
-            load_to_lds16(vy_in + cy + nz, base + 3 * kStageField);
+            load_to_lds16(vy_in + cy + zp, base + 3 * kStageField);
             load_to_lds16(p_in + cc - c.dy_lo, base + 4 * kStageField);
             load_to_lds16(p_in + cc + c.dy_hi, base + 5 * kStageField);
 #pragma unroll
@@ -890,7 +905,8 @@ __device__ __forceinline__ void fused2_march(
         constexpr bool XWALL = decltype(xwall_tag)::value;
         real tape_value[V];
         auto run = [&](auto plain_tag) __attribute__((always_inline)) {
-            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value>(
+            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value,
+                          PADDED>(
                 a, c, s, step, pn_c, in.p_up, in.q_lo, in.q_hi, in.pz_lo, in.pz_hi, s1_vx_lo,
                 in.vx_v, in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value);
         };
@@ -947,7 +963,8 @@ __device__ __forceinline__ void fused2_march(
         PlaneState<real, V, NR> o;
         real tape_value[V];
         auto run = [&](auto plain_tag) __attribute__((always_inline)) {
-            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value>(
+            advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value,
+                          PADDED>(
                 a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi,
                 A.vy_lo, A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
         };
@@ -957,14 +974,14 @@ __device__ __forceinline__ void fused2_march(
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
         vstore<real, V>(vy_out + cy, o.vy_lo);
-        if (!c.has_yhi) vstore<real, V>(vy_out + cy + nz, o.vy_hi);
+        if (!c.has_yhi) vstore<real, V>(vy_out + cy + zp, o.vy_hi);
         {
             real lower[V];
 #pragma unroll
             for (int e = 0; e < V; ++e) lower[e] = o.vz[e];
             vstore<real, V>(vz_out + cz, lower);
         }
-        if (!c.has_zhi) vz_out[cz + V] = o.vz[V];
+        if (!c.has_zhi && !(PADDED && c.partial)) vz_out[cz + V] = o.vz[V];
 #pragma unroll
         for (int r = 0; r < NR; ++r) vstore<real, V>(psi_out + (int64_t)r * n_cells + cc, o.psi[r]);
         vstore<real, V>(p_out + cc, o.p);

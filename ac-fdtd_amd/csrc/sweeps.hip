@@ -194,24 +194,35 @@ VelocityKernel<real> pick_velocity(bool layer, bool comp) {
 }
 
 template <typename real>
-int run_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
-              hipStream_t stream) {
+int inplace_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
+                  hipStream_t stream) {
     SweepArgs<real> a;
     fill_args(d, a);
     const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
     const bool walls = d.wall_mask != 0, layer = d.has_layer != 0;
-    int64_t n = 0;
-    if (n_steps > 3) n = launch_fused2_quads<real>(d, step_offset, n_steps, column, stream);
-    if (has_swap(d) && n_steps - n > 1)
-        n += launch_fused_pairs<real>(d, step_offset + n, n_steps - n, column + n, stream);
     const dim3 block(64, a.tiles.tile_j);
     const dim3 grid(a.tiles.launched);
     const VelocityKernel<real> vk = pick_velocity<real>(layer, comp);
     const PressureKernel<real> pk = pick_pressure<real>(d.n_relax, walls, layer, comp);
-    for (; n < n_steps; ++n) {
+    for (int64_t n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(vk, grid, block, 0, stream, a);
         hipLaunchKernelGGL(pk, grid, block, 0, stream, a, step_offset + n, column + n);
     }
+    return check_hip(hipGetLastError(), "kernel launch");
+}
+
+template int inplace_steps<float>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t, hipStream_t);
+template int inplace_steps<double>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
+                                   hipStream_t);
+
+template <typename real>
+int run_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
+              hipStream_t stream) {
+    int64_t n = 0;
+    if (n_steps > 3) n = launch_fused2_quads<real>(d, step_offset, n_steps, column, stream);
+    if (has_swap(d) && n_steps - n > 1)
+        n += launch_fused_pairs<real>(d, step_offset + n, n_steps - n, column + n, stream);
+    if (n < n_steps) return inplace_steps<real>(d, step_offset + n, n_steps - n, column + n, stream);
     return check_hip(hipGetLastError(), "kernel launch");
 }
 

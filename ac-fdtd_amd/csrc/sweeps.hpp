@@ -56,19 +56,23 @@ inline Tiling tiling_for(const acfdtd_hip_desc &d) {
     return make_tiling(d.nx, d.ny, d.nz, d.tile_j, d.chunk_x);
 }
 
+// Elements per row of the cell fields: nz, or the padded row length z_pitch.
+inline int row_cells(const acfdtd_hip_desc &d) { return d.z_pitch > 0 ? d.z_pitch : d.nz; }
+
 // z cells per lane of the fused step: as many as the rows allow (4 bytes x vec per access),
-// 4 for fp32 and 2 for fp64 when nz is a multiple, else fewer.
+// 4 for fp32 and 2 for fp64 when the row length is a multiple, else fewer. With padded rows the
+// last lane of a row may hold fewer cells of the room than its width (see lane_consts).
 inline int fused_vec(const acfdtd_hip_desc &d) {
     const int widest = d.real_bytes == 4 ? 4 : 2;
     for (int v = widest; v > 1; v /= 2)
-        if (d.nz % v == 0) return v;
+        if (row_cells(d) % v == 0) return v;
     return 1;
 }
 
 // Tiling of the fused kernels: the (y, z / vec) plane is flattened and cut into workgroups of
 // 64 * tile_j lanes (see flat_lane), so nblk_k is 1 and nblk_j counts those workgroups.
 inline Tiling flat_tiling(const acfdtd_hip_desc &d, int vec) {
-    const int64_t lanes = (int64_t)d.ny * (d.nz / vec);
+    const int64_t lanes = (int64_t)d.ny * (row_cells(d) / vec);
     const int tile_j = d.tile_j > 0 ? d.tile_j : 4;
     // Four planes per march measured fastest at 256^3 and 384^3 (fewer halo re-reads pay less
     // than the extra workgroups gain).
@@ -104,6 +108,15 @@ struct Layout {
           jk_z((int64_t)j * (nz + 1) + k) {
         (void)nx;
     }
+    // Rows of zp cells (p, psi, vx, vy) and of zpz faces (vz): the layout above when zp = nz and
+    // zpz = nz + 1, a padded one when both are the row length z_pitch.
+    struct Rows {};
+    __device__ Layout(Rows, int ny, int zp, int zpz, int j, int k)
+        : plane((int64_t)ny * zp),
+          plane_y((int64_t)(ny + 1) * zp),
+          plane_z((int64_t)ny * zpz),
+          jk((int64_t)j * zp + k),
+          jk_z((int64_t)j * zpz + k) {}
     __device__ int64_t cell(int i) const { return (int64_t)i * plane + jk; }
     __device__ int64_t face_y(int i) const { return (int64_t)i * plane_y + jk; }
     __device__ int64_t face_z(int i) const { return (int64_t)i * plane_z + jk_z; }
@@ -126,6 +139,8 @@ template <typename real>
 struct SweepArgs {
     Tiling tiles;
     int nx, ny, nz;
+    // Row lengths: cells (p, psi, vx, vy) and vz faces; nz and nz + 1 unless rows are padded.
+    int zp, zpz;
     int n_sources, n_receivers;
     real vel, pres, src, relax_scale;
     real decay[ACFDTD_HIP_MAX_RELAXATION];
@@ -150,6 +165,12 @@ struct SweepArgs {
     // lane's V cells of an x or y face row move as one vector access.
     int wall_tape_rows16;
 };
+
+// The layout of a room's fields from its arguments (padded rows or not).
+template <typename real>
+__device__ inline Layout room_layout(const SweepArgs<real> &a, int j, int k) {
+    return Layout(Layout::Rows{}, a.ny, a.zp, a.zpz, j, k);
+}
 
 // Offsets of the six face planes in one row of the boundary tape, and the row length.
 struct TapePlanes {
@@ -200,6 +221,8 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
     a.nx = d.nx;
     a.ny = d.ny;
     a.nz = d.nz;
+    a.zp = row_cells(d);
+    a.zpz = d.z_pitch > 0 ? d.z_pitch : d.nz + 1;
     a.n_sources = d.n_sources;
     a.n_receivers = d.n_receivers;
     a.vel = (real)d.vel_coef;
@@ -237,7 +260,9 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
                          d.wall_tape_stride % 4 == 0;
 }
 
-inline int validate(const acfdtd_hip_desc *d) {
+// `padded`: the caller's kernels take padded rows (z_pitch > 0); the other entry points refuse
+// them.
+inline int validate(const acfdtd_hip_desc *d, const bool padded = false) {
     if (d == nullptr) return fail(ACFDTD_HIP_EINVAL, "descriptor is NULL");
     if (d->abi_version != ACFDTD_HIP_ABI_VERSION)
         return fail(ACFDTD_HIP_EABI, "descriptor ABI version " + std::to_string(d->abi_version) +
@@ -248,7 +273,15 @@ inline int validate(const acfdtd_hip_desc *d) {
                                          " bytes, library expects " +
                                          std::to_string(sizeof(acfdtd_hip_desc)));
     if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(ACFDTD_HIP_EINVAL, "cell counts must be >= 1");
-    if ((int64_t)d->ny * d->nz > INT32_MAX)
+    if (d->reserved != 0) return fail(ACFDTD_HIP_EINVAL, "reserved descriptor field must be 0");
+    if (d->z_pitch != 0) {
+        if (!padded)
+            return fail(ACFDTD_HIP_EINVAL, "padded rows (z_pitch) are taken by the batch API only");
+        if (d->z_pitch <= d->nz)
+            return fail(ACFDTD_HIP_EINVAL, "z_pitch must be 0 or greater than nz");
+        if (d->wall_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no wall tape");
+    }
+    if ((int64_t)d->ny * row_cells(*d) > INT32_MAX)
         return fail(ACFDTD_HIP_EINVAL, "ny * nz must fit in 31 bits");
     if (d->real_bytes != 4 && d->real_bytes != 8)
         return fail(ACFDTD_HIP_EINVAL, "real_bytes must be 4 or 8");

@@ -267,6 +267,7 @@ def batched_rooms(n_rooms, steps, warmup, check=True, seed=3):
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop)
     groups = len(batch._groups)
+    padded = sum(1 for g in batch._groups for r in g._padded if r is not None)
     batch.close()
     recorded = [r.recorded_pressure.astype(np.float64) for r in rooms]
     out = {
@@ -274,7 +275,8 @@ def batched_rooms(n_rooms, steps, warmup, check=True, seed=3):
         "cells": cells,
         "cells_per_axis_range": [int(min(min(s) for s in shapes)), int(max(max(s) for s in shapes))],
         "steps": steps,
-        "lane_width_groups": groups,
+        "launch_groups": groups,
+        "rooms_in_padded_rows": padded,
         "gcells": round(cells * steps / (ms * 1e-3) / 1e9, 3),
         "ms_per_step": round(ms / steps, 4),
     }

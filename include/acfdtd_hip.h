@@ -129,6 +129,19 @@ typedef struct acfdtd_hip_desc {
      * multiple of the lane width (4 cells fp32, 2 fp64); the remaining steps run one per
      * sweep. Results (boundary tape included) are bit-identical either way. */
     void *psi_swap;
+
+    /* Padded rows, accepted by the batch API only (the single-room entry points return
+     * ACFDTD_HIP_EINVAL unless it is 0). 0: the layout above. Otherwise z_pitch > nz is the row
+     * length, in elements, of every field and swap buffer, vz included: p and each psi field
+     * nx*ny*z_pitch, vx (nx+1)*ny*z_pitch, vy nx*(ny+1)*z_pitch, vz nx*ny*z_pitch, cell (i, j, k)
+     * at (i*ny + j)*z_pitch + k, vz face k of row (i, j) at (i*ny + j)*z_pitch + k; source and
+     * receiver cells are given as j * z_pitch + k. The elements past the room's own (k >= nz,
+     * k >= nz + 1 for vz) belong to no cell: the sweeps read and write them but never let them
+     * reach a cell, so their contents are arbitrary. It lets a room whose nz is not a multiple
+     * of the lane width (4 cells fp32, 2 fp64) take full-width lanes: z_pitch a multiple of
+     * that width. */
+    int32_t z_pitch;
+    int32_t reserved; /* 0 */
 } acfdtd_hip_desc;
 
 /* sizeof(acfdtd_hip_desc), for the caller's mirror check. Replaces acfdtd_struct_size
@@ -175,7 +188,10 @@ int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step, int64_t re
  * buffers, else two sweeps): a tile table covering every room is built once in a caller-owned
  * device workspace, with each room's arguments beside it. Rooms must share real_bytes,
  * n_relax, has_layer, flags, tile_j and whether swap buffers are given; sizes, walls, air,
- * sources and receivers may differ. No reference counterpart (the reference runs one room). */
+ * sources and receivers may differ. The lanes are as wide as every room's row length allows
+ * (4 cells fp32, 2 fp64 at most): a room whose nz is not a multiple of that width can be given
+ * in padded rows (z_pitch), so that it does not narrow the whole batch's lanes. No reference
+ * counterpart (the reference runs one room). */
 typedef struct acfdtd_hip_batch acfdtd_hip_batch; /* opaque host handle */
 
 size_t acfdtd_hip_batch_workspace_bytes(const acfdtd_hip_desc *rooms, int32_t n_rooms);

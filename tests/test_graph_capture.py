@@ -64,7 +64,7 @@ def test_captured_batch_replays_the_eager_batch(dtype):
     fields and samples, also after an eager run has re-pointed the room tables."""
     from ac_fdtd_amd.batch import BatchedRooms
 
-    shapes = [(14, 11, 16), (9, 13, 10), (12, 7, 9)]  # lane widths 4, 2, 1 (fp32)
+    shapes = [(14, 11, 16), (9, 13, 10), (12, 7, 9)]  # fp32: the last two in padded rows
     eager = [_solver(shape, dtype, 2) for shape in shapes]
     graph = [_solver(shape, dtype, 2) for shape in shapes]
     eager_batch, graph_batch = BatchedRooms(eager), BatchedRooms(graph)

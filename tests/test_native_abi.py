@@ -80,6 +80,8 @@ def _desc(**overrides):
         ({"has_layer": 1}, EINVAL, "sponge profile"),
         ({"n_sources": 1}, EINVAL, "source arrays"),
         ({"n_receivers": 2}, EINVAL, "receiver arrays"),
+        ({"z_pitch": 8}, EINVAL, "batch API only"),
+        ({"reserved": 1}, EINVAL, "reserved"),
     ],
 )
 def test_malformed_descriptors_are_refused(lib, overrides, code, message):
@@ -119,6 +121,23 @@ def test_short_wall_tapes_are_refused_before_any_launch(lib):
         assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(desc), step, 0, None) == EINVAL
     assert lib.acfdtd_hip_run(ctypes.byref(desc), 11, 4, 0, None) == EINVAL
     assert "wall tape" in lib.acfdtd_hip_last_error().decode()
+
+
+def test_padded_rows_are_checked_by_the_batch_api(lib):
+    """z_pitch (padded rows) is a batch-only layout: it must exceed nz and takes no wall tape;
+    a valid one is accepted, and a padded room takes lanes as wide as the pitch allows."""
+    from ac_fdtd_amd.batch import _bind
+
+    _bind(lib)
+    swaps = {"p_swap": 4096, "vx_swap": 4096, "vy_swap": 4096, "vz_swap": 4096}
+    rooms = (_native.Desc * 2)(_desc(nz=4, **swaps), _desc(nz=5, z_pitch=8, **swaps))
+    assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 2) > 0
+    for bad, message in (({"z_pitch": 5}, "greater than nz"), ({"z_pitch": 3}, "greater than nz"),
+                         ({"z_pitch": 8, "wall_tape": 4096, "wall_tape_stride": 1000,
+                           "wall_tape_rows": 1}, "no wall tape")):
+        rooms[1] = _desc(nz=5, **swaps, **bad)
+        assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 2) == 0
+        assert message in lib.acfdtd_hip_last_error().decode()
 
 
 def test_energy_without_allocation_refuses_grids_beyond_its_scratch(lib):

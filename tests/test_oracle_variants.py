@@ -9,8 +9,9 @@ prefetch when there is no sponge), nz = 14 gives 2-cell lanes in both precisions
 single-cell lanes. Each runs one step per sweep, two steps per sweep, and two steps per sweep
 staged through LDS, on the reference's five parity configurations (rigid, walls, sponge, air,
 all of them), plus the bench's own tile shape (6 rows of 64 lane groups, 24-plane marches) on
-a slab of the 256^3 workload's rows. Batched rooms (the batch's two-step kernel on uneven rooms
-of every lane width) are compared room by room with their own oracles.
+a slab of the 256^3 workload's rows. Batched rooms (the batch's two-step kernel on uneven rooms,
+those whose nz the widest lane does not divide in padded rows) are compared room by room with
+their own oracles.
 """
 
 from __future__ import annotations
@@ -142,7 +143,7 @@ def test_fp32_four_cell_lanes_against_fp64(compensated):
     assert error < 1e-4
 
 
-# Uneven rooms: nz multiples of 4, of 2 only and odd (the batch's three lane-width groups).
+# Uneven rooms: nz multiples of 4, of 2 only and odd (fp32: the last two in padded rows).
 BATCH_SHAPES = [(23, 17, 12), (9, 31, 22), (16, 8, 36), (30, 21, 9), (12, 14, 44), (7, 19, 15)]
 
 
