@@ -98,6 +98,9 @@ class Desc(_c.Structure):
         ("psi_swap", _ptr),
         ("z_pitch", _c.c_int32),
         ("reserved", _c.c_int32),
+        ("div_tape", _ptr),
+        ("div_tape_step0", _c.c_int64),
+        ("div_tape_rows", _c.c_int64),
     ]
 
 

@@ -15,10 +15,17 @@ forward state (the step is linear in it); the parameter gradients do:
 * walls need ``p_tmp + p_old`` on the boundary only, which the forward kernels write to a tape
   of ``2 (ny nz + nx nz + nx ny)`` values per step, so wall and source gradients cost one
   forward pass plus one adjoint pass;
-* the sponge and air strengths need the forward fields at every step. The forward pass then
-  keeps a checkpoint of the full state every ``checkpoint_every`` steps, and the backward pass
-  recomputes each segment from its checkpoint, keeping that segment's fields, before running
-  the adjoint through it.
+* the air strengths need every cell's velocity divergence at every step (one value per cell
+  and step). The forward kernels write it to a divergence tape when ``n_steps`` rows of it fit
+  in ``history_bytes``, and the adjoint reads it, two steps per sweep like the wall gradient:
+  one forward and one adjoint pass again. Beyond that budget the forward pass keeps a
+  checkpoint of the full state every ``checkpoint_every`` steps, and the backward pass
+  recomputes each segment's divergences (two steps per sweep) into a ring of
+  ``checkpoint_every`` rows before running the adjoint through it;
+* the sponge strength needs the forward fields at every step: checkpoints every
+  ``checkpoint_every`` steps, and each segment's fields recomputed into a ring of states before
+  the adjoint (one step per sweep) runs through it; requested together with the sponge, the
+  air gradients take the same path.
 
 There is no reference counterpart: the reference's PyTorch backend uses ``out=`` operations and
 host-float coefficients and cannot be differentiated (``src/ac_fdtd/torch_backend.py:149-195``,
@@ -55,6 +62,7 @@ class GradientReport:
     checkpoints: int = 0
     checkpoint_bytes: int = 0
     tape_bytes: int = 0
+    div_tape_bytes: int = 0  # divergence rows held for the air gradients (tape or ring)
 
 
 class RoomResponse:
@@ -67,8 +75,13 @@ class RoomResponse:
         n_steps: steps per evaluation.
         sources: ``[(point, signal), ...]``; the signals are the defaults for ``signal``.
         receivers: receiver points.
-        checkpoint_every: state checkpoint interval K, used when sponge or air gradients are
-            requested (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field sets).
+        checkpoint_every: state checkpoint interval K, used when the sponge gradient is
+            requested (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field sets), and
+            for the air gradients beyond ``history_bytes`` (``ceil(n_steps / K)`` full states
+            plus ``K`` divergence rows).
+        history_bytes: device memory the forward pass may keep for the air gradients: a
+            divergence tape of ``n_steps * n_cells`` values when it fits (default 64 GiB, of
+            the MI355X's 288 GB), else checkpoints and recomputation.
         autotune: pick the tilings of the forward and of the adjoint step by timing them
             (:mod:`ac_fdtd_amd.autotune`); gradients then differ from an untuned run only in
             the summation order of their fp64 partials.
@@ -91,6 +104,7 @@ class RoomResponse:
         air_absorption: AirAbsorption | None = None,
         device="cuda",
         checkpoint_every: int = 100,
+        history_bytes: int = 64 << 30,
         fused: bool = True,
         compensated: bool = False,
         autotune: bool = False,
@@ -110,6 +124,7 @@ class RoomResponse:
         self.air_absorption = air_absorption
         self.device = device
         self.checkpoint_every = int(checkpoint_every)
+        self.history_bytes = int(history_bytes)
         self.fused, self.compensated = fused, compensated
         self.autotune = autotune
         if time_block not in (1, 2):
@@ -217,7 +232,16 @@ class _RoomResponseFunction(torch.autograd.Function):
             solver.attach_wall_tape(tape, 0)
             report.tape_bytes = tape.numel() * tape.element_size()
         checkpoints = []
-        if want_sponge or want_air:
+        div_tape = None
+        tape_bytes = room.n_steps * room.grid.n_cells * solver.np_dtype.itemsize
+        if want_air and not want_sponge and tape_bytes <= room.history_bytes:
+            # Every step's divergences, for the adjoint to read: no recomputation.
+            div_tape = torch.empty((room.n_steps, room.grid.n_cells), dtype=solver.dtype,
+                                   device=solver.device)
+            solver.attach_div_tape(div_tape, 0)
+            solver.run(room.n_steps)
+            report.div_tape_bytes = tape_bytes
+        elif want_sponge or want_air:
             K = room.checkpoint_every
             for start in range(0, room.n_steps, K):
                 checkpoints.append(solver.save_state())
@@ -271,8 +295,11 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
     ldiv = torch.zeros(g.shape, **opts)
     G = grad_recording.to(**opts).contiguous()
     recording = solver.recording_tensor()
+    # Air gradients from divergences (a tape of every step, or a ring per recomputed segment)
+    # unless the sponge gradient's forward states are recomputed anyway.
+    divergences = want_air and not want_sponge
     fwd = solver._descriptor(recording)
-    two_step = (room.fused and room.time_block == 2 and not (want_sponge or want_air)
+    two_step = (room.fused and room.time_block == 2 and not want_sponge
                 and solver.adjoint_two_step_applies())
     if room.autotune:
         from .autotune import adjoint_tiling
@@ -325,6 +352,13 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
             adj.lp_swap = swaps[4].data_ptr()
             adj.lpsi_swap = swaps[5].data_ptr() if k.n_relax else None
             adj.time_block = 2
+    div_ring = None
+    if checkpoints and divergences:
+        # One ring of K divergence rows, reused by every segment: row q = step start + q.
+        div_ring = torch.empty((room.checkpoint_every, g.n_cells), **opts)
+        report.div_tape_bytes = div_ring.numel() * div_ring.element_size()
+        fwd.div_tape, fwd.div_tape_step0, fwd.div_tape_rows = (div_ring.data_ptr(), 0,
+                                                               room.checkpoint_every)
     report.adjoint_two_step = bool(lib.acfdtd_hip_adjoint_two_step(ctypes.byref(fwd), ctypes.byref(adj)))
 
     stream = torch.cuda.current_stream(solver.device).cuda_stream
@@ -354,7 +388,29 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
                                                  step_lo, handle), "acfdtd_hip_adjoint_run")
 
     with torch.cuda.device(solver.device):
-        if checkpoints:
+        if div_ring is not None:
+            K = room.checkpoint_every
+            ring = div_ring
+            use_ring(None, 0, 0)
+            last = len(checkpoints) - 1
+            for seg in range(last, -1, -1):
+                start, stop = seg * K, min(seg * K + K, n_steps)
+                solver.load_state(checkpoints[seg], start)
+                solver.attach_div_tape(ring, start)
+                solver.run(stop - start, record=False)
+                report.recomputed_steps += stop - start
+                fwd.div_tape, fwd.div_tape_step0, fwd.div_tape_rows = ring.data_ptr(), start, K
+                # Adjoint step m reads the divergences of step m - 1: this segment's rows serve
+                # steps start + 1 .. stop (and the last step's local part).
+                if seg == last:
+                    last_local(None, 0)
+                hi = n_steps - 1 if seg == last else stop
+                lo = start + 1 if start > 0 else 0
+                if hi >= lo:
+                    adjoint_steps(hi, lo)
+            solver.attach_div_tape(None)
+            del ring, div_ring
+        elif checkpoints:
             K = room.checkpoint_every
             # One ring of K + 1 states, reused by every segment: slot q = step start + q.
             ring = [torch.empty((K + 1, *g.shape), **opts)] + [

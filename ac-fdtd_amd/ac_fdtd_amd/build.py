@@ -43,6 +43,8 @@ SOURCES = (
     "adjoint2_64.hip",
     "fused2s.hip",
     "fused2s_64.hip",
+    "fused2d.hip",
+    "fused2d_64.hip",
     "fused2_v2.hip",
     "fused2_v1.hip",
     "fused2_64_v1.hip",

@@ -159,6 +159,7 @@ class AcousticFDTD:
             self._wall_table = torch.as_tensor(table, **opts)
             self._wall_mask = int(sum(1 << c for c in range(64) if k.wall_is_wall[c]))
         self._wall_tape: tuple[torch.Tensor, int] | None = None
+        self._div_tape: tuple[torch.Tensor, int] | None = None
         self._cell = self._face = None
         if k.has_layer:
             self._cell = [torch.as_tensor(x, **opts) for x in k.cell_profiles]
@@ -252,14 +253,15 @@ class AcousticFDTD:
         """Advance ``n_steps`` on the current stream (asynchronous, like any GPU launch).
 
         ``record=False`` skips the receivers (and an attached boundary tape) for these steps,
-        as a recomputation from a checkpoint does."""
+        as a recomputation from a checkpoint does; an attached divergence tape is written
+        either way."""
         n_steps = int(n_steps)
         if n_steps <= 0:
             return
         recording = None
         if self._receivers and record:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
-        desc = self._descriptor(recording, tape=record)
+        desc = self._descriptor(recording, tape=record, div=True)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         with torch.cuda.device(self.device):
             status = self._lib.acfdtd_hip_run(
@@ -437,6 +439,20 @@ class AcousticFDTD:
                 raise ValueError(f"wall tape must be {self.dtype} on {self.device}")
         self._wall_tape = None if tape is None else (tape, int(step0))
 
+    def attach_div_tape(self, tape: torch.Tensor | None, step0: int = 0) -> None:
+        """Write every cell's velocity divergence of each step ``n`` (the one its pressure
+        update uses) to row ``n - step0`` of ``tape``, a contiguous ``(rows, nx * ny * nz)``
+        tensor of the working precision; None detaches. The air-strength gradients read it
+        (:mod:`ac_fdtd_amd.autograd`). Needs air absorption."""
+        if tape is not None:
+            if not self.coefficients.n_relax:
+                raise ValueError("a divergence tape needs air absorption (relaxation states)")
+            if tape.dim() != 2 or tape.shape[1] != self.grid.n_cells or not tape.is_contiguous():
+                raise ValueError("divergence tape must be a contiguous (rows, n_cells) tensor")
+            if tape.dtype != self.dtype or tape.device != self.device:
+                raise ValueError(f"divergence tape must be {self.dtype} on {self.device}")
+        self._div_tape = None if tape is None else (tape, int(step0))
+
     def wall_tape_stride(self) -> int:
         return int(self._lib.acfdtd_hip_wall_tape_stride(*self.grid.shape))
 
@@ -450,7 +466,10 @@ class AcousticFDTD:
             (len(self._receivers), 0), **self._opts
         )
 
-    def _descriptor(self, recording: torch.Tensor | None, tape: bool = True) -> _native.Desc:
+    def _descriptor(self, recording: torch.Tensor | None, tape: bool = True,
+                    div: bool | None = None) -> _native.Desc:
+        """The C-ABI descriptor of the current state; ``tape`` includes the boundary tape,
+        ``div`` (default: as ``tape``) the divergence tape."""
         g, k = self.grid, self.coefficients
         d = _native.Desc()
         d.abi_version = _native.ABI_VERSION
@@ -502,6 +521,11 @@ class AcousticFDTD:
             d.wall_tape_stride = buffer.shape[1]
             d.wall_tape_step0 = step0
             d.wall_tape_rows = buffer.shape[0]
+        if (tape if div is None else div) and self._div_tape is not None:
+            buffer, step0 = self._div_tape
+            d.div_tape = buffer.data_ptr()
+            d.div_tape_step0 = step0
+            d.div_tape_rows = buffer.shape[0]
         if self._swap is not None:
             d.p_swap = self._checked(self._swap[0], g.shape, "p swap")
             for a, name in enumerate(("vx_swap", "vy_swap", "vz_swap")):

@@ -145,12 +145,16 @@ __global__ __launch_bounds__(512) void adjoint_cell_sweep(const SweepArgs<real> 
     const bool active = tile && k < a.nz && j < a.ny;
     const bool walls = local && (g.grads & ACFDTD_HIP_GRAD_WALLS) && a.wall_tape;
     const bool sponge = local && LAYER && (g.grads & ACFDTD_HIP_GRAD_SPONGE) && g.fwd_p;
-    const bool air = local && NR > 0 && (g.grads & ACFDTD_HIP_GRAD_AIR) && g.fwd_vx;
+    // Air gradients: the divergence of step `step` from the divergence tape, else from the
+    // forward velocities v^{step+1/2}.
+    const bool air = local && NR > 0 && (g.grads & ACFDTD_HIP_GRAD_AIR) && (a.div_tape || g.fwd_vx);
     if (active) {
         const int i0 = xc * a.tiles.chunk;
         const int i1 = min(i0 + a.tiles.chunk, a.nx);
         const Layout L(a.nx, a.ny, a.nz, j, k);
         const int64_t n_cells = L.plane * a.nx;
+        const real *div_row =
+            a.div_tape ? a.div_tape + (step - a.div_tape_step0) * n_cells : nullptr;
         const int jk = j * a.nz + k;
         const int code_yz = (wall_class(j, a.ny) << 2) | (wall_class(k, a.nz) << 4);
         real fyc = 1, fzc = 1, wyz = 0;
@@ -224,10 +228,14 @@ __global__ __launch_bounds__(512) void adjoint_cell_sweep(const SweepArgs<real> 
             real ldv = ltil * cdiv;
             double div = 0;
             if (air) {
-                const int64_t cy = L.face_y(i), cz = L.face_z(i);
-                div = (double)(((g.fwd_vx[c + L.plane] - g.fwd_vx[c]) +
-                                (g.fwd_vy[cy + a.nz] - g.fwd_vy[cy])) +
-                               (g.fwd_vz[cz + 1] - g.fwd_vz[cz]));
+                if (div_row) {
+                    div = (double)div_row[c];
+                } else {
+                    const int64_t cy = L.face_y(i), cz = L.face_z(i);
+                    div = (double)(((g.fwd_vx[c + L.plane] - g.fwd_vx[c]) +
+                                    (g.fwd_vy[cy + a.nz] - g.fwd_vy[cy])) +
+                                   (g.fwd_vz[cz + 1] - g.fwd_vz[cz]));
+                }
                 acc[kGradScale] += div * (double)ltil;
             }
 #pragma unroll
@@ -403,6 +411,8 @@ extern "C" int acfdtd_hip_adjoint_cell_sweep(const acfdtd_hip_desc *desc,
         if ((adjoint->grads & ACFDTD_HIP_GRAD_WALLS) &&
             (step < desc->wall_tape_step0 || step - desc->wall_tape_step0 >= desc->wall_tape_rows))
             return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
+        if ((adjoint->grads & ACFDTD_HIP_GRAD_AIR) && !div_tape_covers(*desc, step, step))
+            return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover this step");
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8)
@@ -439,15 +449,22 @@ extern "C" int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc,
     }
     const bool sponge = (adjoint->grads & ACFDTD_HIP_GRAD_SPONGE) && desc->has_layer;
     const bool air = (adjoint->grads & ACFDTD_HIP_GRAD_AIR) && desc->n_relax > 0;
+    // The air gradients read the divergence tape when the forward descriptor has one, else
+    // (like the sponge gradient) the forward states.
+    const bool states = sponge || (air && desc->div_tape == nullptr);
     if (adjoint->fwd_count < 0) return fail(ACFDTD_HIP_EINVAL, "fwd_count must be >= 0");
-    if (sponge || air) {
+    if (states) {
         if (adjoint->fwd_count == 0 || !adjoint->fwd_vx || !adjoint->fwd_vy || !adjoint->fwd_vz ||
             (sponge && !adjoint->fwd_p))
-            return fail(ACFDTD_HIP_EINVAL, "sponge and air gradients need the forward states");
+            return fail(ACFDTD_HIP_EINVAL,
+                        "sponge gradients, and air gradients without a divergence tape, need the "
+                        "forward states");
         if (step_lo < adjoint->fwd_step0 ||
             step_hi + 1 - adjoint->fwd_step0 >= adjoint->fwd_count)
             return fail(ACFDTD_HIP_EINVAL, "forward states do not cover these steps");
     }
+    if (air && desc->div_tape && !div_tape_covers(*desc, local_lo, local_hi))
+        return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover these steps");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return adjoint_run<double>(*desc, *adjoint, step_hi, step_lo, s);

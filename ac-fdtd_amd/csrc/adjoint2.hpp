@@ -16,9 +16,11 @@
 // lp and lpsi in place).
 //
 // Every cell's arithmetic is adjoint_march_vec's term by term, so the adjoint fields and the
-// source-signal gradient are bit-identical to single adjoint steps; the fp64 wall-gradient
-// partials are summed in another order. Only the wall and signal gradients are carried: the
-// sponge and air gradients read the forward states of every step and stay on the single step.
+// source-signal gradient are bit-identical to single adjoint steps; the fp64 gradient partials
+// are summed in another order. The wall and signal gradients are carried, and with AIR the
+// air-strength gradients, whose forward input (every cell's divergence of every step) comes
+// from the forward pass's divergence tape; the sponge gradient reads the forward states of
+// every step and stays on the single step.
 
 #pragma once
 
@@ -166,8 +168,9 @@ __device__ __forceinline__ void copy_adj(AdjPlane<real, V, NR> &to, const AdjPla
 // old adjoint velocities, with the new lower x face lvx_lo; the stencil into lp; the local part
 // of `step` (receiver injection rx_now[2 r + h], the source-signal gradient, sponge, walls,
 // relaxation). `own` marks the cells whose gradients this lane adds and writes; tv holds the
-// boundary tape of `step` for the lane's wall cells.
-template <typename real, int V, int NR, bool LAYER, bool XWALL>
+// boundary tape of `step` for the lane's wall cells, dv (AIR) the forward divergences of `step`,
+// whose air-gradient terms go to air_acc (relax_scale, then one per relaxation state).
+template <typename real, int V, int NR, bool LAYER, bool XWALL, bool AIR = false>
 __device__ __forceinline__ void adjoint_cells(
     const SweepArgs<real> &a, const AdjLane<real, V> &c, const int i, const int64_t step,
     const int h, const bool own, const bool want_walls, real *grad_signal, const real (&tv)[V],
@@ -175,7 +178,8 @@ __device__ __forceinline__ void adjoint_cells(
     const real (&ld_yhi)[V], const real ldz_lo, const real ldz_hi, const real (&lvx_lo)[V],
     const real (&lvx_v)[V], const real (&lvy_lo_in)[V], const real (&lvy_hi_in)[V],
     const real (&lvz_v)[V], const real lvz_top, const real (&lp_in)[V],
-    const real (&states_in)[NR > 0 ? NR : 1][V], AdjPlane<real, V, NR> &o, double (&acc)[6]) {
+    const real (&states_in)[NR > 0 ? NR : 1][V], AdjPlane<real, V, NR> &o, double (&acc)[6],
+    const real (&dv)[V], double (&air_acc)[1 + (NR > 0 ? NR : 1)]) {
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
@@ -260,11 +264,16 @@ __device__ __forceinline__ void adjoint_cells(
         }
         lp_new = lp_new + ltil;
         real ldv = ltil * c.cdiv;
+        const double div = AIR ? (double)dv[e] : 0.0;
+        if (AIR && own) air_acc[0] += div * (double)ltil;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const real old = states_in[r][e];
             o.psi[r][e] = a.mean[r] * ltil + a.decay[r] * old;
             ldv = ldv + old * (a.gain[r] * (real(1) - a.decay[r]));
+            if (AIR && own)
+                air_acc[1 + r] += div * (-(double)a.mean[r] * (double)ltil +
+                                         (1.0 - (double)a.decay[r]) * (double)old);
         }
         o.lp[e] = lp_new;
         o.ld[e] = ldv;
@@ -300,13 +309,13 @@ __device__ __forceinline__ void load_tape(const SweepArgs<real> &a, const AdjLan
     }
 }
 
-template <typename real, int V, int NR, bool LAYER>
+template <typename real, int V, int NR, bool LAYER, bool AIR = false>
 __device__ __forceinline__ void adjoint2_march(
     const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
     const int k0, const bool valid, const bool owner, const int t, const int64_t m,
     const bool want_walls, real *grad_signal, const Adjoint2IO<real> &io, real *__restrict__ lds,
     const int32_t *src_cell, const int64_t *src_len, const int32_t *rx_cell, const real *rx_now,
-    const real *wtab, double (&acc)[6]) {
+    const real *wtab, double (&acc)[6], double (&air_acc)[1 + (NR > 0 ? NR : 1)]) {
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
     const int nx = a.nx, nz = a.nz;
@@ -370,10 +379,15 @@ __device__ __forceinline__ void adjoint2_march(
         else
 #pragma unroll
             for (int e = 0; e < V; ++e) tv[e] = 0;
-        adjoint_cells<real, V, NR, LAYER, XWALL>(a, c, s, m - 1, 0, own, want_walls, grad_signal,
-                                                 tv, ld_a, ld_up, ld_ylo, ld_yhi, ldz_lo, ldz_hi,
-                                                 s1_lvx_lo, lvx_v, lvy_lo_in, lvy_hi_in, lvz_v,
-                                                 lvz_top, lpc, states, o, acc);
+        real dv[V];
+        if (AIR && own) vload<real, V>(a.div_tape + (m - 1 - a.div_tape_step0) * n_cells + cc, dv);
+        else
+#pragma unroll
+            for (int e = 0; e < V; ++e) dv[e] = 0;
+        adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
+            a, c, s, m - 1, 0, own, want_walls, grad_signal, tv, ld_a, ld_up, ld_ylo, ld_yhi,
+            ldz_lo, ldz_hi, s1_lvx_lo, lvx_v, lvy_lo_in, lvy_hi_in, lvz_v, lvz_top, lpc, states, o,
+            acc, dv, air_acc);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             ld_a[e] = ld_up[e];
@@ -400,14 +414,16 @@ __device__ __forceinline__ void adjoint2_march(
         const real ldz_hi = slot(i, t + 1)[0];
         real tv[V];
         if (want_walls) load_tape<real, V, XWALL>(a, c, i, m - 2, tv);
+        real dv[V];
+        if (AIR) vload<real, V>(a.div_tape + (m - 2 - a.div_tape_step0) * n_cells + cc, dv);
         real lvz_v[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) lvz_v[e] = A.lvz[e];
         AdjPlane<real, V, NR> o;
-        adjoint_cells<real, V, NR, LAYER, XWALL>(a, c, i, m - 2, 1, true, want_walls, grad_signal,
-                                                 tv, A.ld, ld_next, ld_ylo, ld_yhi, ldz_lo, ldz_hi,
-                                                 s2_lvx_lo, A.lvx_hi, A.lvy_lo, A.lvy_hi, lvz_v,
-                                                 A.lvz[V], A.lp, A.psi, o, acc);
+        adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
+            a, c, i, m - 2, 1, true, want_walls, grad_signal, tv, A.ld, ld_next, ld_ylo, ld_yhi,
+            ldz_lo, ldz_hi, s2_lvx_lo, A.lvx_hi, A.lvy_lo, A.lvy_hi, lvz_v, A.lvz[V], A.lp, A.psi,
+            o, acc, dv, air_acc);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(io.lvx_out + cc + L.plane, o.lvx_hi);
         vstore<real, V>(io.lvy_out + cy, o.lvy_lo);
@@ -494,7 +510,25 @@ __device__ inline void flush_wall_rows(const double (&acc)[6], const int64_t row
     }
 }
 
-template <typename real, int V, int NR, bool LAYER>
+// Each wave adds its sums of the air partials (relax_scale, then one per relaxation state) into
+// its own row of `partials`.
+template <int NA>
+__device__ inline void flush_air_rows(const double (&acc)[NA], const int count, const int64_t row,
+                                      double *partials) {
+    bool nonzero = false;
+#pragma unroll
+    for (int q = 0; q < NA; ++q) nonzero = nonzero || acc[q] != 0;
+    if (!__any(nonzero)) return;
+    double sum[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) sum[q] = wave_sum(acc[q]);
+    if ((threadIdx.x & 63u) == 0) {
+        double *p = partials + row * kGradSlots + kGradScale;
+        for (int q = 0; q < count; ++q) p[q] = p[q] + sum[q];
+    }
+}
+
+template <typename real, int V, int NR, bool LAYER, bool AIR = false>
 __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     const SweepArgs<real> a, const AdjointArgs<real> g, const Adjoint2IO<real> io,
     const Tiling2 t2, const int64_t m) {
@@ -544,10 +578,13 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     lds_barrier();
     const bool want_walls = (g.grads & ACFDTD_HIP_GRAD_WALLS) && a.wall_tape;
     double acc[6] = {};
-    adjoint2_march<real, V, NR, LAYER>(a, t2, i0, i1, j, k0, valid, owner, t, m, want_walls,
-                                       g.grad_signal, io, lds, src_lds, len_lds, rx_lds, rxg_lds,
-                                       walls ? wt_lds : nullptr, acc);
-    if (want_walls) flush_wall_rows(acc, (int64_t)b * (t2.threads / 64) + t / 64, g.partials);
+    double air_acc[1 + (NR > 0 ? NR : 1)] = {};
+    adjoint2_march<real, V, NR, LAYER, AIR>(a, t2, i0, i1, j, k0, valid, owner, t, m, want_walls,
+                                            g.grad_signal, io, lds, src_lds, len_lds, rx_lds,
+                                            rxg_lds, walls ? wt_lds : nullptr, acc, air_acc);
+    const int64_t row = (int64_t)b * (t2.threads / 64) + t / 64;
+    if (want_walls) flush_wall_rows(acc, row, g.partials);
+    if (AIR) flush_air_rows(air_acc, 1 + NR, row, g.partials);
 }
 
 template <typename real>
@@ -555,32 +592,39 @@ using Adjoint2Kernel = void (*)(const SweepArgs<real>, const AdjointArgs<real>,
                                 const Adjoint2IO<real>, const Tiling2, const int64_t);
 
 template <typename real, int NR>
-Adjoint2Kernel<real> pick_adjoint2_nr(bool layer) {
+Adjoint2Kernel<real> pick_adjoint2_nr(bool layer, bool air) {
     constexpr int V = sizeof(real) == 4 ? 4 : 2;
+    if constexpr (NR > 0) {
+        if (air) return layer ? adjoint2_step<real, V, NR, true, true>
+                              : adjoint2_step<real, V, NR, false, true>;
+    }
+    (void)air;
     return layer ? adjoint2_step<real, V, NR, true> : adjoint2_step<real, V, NR, false>;
 }
 
+// AIR kernels (air gradients from the divergence tape) when the air gradients are asked for.
 template <typename real>
-Adjoint2Kernel<real> pick_adjoint2(int nr, bool layer) {
+Adjoint2Kernel<real> pick_adjoint2(int nr, bool layer, bool air) {
     switch (nr) {
-        case 0: return pick_adjoint2_nr<real, 0>(layer);
-        case 1: return pick_adjoint2_nr<real, 1>(layer);
-        case 2: return pick_adjoint2_nr<real, 2>(layer);
-        case 3: return pick_adjoint2_nr<real, 3>(layer);
-        default: return pick_adjoint2_nr<real, 4>(layer);
+        case 0: return pick_adjoint2_nr<real, 0>(layer, false);
+        case 1: return pick_adjoint2_nr<real, 1>(layer, air);
+        case 2: return pick_adjoint2_nr<real, 2>(layer, air);
+        case 3: return pick_adjoint2_nr<real, 3>(layer, air);
+        default: return pick_adjoint2_nr<real, 4>(layer, air);
     }
 }
 
 // Whether acfdtd_hip_adjoint_run may advance two adjoint steps per sweep: asked for, both
-// sets complete, no sponge or air gradient (they read the forward states of every step), the
-// source and receiver tables within the LDS copies.
+// sets complete, no sponge gradient (it reads the forward states of every step), air gradients
+// only from a divergence tape, the source and receiver tables within the LDS copies.
 inline bool adjoint2_applies(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g) {
-    const bool fwdgrad = g.fwd_count > 0 &&
-                         (((g.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer) ||
-                          ((g.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0));
+    const bool sponge = (g.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer;
+    const bool air = (g.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0;
+    const bool states = g.fwd_count > 0 && (sponge || (air && d.div_tape == nullptr));
     return g.time_block == 2 && has_adjoint_swap(g) && g.lp_swap != nullptr &&
-           (d.n_relax == 0 || g.lpsi_swap != nullptr) && !fwdgrad &&
-           d.n_sources <= kFused2Sources && d.n_receivers <= kFused2Receivers;
+           (d.n_relax == 0 || g.lpsi_swap != nullptr) && !states &&
+           !(air && d.div_tape == nullptr) && d.n_sources <= kFused2Sources &&
+           d.n_receivers <= kFused2Receivers;
 }
 
 template <typename real>
@@ -592,7 +636,8 @@ int64_t launch_adjoint2_quads(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint
     fill_args(d, a);
     AdjointArgs<real> g;
     fill_adjoint(gd, g);
-    const Adjoint2Kernel<real> k = pick_adjoint2<real>(d.n_relax, d.has_layer != 0);
+    const bool air = (gd.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0 && d.div_tape != nullptr;
+    const Adjoint2Kernel<real> k = pick_adjoint2<real>(d.n_relax, d.has_layer != 0, air);
     real *first[6] = {static_cast<real *>(gd.ldiv), static_cast<real *>(gd.lvx),
                       static_cast<real *>(gd.lvy), static_cast<real *>(gd.lvz),
                       static_cast<real *>(gd.lp), static_cast<real *>(gd.lpsi)};

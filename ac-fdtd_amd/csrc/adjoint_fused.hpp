@@ -55,6 +55,10 @@ __device__ __forceinline__ void adjoint_march_vec(
     const Layout L(nx, ny, nz, j, k0);
     const int64_t n_cells = L.plane * nx;
     const int jk0 = j * nz + k0;
+    // Air gradients: the divergences of step m - 1 from the divergence tape when there is one,
+    // else from the forward velocities of that step.
+    const real *div_row =
+        FWDGRAD && air && a.div_tape ? a.div_tape + (m - 1 - a.div_tape_step0) * n_cells : nullptr;
     const real vel = a.vel;
     const bool local = m >= 1;
     const int64_t step = m - 1;  // the step whose local part this sweep transposes
@@ -301,10 +305,12 @@ __device__ __forceinline__ void adjoint_march_vec(
                     }
                 }
             }
-            real bp[V], bvx_lo[V], bvx_hi[V], bvy_lo[V], bvy_hi[V], bvz[V];
+            real bp[V], bvx_lo[V], bvx_hi[V], bvy_lo[V], bvy_hi[V], bvz[V], dvt[V];
             real bvz_top = 0;
             if (sponge_cell) vload<real, V>(io.before_p + c, bp);
-            if (air) {
+            if (air && div_row) {
+                vload<real, V>(div_row + c, dvt);
+            } else if (air) {
                 vload<real, V>(io.before_vx + c, bvx_lo);
                 vload<real, V>(io.before_vx + c + L.plane, bvx_hi);
                 vload<real, V>(io.before_vy + cy, bvy_lo);
@@ -352,9 +358,13 @@ __device__ __forceinline__ void adjoint_march_vec(
                 real ldv = ltil * cdiv;
                 double div = 0;
                 if (air) {
-                    const real top = e + 1 < V ? bvz[e + 1] : bvz_top;
-                    div = (double)(((bvx_hi[e] - bvx_lo[e]) + (bvy_hi[e] - bvy_lo[e])) +
-                                   (top - bvz[e]));
+                    if (div_row) {
+                        div = (double)dvt[e];
+                    } else {
+                        const real top = e + 1 < V ? bvz[e + 1] : bvz_top;
+                        div = (double)(((bvx_hi[e] - bvx_lo[e]) + (bvy_hi[e] - bvy_lo[e])) +
+                                       (top - bvz[e]));
+                    }
                     acc[kGradScale] += div * (double)ltil;
                 }
 #pragma unroll
@@ -408,7 +418,8 @@ __global__ __launch_bounds__(512) void adjoint_fused_step(const SweepArgs<real> 
     const bool sponge_face = FWDGRAD && LAYER && (g.grads & ACFDTD_HIP_GRAD_SPONGE) && io.after_vx;
     const bool sponge_cell =
         FWDGRAD && local && LAYER && (g.grads & ACFDTD_HIP_GRAD_SPONGE) && io.before_p;
-    const bool air = FWDGRAD && local && NR > 0 && (g.grads & ACFDTD_HIP_GRAD_AIR) && io.before_vx;
+    const bool air = FWDGRAD && local && NR > 0 && (g.grads & ACFDTD_HIP_GRAD_AIR) &&
+                     (io.before_vx || a.div_tape);
     if (active) {
         const int i0 = xc * a.tiles.chunk;
         const int i1 = min(i0 + a.tiles.chunk, a.nx);
@@ -465,9 +476,10 @@ int64_t launch_adjoint_pairs(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_
     a.tiles = fused_tiling(d);
     AdjointArgs<real> g;
     fill_adjoint(gd, g);
-    const bool fwdgrad = gd.fwd_count > 0 &&
-                         (((gd.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer) ||
-                          ((gd.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0));
+    const bool air = (gd.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0;
+    const bool fwdgrad = (gd.fwd_count > 0 &&
+                          (((gd.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer) || air)) ||
+                         (air && d.div_tape);
     const AdjointFusedKernel<real> k =
         pick_adjoint<real>(a.tiles.vec, d.n_relax, d.has_layer != 0, fwdgrad);
     real *first[4] = {static_cast<real *>(gd.ldiv), static_cast<real *>(gd.lvx),

@@ -131,6 +131,8 @@ extern "C" int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step,
     if (desc->wall_tape &&
         (step < desc->wall_tape_step0 || step - desc->wall_tape_step0 >= desc->wall_tape_rows))
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
+    if (!div_tape_covers(*desc, step, step))
+        return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover this step");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return fused_once<double>(*desc, step, recording_column, reverse != 0, s);
@@ -159,6 +161,9 @@ extern "C" int acfdtd_hip_run_states(const acfdtd_hip_desc *desc, int64_t step_o
     d.wall_tape_stride = 0;
     d.wall_tape_step0 = 0;
     d.wall_tape_rows = 0;
+    d.div_tape = nullptr;
+    d.div_tape_step0 = 0;
+    d.div_tape_rows = 0;
     const int status = validate(&d);
     if (status != ACFDTD_HIP_OK) return status;
     g_last_error.clear();

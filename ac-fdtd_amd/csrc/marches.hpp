@@ -120,6 +120,7 @@ __device__ __forceinline__ void pressure_march(
         const int64_t cy = L.face_y(i);
         const int64_t cz = L.face_z(i);
         const real div = ((vx_hi - vx_lo) + (vy[cy + a.zp] - vy[cy])) + (vz[cz + 1] - vz[cz]);
+        if (a.div_tape) a.div_tape[(step - a.div_tape_step0) * n_cells + c] = div;
 
         const real p_old = p[c];
         real pn;
@@ -460,11 +461,12 @@ __device__ __forceinline__ void fused_march_vec(
 
         // Pressure with the relaxation states, walls and sponge.
         const int cx = XWALL ? wall_class(i, nx) : 0;
-        real p_new[V], tape_value[V];
+        real p_new[V], tape_value[V], div_value[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real div =
                 ((vx_hi[e] - vx_lo[e]) + (vy_hi[e] - vy_lo[e])) + (vz_f[e + 1] - vz_f[e]);
+            div_value[e] = div;
             const real p_old = p_c[e];
             real pn;
             if (COMPENSATED) {
@@ -512,6 +514,7 @@ __device__ __forceinline__ void fused_march_vec(
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) vstore<real, V>(psi + (int64_t)r * n_cells + c, states[r]);
+        if (a.div_tape) vstore<real, V>(a.div_tape + (step - a.div_tape_step0) * n_cells + c, div_value);
         if (a.wall_tape && (XWALL || yz_boundary)) {
             real *row = a.wall_tape + (step - a.wall_tape_step0) * a.wall_tape_stride;
 #pragma unroll

@@ -546,7 +546,7 @@ __device__ __forceinline__ void advance_cells(
     const real pz_lo, const real pz_hi, const real (&vx_lo)[V], const real (&vx_v)[V],
     const real (&vy_lo_in)[V], const real (&vy_hi_in)[V], const real (&vz_v)[V],
     const real vz_top, real (&states)[NR > 0 ? NR : 1][V], PlaneState<real, V, NR> &o,
-    real (&tape_value)[V]) {
+    real (&tape_value)[V], real (&div_value)[V]) {
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
@@ -587,6 +587,7 @@ __device__ __forceinline__ void advance_cells(
     for (int e = 0; e < V; ++e) {
         const real div =
             ((o.vx_hi[e] - vx_lo[e]) + (o.vy_hi[e] - o.vy_lo[e])) + (o.vz[e + 1] - o.vz[e]);
+        div_value[e] = div;
         const real p_old = p_c[e];
         real pn;
         if (COMPENSATED) {
@@ -734,8 +735,9 @@ constexpr int stage_bytes() {
 }
 
 // PADDED: rows may be padded (z_pitch), see fused_march_vec; not with STAGED or a tape.
+// DIVT: owner lanes write both steps' divergences to the divergence tape (a.div_tape).
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED,
-          bool PADDED = false>
+          bool PADDED = false, bool DIVT = false>
 __device__ __forceinline__ void fused2_march(
     const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
     const int k0, const bool valid, const bool owner, const int t, const int64_t step,
@@ -748,6 +750,7 @@ __device__ __forceinline__ void fused2_march(
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
     static_assert(!(PADDED && STAGED), "the staged sweep takes unpadded rows");
+    static_assert(!(DIVT && (PADDED || STAGED)), "the divergence tape is a plain-sweep option");
     const int nx = a.nx, zp = a.zp;  // zp: the row length (the room's nz unless padded)
     const Layout L = room_layout(a, j, k0);
     const int64_t n_cells = L.plane * nx;
@@ -903,17 +906,19 @@ __device__ __forceinline__ void fused2_march(
     auto compute1 = [&](const int s, Step1In<real, V, NR> &in, PlaneState<real, V, NR> &o,
                         auto xwall_tag) __attribute__((always_inline)) {
         constexpr bool XWALL = decltype(xwall_tag)::value;
-        real tape_value[V];
+        real tape_value[V], div_value[V];
         auto run = [&](auto plain_tag) __attribute__((always_inline)) {
             advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value,
                           PADDED>(
                 a, c, s, step, pn_c, in.p_up, in.q_lo, in.q_hi, in.pz_lo, in.pz_hi, s1_vx_lo,
-                in.vx_v, in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value);
+                in.vx_v, in.vy_lo, in.vy_hi, in.vz_v, in.vz_top, in.psi, o, tape_value, div_value);
         };
         if (plain_xy(s)) run(Wall{});
         else run(Interior{});
         if (a.wall_tape && owner && s >= i0 && s < i1)
             tape_cells<real, V, XWALL>(a, c, s, step, t2.tape_vec != 0, tape_value);
+        if (DIVT && owner && s >= i0 && s < i1)
+            vstore<real, V>(a.div_tape + (step - a.div_tape_step0) * n_cells + L.cell(s), div_value);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             pn_c[e] = in.p_up[e];
@@ -961,16 +966,17 @@ __device__ __forceinline__ void fused2_march(
 #pragma unroll
         for (int e = 0; e < V; ++e) vz_v[e] = A.vz[e];
         PlaneState<real, V, NR> o;
-        real tape_value[V];
+        real tape_value[V], div_value[V];
         auto run = [&](auto plain_tag) __attribute__((always_inline)) {
             advance_cells<real, V, NR, WALLS, LAYER, COMPENSATED, XWALL, decltype(plain_tag)::value,
                           PADDED>(
                 a, c, i, step + 1, A.p, p_next, q_lo, q_hi, pz_lo, pz_hi, s2_vx_lo, A.vx_hi,
-                A.vy_lo, A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value);
+                A.vy_lo, A.vy_hi, vz_v, A.vz[V], A.psi, o, tape_value, div_value);
         };
         if (plain_xy(i)) run(Wall{});
         else run(Interior{});
         if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, t2.tape_vec != 0, tape_value);
+        if (DIVT) vstore<real, V>(a.div_tape + (step + 1 - a.div_tape_step0) * n_cells + cc, div_value);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
         vstore<real, V>(vy_out + cy, o.vy_lo);
@@ -992,8 +998,10 @@ __device__ __forceinline__ void fused2_march(
 
     // Prologue: step 1 of the halo plane i0 - 1 (its p^(n+1) only) and of plane i0.
     PlaneState<real, V, NR> A, B;
+    // With the divergence tape and the sponge the prefetch would spill (three relaxation
+    // states); those load each plane's inputs when they are needed.
     constexpr bool PREFETCH =
-        ACFDTD_FUSED2_PREFETCH && !STAGED && NR <= 3 &&
+        ACFDTD_FUSED2_PREFETCH && !STAGED && NR <= 3 && !(DIVT && LAYER) &&
         (sizeof(real) == 4 ? V == 4 : (V == 2 && !LAYER));
     Step1In<real, V, NR> ahead;  // PREFETCH: the next plane's step-1 inputs, in registers
     real p_prev[V];

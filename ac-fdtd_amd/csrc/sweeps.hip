@@ -286,6 +286,8 @@ extern "C" int acfdtd_hip_run(const acfdtd_hip_desc *desc, int64_t step_offset, 
     if (desc->wall_tape && (step_offset < desc->wall_tape_step0 ||
                             step_offset + n_steps - desc->wall_tape_step0 > desc->wall_tape_rows))
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these steps");
+    if (!div_tape_covers(*desc, step_offset, step_offset + n_steps - 1))
+        return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover these steps");
     if (n_steps == 0) return ACFDTD_HIP_OK;
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -314,6 +316,8 @@ extern "C" int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t st
     if (desc->wall_tape &&
         (step < desc->wall_tape_step0 || step - desc->wall_tape_step0 >= desc->wall_tape_rows))
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
+    if (!div_tape_covers(*desc, step, step))
+        return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover this step");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return single_sweep<double>(*desc, true, step, recording_column, s);
@@ -450,6 +454,8 @@ extern "C" int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step,
     if (desc->wall_tape && (step < desc->wall_tape_step0 ||
                             step + 1 - desc->wall_tape_step0 >= desc->wall_tape_rows))
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these two steps");
+    if (!div_tape_covers(*desc, step, step + 1))
+        return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover these two steps");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8)

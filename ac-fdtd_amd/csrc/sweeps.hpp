@@ -164,6 +164,9 @@ struct SweepArgs {
     // Tape rows start 16-byte aligned (aligned base, stride a multiple of 4 elements), so a
     // lane's V cells of an x or y face row move as one vector access.
     int wall_tape_rows16;
+    // Divergence tape: row n - div_tape_step0 of nx*ny*nz values holds step n's divergences.
+    real *div_tape;
+    int64_t div_tape_step0;
 };
 
 // The layout of a room's fields from its arguments (padded rows or not).
@@ -258,6 +261,14 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
     a.wall_tape_rows16 = d.wall_tape != nullptr &&
                          reinterpret_cast<uintptr_t>(d.wall_tape) % 16 == 0 &&
                          d.wall_tape_stride % 4 == 0;
+    a.div_tape = static_cast<real *>(d.div_tape);
+    a.div_tape_step0 = d.div_tape_step0;
+}
+
+// Whether the descriptor's divergence tape holds rows for steps lo .. hi (true without a tape).
+inline bool div_tape_covers(const acfdtd_hip_desc &d, int64_t lo, int64_t hi) {
+    return d.div_tape == nullptr || hi < lo ||
+           (lo >= d.div_tape_step0 && hi - d.div_tape_step0 < d.div_tape_rows);
 }
 
 // `padded`: the caller's kernels take padded rows (z_pitch > 0); the other entry points refuse
@@ -280,7 +291,11 @@ inline int validate(const acfdtd_hip_desc *d, const bool padded = false) {
         if (d->z_pitch <= d->nz)
             return fail(ACFDTD_HIP_EINVAL, "z_pitch must be 0 or greater than nz");
         if (d->wall_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no wall tape");
+        if (d->div_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no divergence tape");
     }
+    if (d->div_tape && (d->n_relax < 1 || d->div_tape_rows < 0 || d->div_tape_step0 < 0))
+        return fail(ACFDTD_HIP_EINVAL,
+                    "a divergence tape needs relaxation states and rows from a step >= 0");
     if ((int64_t)d->ny * row_cells(*d) > INT32_MAX)
         return fail(ACFDTD_HIP_EINVAL, "ny * nz must fit in 31 bits");
     if (d->real_bytes != 4 && d->real_bytes != 8)

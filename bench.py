@@ -300,7 +300,8 @@ def gradient_cost(size, steps, seed=4):
     of six absorptions U[0.05, 0.5], the model at uniform 0.2, 4 receivers, the band-limited
     impulse at 1 kHz on a 34.3 mm grid. Returns seconds for a forward pass and for forward +
     backward, with walls only (boundary tape, no checkpoints) and with walls and the air
-    strengths (state checkpoints every 100 steps plus recomputation)."""
+    strengths: from a divergence tape of every step (the default within 64 GiB), and from
+    checkpoints every 100 steps with each segment's divergences recomputed (as beyond it)."""
     import torch
 
     from ac_fdtd_amd import AIR, AirAbsorption, Grid, WallAdmittances, admittance_from_absorption
@@ -315,7 +316,7 @@ def gradient_cost(size, steps, seed=4):
     points = [tuple((0.15 + 0.7 * rng.uniform(0.0, 1.0, 3)) * lengths) for _ in range(5)]
     start = admittance_from_absorption(0.2)
 
-    def model(air):
+    def model(air, history_bytes=64 << 30):
         from ac_fdtd_amd.grid import max_stable_time_step
 
         dt = max_stable_time_step(dx, AIR.sound_speed)
@@ -324,7 +325,7 @@ def gradient_cost(size, steps, seed=4):
             receivers=points[1:], dtype=np.float32,
             walls=WallAdmittances.from_absorption(0.2),
             air_absorption=AirAbsorption(20.0, 50.0) if air else None, checkpoint_every=100,
-            autotune=True,
+            history_bytes=history_bytes, autotune=True,
         )
 
     def seconds(call):
@@ -335,8 +336,10 @@ def gradient_cost(size, steps, seed=4):
         return time.perf_counter() - began
 
     out = {"grid": [size] * 3, "steps": steps}
-    for air in (False, True):
-        room = model(air)
+    cases = (("walls_tape", False, 64 << 30), ("walls_and_air_div_tape", True, 64 << 30),
+             ("walls_and_air_checkpointed", True, 0))
+    for key, air, history in cases:
+        room = model(air, history)
         target = room(torch.tensor(hidden, dtype=torch.float64)).detach()
         xi = torch.full((6,), start, dtype=torch.float64, requires_grad=True)
         strengths = room.default_air_strength().requires_grad_(True) if air else None
@@ -353,13 +356,14 @@ def gradient_cost(size, steps, seed=4):
         forward_backward()
         t_fwd = min(seconds(forward) for _ in range(3))
         t_grad = min(seconds(forward_backward) for _ in range(3))
-        key = "walls_and_air_checkpointed" if air else "walls_tape"
         out[key] = {
             "forward_s": round(t_fwd, 4),
             "forward_backward_s": round(t_grad, 4),
             "grad_over_forward": round(t_grad / t_fwd, 3),
             "recomputed_steps": room.report.recomputed_steps,
             "checkpoints": room.report.checkpoints,
+            "divergence_bytes": room.report.div_tape_bytes,
+            "adjoint_two_steps_per_sweep": room.report.adjoint_two_step,
         }
         if not np.all(np.isfinite(xi.grad.cpu().numpy())):
             raise RuntimeError("non-finite admittance gradient")

@@ -142,6 +142,18 @@ typedef struct acfdtd_hip_desc {
      * that width. */
     int32_t z_pitch;
     int32_t reserved; /* 0 */
+
+    /* Optional divergence tape, for the air-strength gradients (device, working precision).
+     * For absolute step n, row n - div_tape_step0 (of div_tape_rows rows of nx*ny*nz elements,
+     * C order like p) receives the velocity divergence every cell's pressure update of step n
+     * uses, ((vx_hi - vx_lo) + (vy_hi - vy_lo)) + (vz_hi - vz_lo) of v^{n+1/2}, bit for bit.
+     * The adjoint reads it in place of recomputed forward velocities (acfdtd_hip_adjoint_run),
+     * so the air gradients need no recomputation and run two adjoint steps per sweep. Needs
+     * n_relax > 0; not with padded rows. With a tape the two-step sweep runs on the widest
+     * lanes only (nz a multiple of 4 fp32, 2 fp64); other grids take one step per sweep. */
+    void *div_tape;
+    int64_t div_tape_step0;
+    int64_t div_tape_rows;
 } acfdtd_hip_desc;
 
 /* sizeof(acfdtd_hip_desc), for the caller's mirror check. Replaces acfdtd_struct_size

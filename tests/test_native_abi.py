@@ -140,6 +140,32 @@ def test_padded_rows_are_checked_by_the_batch_api(lib):
         assert message in lib.acfdtd_hip_last_error().decode()
 
 
+def test_divergence_tapes_are_checked_before_any_launch(lib):
+    """A divergence tape needs relaxation states, must hold a row for every step a call
+    advances, and is refused with padded rows and by the batch API."""
+    from ac_fdtd_amd.batch import _bind
+
+    _bind(lib)
+    air = {"n_relax": 1, "psi": 4096}
+    tape = {"div_tape": 4096, "div_tape_step0": 10, "div_tape_rows": 4}
+    assert lib.acfdtd_hip_run(ctypes.byref(_desc(**tape)), 10, 1, 0, None) == EINVAL
+    assert "relaxation states" in lib.acfdtd_hip_last_error().decode()
+    desc = _desc(**air, **tape)
+    for offset, n in ((9, 1), (10, 5), (13, 2)):
+        assert lib.acfdtd_hip_run(ctypes.byref(desc), offset, n, 0, None) == EINVAL
+        assert "divergence tape" in lib.acfdtd_hip_last_error().decode()
+    assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(desc), 14, 0, None) == EINVAL
+    assert lib.acfdtd_hip_fused2_step(ctypes.byref(desc), 13, 0, 0, None) == EINVAL
+    assert "divergence tape" in lib.acfdtd_hip_last_error().decode()
+    swaps = {"p_swap": 4096, "vx_swap": 4096, "vy_swap": 4096, "vz_swap": 4096}
+    rooms = (_native.Desc * 1)(_desc(nz=5, z_pitch=8, **swaps, **air, **tape))
+    assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 1) == 0
+    assert "padded rows take no divergence tape" in lib.acfdtd_hip_last_error().decode()
+    rooms[0] = _desc(**swaps, **air, **tape)
+    assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 1) == 0
+    assert "divergence tapes" in lib.acfdtd_hip_last_error().decode()
+
+
 def test_energy_without_allocation_refuses_grids_beyond_its_scratch(lib):
     """ceil(ny / 8) * ceil(nz / 64) workgroup columns must fit the fixed scratch; refused before
     any HIP call."""
