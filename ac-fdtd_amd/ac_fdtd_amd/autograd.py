@@ -305,7 +305,7 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
         from .autotune import adjoint_tiling
 
         if two_step:
-            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, two_step=True)
+            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, two_step=True, air=divergences)
         else:
             fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, fwdgrad=want_sponge or want_air)
     n_partials = lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd))

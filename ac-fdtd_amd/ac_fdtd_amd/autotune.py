@@ -250,10 +250,11 @@ def forward_plan(solver, tape: bool = False) -> tuple[int, int, int, bool]:
 
 
 def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
-                   two_step: bool = False) -> tuple[int, int]:
+                   two_step: bool = False, air: bool = False) -> tuple[int, int]:
     """Fastest (tile_j, chunk_x) of the fused adjoint step: with the wall gradient when the
     solver has a wall table, and with the sponge and air gradients too when ``fwdgrad``; of the
-    two-step adjoint sweep (wall gradient only) when ``two_step``."""
+    two-step adjoint sweep when ``two_step`` (wall gradient, and with ``air`` the air gradients
+    from a divergence tape)."""
     if not solver.fused:
         return solver.tile_j, solver.chunk_x
     if candidates is None:
@@ -292,7 +293,13 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
         adj.lp_swap = second[0].data_ptr()
         adj.lpsi_swap = second[1].data_ptr() if k.n_relax else None
         adj.time_block = 2
-    ring = weights = None
+    ring = weights = divergences = None
+    if two_step and air and k.n_relax:
+        # Adjoint steps n .. 1 read the divergences of steps n - 1 .. 0.
+        divergences = torch.zeros((steps + 2, g.n_cells), **opts)
+        fwd.div_tape, fwd.div_tape_step0, fwd.div_tape_rows = (divergences.data_ptr(), 0,
+                                                               divergences.shape[0])
+        adj.grads |= _native.GRAD_AIR
     if fwdgrad and not two_step and (k.has_layer or k.n_relax):
         # Adjoint steps n .. 1 read forward states 1 .. n + 1.
         ring = [torch.zeros((steps + 2, *g.shape), **opts)] + [
@@ -321,5 +328,6 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
         _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), n, 1,
                                                  handle), "acfdtd_hip_adjoint_run")
 
-    kind = "adjoint2" if two_step else "adjoint-fwdgrad" if ring is not None else "adjoint"
+    kind = ("adjoint2-air" if divergences is not None else "adjoint2") if two_step else (
+        "adjoint-fwdgrad" if ring is not None else "adjoint")
     return _pick(solver, kind, launch, candidates)

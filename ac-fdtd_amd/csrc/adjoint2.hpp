@@ -137,30 +137,15 @@ __device__ __forceinline__ void adj_lane(const SweepArgs<real> &a, const int j, 
 }
 
 // What one adjoint step leaves in a lane for one plane: the new adjoint velocities on the faces
-// the next step reads again (upper x, both y, all z), lp, ldiv and lpsi after the local part.
+// the next step reads again (upper x, both y, all z), lp, ldiv and lpsi after the local part;
+// with the air gradients, the divergences step B of the plane needs (loaded with step A's
+// inputs, so that step B, which otherwise reads only registers and LDS, waits for no load).
 template <typename real, int V, int NR>
 struct AdjPlane {
     real lvx_hi[V], lvy_lo[V], lvy_hi[V], lvz[V + 1], lp[V], ld[V];
     real psi[NR > 0 ? NR : 1][V];
+    real dv_b[V];
 };
-
-template <typename real, int V, int NR>
-__device__ __forceinline__ void copy_adj(AdjPlane<real, V, NR> &to, const AdjPlane<real, V, NR> &from) {
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-        to.lvx_hi[e] = from.lvx_hi[e];
-        to.lvy_lo[e] = from.lvy_lo[e];
-        to.lvy_hi[e] = from.lvy_hi[e];
-        to.lvz[e] = from.lvz[e];
-        to.lp[e] = from.lp[e];
-        to.ld[e] = from.ld[e];
-    }
-    to.lvz[V] = from.lvz[V];
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) to.psi[r][e] = from.psi[r][e];
-}
 
 // One adjoint step of the lane's V cells in plane i, adjoint_march_vec's plane body without the
 // sponge and air gradients: the faces from ldiv of the cell (ld_c) and its neighbours (ld_up:
@@ -380,10 +365,14 @@ __device__ __forceinline__ void adjoint2_march(
 #pragma unroll
             for (int e = 0; e < V; ++e) tv[e] = 0;
         real dv[V];
-        if (AIR && own) vload<real, V>(a.div_tape + (m - 1 - a.div_tape_step0) * n_cells + cc, dv);
-        else
+        if (AIR && own) {
+            const real *row = a.div_tape + (m - 1 - a.div_tape_step0) * n_cells + cc;
+            vload<real, V>(row, dv);
+            vload<real, V>(row - n_cells, o.dv_b);  // step m - 2, for step B of this plane
+        } else {
 #pragma unroll
             for (int e = 0; e < V; ++e) dv[e] = 0;
+        }
         adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
             a, c, s, m - 1, 0, own, want_walls, grad_signal, tv, ld_a, ld_up, ld_ylo, ld_yhi,
             ldz_lo, ldz_hi, s1_lvx_lo, lvx_v, lvy_lo_in, lvy_hi_in, lvz_v, lvz_top, lpc, states, o,
@@ -414,8 +403,6 @@ __device__ __forceinline__ void adjoint2_march(
         const real ldz_hi = slot(i, t + 1)[0];
         real tv[V];
         if (want_walls) load_tape<real, V, XWALL>(a, c, i, m - 2, tv);
-        real dv[V];
-        if (AIR) vload<real, V>(a.div_tape + (m - 2 - a.div_tape_step0) * n_cells + cc, dv);
         real lvz_v[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) lvz_v[e] = A.lvz[e];
@@ -423,7 +410,7 @@ __device__ __forceinline__ void adjoint2_march(
         adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
             a, c, i, m - 2, 1, true, want_walls, grad_signal, tv, A.ld, ld_next, ld_ylo, ld_yhi,
             ldz_lo, ldz_hi, s2_lvx_lo, A.lvx_hi, A.lvy_lo, A.lvy_hi, lvz_v, A.lvz[V], A.lp, A.psi,
-            o, acc, dv, air_acc);
+            o, acc, A.dv_b, air_acc);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(io.lvx_out + cc + L.plane, o.lvx_hi);
         vstore<real, V>(io.lvy_out + cy, o.lvy_lo);
@@ -464,7 +451,7 @@ __device__ __forceinline__ void adjoint2_march(
     if (owner) {
         // Step B's adjoint vx on face i0, the lower face of the first cell (the prologue of
         // adjoint_march_vec with step A's results as inputs).
-        const real fxf = LAYER ? a.face[0][i0] : real(1);
+        const real fxf = LAYER ? uniform_load(a.face[0] + i0) : real(1);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = v_half_lo[e] + ((i0 > 0 ? ld_prev[e] : real(0)) - A.ld[e]);
@@ -473,21 +460,29 @@ __device__ __forceinline__ void adjoint2_march(
         vstore<real, V>(io.lvx_out + L.cell(i0), s2_lvx_lo);
     }
 
-    for (int i = i0; i < i1; ++i) {
+    // One plane: step A of plane i + 1 into `next`, step B of plane i from `here`. The march
+    // alternates the two states' roles, unrolled by two, instead of copying `next` into `here`
+    // after each plane (a copy into registers a load may still target waits for that load,
+    // and with it for every store issued since: see fused2_march).
+    auto plane = [&](const int i, const AdjPlane<real, V, NR> &here,
+                     AdjPlane<real, V, NR> &next) __attribute__((always_inline)) {
         const bool more = i + 1 < nx;
         if (valid && more) {
-            stepA_at(i + 1, B);
-            vstore<real, V>(slot(i + 1, t), B.ld);
+            stepA_at(i + 1, next);
+            vstore<real, V>(slot(i + 1, t), next.ld);
         }
         lds_barrier();
         if (owner) {
             real ld_next[V];
 #pragma unroll
-            for (int e = 0; e < V; ++e) ld_next[e] = more ? B.ld[e] : A.ld[e];
-            if (i == 0 || i == nx - 1) stepB(i, A, ld_next, Wall{});
-            else stepB(i, A, ld_next, Interior{});
+            for (int e = 0; e < V; ++e) ld_next[e] = more ? next.ld[e] : here.ld[e];
+            if (i == 0 || i == nx - 1) stepB(i, here, ld_next, Wall{});
+            else stepB(i, here, ld_next, Interior{});
         }
-        if (valid && more) copy_adj(A, B);
+    };
+    for (int i = i0; i < i1; i += 2) {
+        plane(i, A, B);
+        if (i + 1 < i1) plane(i + 1, B, A);
     }
 }
 
