@@ -255,22 +255,39 @@ class AcousticFDTD:
         ``record=False`` skips the receivers (and an attached boundary tape) for these steps,
         as a recomputation from a checkpoint does; an attached divergence tape is written
         either way."""
+        launch = self.prepare_run(n_steps, record)
+        if launch is not None:
+            launch()
+
+    def prepare_run(self, n_steps: int, record: bool = True):
+        """:meth:`run` in two halves: this one allocates the recording and builds the
+        descriptor; the returned callable (None for ``n_steps <= 0``) launches the steps on the
+        current stream and does run's bookkeeping. Call it once, before any other step of this
+        solver. For timing the steps alone."""
+        if type(self).run is not AcousticFDTD.run:
+            raise NotImplementedError(f"{type(self).__name__} advances its own scheme: use run()")
         n_steps = int(n_steps)
         if n_steps <= 0:
-            return
+            return None
         recording = None
         if self._receivers and record:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
         desc = self._descriptor(recording, tape=record, div=True)
-        stream = torch.cuda.current_stream(self.device).cuda_stream
-        with torch.cuda.device(self.device):
-            status = self._lib.acfdtd_hip_run(
-                ctypes.byref(desc), self.step_index, n_steps, 0, ctypes.c_void_p(stream)
-            )
-        _native.check(status, "acfdtd_hip_run")
-        self.step_index += n_steps
-        if recording is not None:
-            self._runs.append(recording)
+        step = self.step_index
+
+        def launch() -> None:
+            if self.step_index != step:
+                raise RuntimeError("the solver advanced since this run was prepared")
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            with torch.cuda.device(self.device):
+                status = self._lib.acfdtd_hip_run(
+                    ctypes.byref(desc), step, n_steps, 0, ctypes.c_void_p(stream))
+            _native.check(status, "acfdtd_hip_run")
+            self.step_index += n_steps
+            if recording is not None:
+                self._runs.append(recording)
+
+        return launch
 
     def capture(self, n_steps: int, record: bool = True) -> "CapturedSteps":
         """Capture steps ``step_index`` .. ``step_index + n_steps - 1`` of :meth:`run` into one

@@ -126,6 +126,28 @@ def test_running_in_pieces_is_bitwise_running_at_once():
     np.testing.assert_array_equal(pieces.recorded_pressure, whole.recorded_pressure)
 
 
+def test_a_prepared_run_is_run():
+    """prepare_run + its launch (what the bench times) is run() bit for bit; a launch after the
+    solver moved on, and the subclasses that advance their own scheme, are refused."""
+    from ac_fdtd_amd.fourth import FourthOrderFDTD
+
+    direct = prepare(hip(**PARITY["all"]))
+    staged = prepare(hip(**PARITY["all"]))
+    direct.run(37)
+    launch = staged.prepare_run(37)
+    launch()
+    assert staged.step_index == direct.step_index == 37
+    np.testing.assert_array_equal(to_numpy(staged.p), to_numpy(direct.p))
+    np.testing.assert_array_equal(staged.recorded_pressure, direct.recorded_pressure)
+    stale = staged.prepare_run(5)
+    staged.run(1)
+    with pytest.raises(RuntimeError, match="advanced"):
+        stale()
+    assert staged.prepare_run(0) is None
+    with pytest.raises(NotImplementedError):
+        FourthOrderFDTD(PARITY_GRID).prepare_run(4)
+
+
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
 @pytest.mark.parametrize("tile_j,chunk_x", [(1, 1), (2, 3), (8, 7), (4, 64)])
 def test_tiling_does_not_change_the_answer(tile_j, chunk_x, fused):
