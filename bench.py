@@ -477,6 +477,64 @@ def large_room(size, steps, seed=5):
         "t20_max_rel_deviation_by_octave": {f"{int(b)}": float(v) for b, v in deviation.items()},
     }
 
+def large_room_air_gradient(size, steps, seed=5):
+    """The large-room configuration for ``steps`` steps, fp32: seconds of a forward pass and of
+    forward + backward for the gradient with respect to the six wall admittances and the three
+    air strengths. ``steps`` rows of divergences do not fit the 64 GiB history budget at 512^3
+    (537 MB per step), so this is the checkpointed path: the state every 100 steps, each
+    segment's divergences recomputed into a 100-row ring, the adjoint two steps per sweep."""
+    import torch
+
+    from ac_fdtd_amd.autograd import RoomResponse
+
+    grid, config, source, receivers, signal, _ = large_room_workload(size, steps, seed)
+    room = RoomResponse(grid, steps, [(source, signal[:steps])], receivers, dtype=np.float32,
+                        checkpoint_every=100, **config)
+    admittance = room.default_admittance(device="cuda").requires_grad_()
+    strengths = room.default_air_strength(device="cuda").requires_grad_()
+
+    def seconds(call):
+        torch.cuda.synchronize()
+        began = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        return time.perf_counter() - began
+
+    def forward():
+        with torch.no_grad():
+            room(admittance.detach(), air_strength=strengths.detach())
+
+    weights = None
+
+    def forward_backward():
+        nonlocal weights
+        recording = room(admittance, air_strength=strengths)
+        if weights is None:
+            weights = torch.as_tensor(
+                np.random.default_rng(seed + 1).standard_normal(recording.shape),
+                dtype=recording.dtype, device=recording.device)
+        torch.autograd.grad((recording * weights).sum(), [admittance, strengths])
+
+    forward()
+    t_forward = seconds(forward)
+    t_both = seconds(forward_backward)
+    report = room.report
+    out = {
+        "grid": [size] * 3,
+        "steps": steps,
+        "forward_s": round(t_forward, 3),
+        "forward_backward_s": round(t_both, 3),
+        "grad_over_forward": round(t_both / t_forward, 3),
+        "recomputed_steps": report.recomputed_steps,
+        "checkpoints": report.checkpoints,
+        "hbm_bytes": report.checkpoint_bytes + report.div_tape_bytes + report.tape_bytes,
+        "adjoint_two_steps_per_sweep": report.adjoint_two_step,
+    }
+    del room
+    torch.cuda.empty_cache()
+    return out
+
+
 def timed_run(solver, steps: int, warmup: int) -> float:
     """Milliseconds per step of ``solver.run(steps)`` after ``warmup`` steps, from HIP events
     on the current stream."""
@@ -761,7 +819,7 @@ def main() -> int:
                         help="skip the fp64 single-room runs that check the batched rooms")
     parser.add_argument(
         "--mode",
-        choices=("bench", "copy", "gradient", "batch", "variants"),
+        choices=("bench", "copy", "gradient", "batch", "variants", "large-gradient"),
         default="bench",
         help="copy: five plain 512 MiB device-to-device copies, the known-size reference a "
         "profiler's FETCH_SIZE / WRITE_SIZE counters are scaled by; gradient: only the "
@@ -793,6 +851,9 @@ def main() -> int:
             target.copy_(source)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": source.numel() * 4, "copies": 5}), flush=True)
+        return 0
+    if args.mode == "large-gradient":
+        print(json.dumps(large_room_air_gradient(512, 1000)), flush=True)
         return 0
     if args.mode == "gradient":
         print(json.dumps(gradient_cost(128, 2000)), flush=True)
@@ -902,6 +963,9 @@ def main() -> int:
             log(f"large room 512^3, {args.large_steps} steps")
             extra["large_room_512"] = large_room(512, args.large_steps)
             log(f"large room: {extra['large_room_512']}")
+            log("large room 512^3: wall and air gradients, 1000 steps")
+            extra["large_room_512"]["walls_and_air_gradient"] = large_room_air_gradient(512, 1000)
+            log(f"large room air gradient: {extra['large_room_512']['walls_and_air_gradient']}")
 
     log("stream ceiling")
     ceiling = stream_ceiling()
