@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _native
-from .scheme import AcousticFDTD
+from .scheme import AcousticFDTD, PaddedRows, padded_row
 
 __all__ = ["BatchedRooms", "CapturedBatch", "lane_width", "padded_row"]
 
@@ -35,94 +35,13 @@ def lane_width(solver: AcousticFDTD) -> int:
     return width
 
 
-def padded_row(solver: AcousticFDTD) -> int:
-    """Row length of ``solver``'s fields in a batch: nz rounded up to the widest lane of its
-    precision (4 cells fp32, 2 fp64); nz itself when that width divides it."""
-    widest = 4 if solver.np_dtype.itemsize == 4 else 2
-    return -(-solver.grid.nz // widest) * widest
-
-
-class _PaddedRoom:
-    """A room's state, swap set and cell tables in rows of ``padded_row`` elements (every
-    field, vz included), owned by the batch. The elements past the room's own in each row are
-    padding, which the kernels never let reach a cell."""
-
-    def __init__(self, solver: AcousticFDTD) -> None:
-        self.solver = solver
-        g = solver.grid
-        self.zp = padded_row(solver)
-        opts = solver._opts
-        zp = self.zp
-
-        def fields():
-            return [torch.zeros((g.nx, g.ny, zp), **opts),
-                    torch.zeros((g.nx + 1, g.ny, zp), **opts),
-                    torch.zeros((g.nx, g.ny + 1, zp), **opts),
-                    torch.zeros((g.nx, g.ny, zp), **opts)]
-
-        n_relax = solver.coefficients.n_relax
-        self.state = fields()
-        self.psi = torch.zeros((n_relax, g.nx, g.ny, zp), **opts) if n_relax else None
-        self.swap = fields() if solver._swap is not None else None
-        self.psi_swap = (torch.zeros((n_relax, g.nx, g.ny, zp), **opts)
-                         if solver._psi_swap is not None else None)
-        self._tables: dict[tuple, torch.Tensor] = {}
-
-    def _cells(self, cells) -> torch.Tensor:
-        """(i, j * nz + k) pairs re-addressed to rows of zp: (i, j * zp + k), on the device
-        (kept, so that a table a captured graph's room table points at stays alive)."""
-        key = tuple(cells)
-        if key not in self._tables:
-            nz = self.solver.grid.nz
-            table = np.array([(i, (jk // nz) * self.zp + jk % nz) for i, jk in cells],
-                             dtype=np.int32).ravel()
-            self._tables[key] = torch.as_tensor(table, device=self.solver.device)
-        return self._tables[key]
-
-    def _views(self):
-        """(padded view, room tensor) pairs of the state that crosses a run's boundaries."""
-        s, nz = self.solver, self.solver.grid.nz
-        pairs = [(self.state[0][..., :nz], s.p)]
-        pairs += [(self.state[a + 1][..., : nz + (a == 2)], s.velocity[a]) for a in range(3)]
-        if self.psi is not None:
-            pairs.append((self.psi[..., :nz], s.relaxation_states))
-        return pairs
-
-    def load(self) -> None:
-        """Copy the room's state into the padded rows."""
-        for padded, room in self._views():
-            padded.copy_(room)
-
-    def store(self) -> None:
-        """Copy the padded rows' state back into the room's tensors."""
-        for padded, room in self._views():
-            room.copy_(padded)
-
-    def descriptor(self, recording) -> _native.Desc:
-        s = self.solver
-        d = s._descriptor(recording, tape=False)
-        d.z_pitch = self.zp
-        d.p, d.vx, d.vy, d.vz = (t.data_ptr() for t in self.state)
-        if self.psi is not None:
-            d.psi = self.psi.data_ptr()
-        if self.swap is not None:
-            d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.swap)
-        if self.psi_swap is not None:
-            d.psi_swap = self.psi_swap.data_ptr()
-        if s._sources:
-            d.source_cell = self._cells([(i, jk) for i, jk, _ in s._sources]).data_ptr()
-        if recording is not None and s._receivers:
-            d.receiver_cell = self._cells(s._receivers).data_ptr()
-        return d
-
-
 class _Group:
     """One library batch handle over ``solvers``; rooms whose nz the lane width does not
-    divide advance in padded rows (:class:`_PaddedRoom`)."""
+    divide advance in padded rows (:class:`~ac_fdtd_amd.scheme.PaddedRows`)."""
 
     def __init__(self, solvers: list[AcousticFDTD], lib) -> None:
         self.solvers = solvers
-        self._padded = [_PaddedRoom(s) if padded_row(s) != s.grid.nz else None for s in solvers]
+        self._padded = [PaddedRows(s) if padded_row(s) != s.grid.nz else None for s in solvers]
         self.device = solvers[0].device
         self._lib = lib
         self._handle = None
@@ -223,12 +142,22 @@ def _bind(lib) -> None:
     lib.acfdtd_hip_batch_destroy.argtypes = [ctypes.c_void_p]
 
 
+def _kernel_key(solver: AcousticFDTD) -> tuple:
+    """What rooms of one launch must share: the kernel's template (relaxation count, sponge,
+    compensation) and the workgroup shape."""
+    return (solver.coefficients.n_relax, bool(solver.coefficients.has_layer), solver.compensated,
+            solver.tile_j, solver._psi_swap is not None)
+
+
 class BatchedRooms:
-    """Advance ``solvers`` together. They must share device, precision, relaxation count,
-    sponge presence, compensation and tiling, use the fused path, and be at the same step.
+    """Advance ``solvers`` together. They must share device and precision, use the fused path,
+    and be at the same step.
 
     Every room takes the widest lanes of the precision (rooms whose nz it does not divide in
-    padded rows, see :func:`padded_row`), so all rooms advance in one launch per two steps.
+    padded rows, see :func:`padded_row`). Rooms that share the kernel's configuration
+    (relaxation count, sponge presence, compensation, tiling) advance in one launch per two
+    steps, one launch per configuration: the 64 rooms of one air model and wall model, whatever
+    their sizes, in one.
     """
 
     def __init__(self, solvers: list[AcousticFDTD]) -> None:
@@ -246,9 +175,13 @@ class BatchedRooms:
         self._tables = None  # the CapturedBatch whose buffers the room tables point at, if any
         self._lib = _native.library()
         _bind(self._lib)
+        by_kernel: dict[tuple, list[AcousticFDTD]] = {}
+        for solver in self.solvers:
+            by_kernel.setdefault(_kernel_key(solver), []).append(solver)
         self._groups = []
         try:
-            self._groups.append(_Group(self.solvers, self._lib))
+            for group in by_kernel.values():
+                self._groups.append(_Group(group, self._lib))
         except Exception:
             self.close()
             raise
@@ -259,7 +192,8 @@ class BatchedRooms:
 
     @property
     def two_step(self) -> list[bool]:
-        """Per launch group (one: every room), whether it advances two steps per sweep."""
+        """Per launch group (one per kernel configuration), whether it advances two steps per
+        sweep."""
         return [group.two_step for group in self._groups]
 
     @property

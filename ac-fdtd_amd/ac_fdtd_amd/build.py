@@ -45,6 +45,8 @@ SOURCES = (
     "fused2s_64.hip",
     "fused2d.hip",
     "fused2d_64.hip",
+    "fused2p.hip",
+    "fused2p_64.hip",
     "fused2_v2.hip",
     "fused2_v1.hip",
     "fused2_64_v1.hip",

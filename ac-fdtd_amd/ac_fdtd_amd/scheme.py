@@ -33,7 +33,7 @@ from .coefficients import build_coefficients
 from .grid import Grid
 from .medium import AIR, Medium
 
-__all__ = ["AcousticFDTD", "CapturedSteps", "synchronize"]
+__all__ = ["AcousticFDTD", "CapturedSteps", "PaddedRows", "padded_row", "synchronize"]
 
 _TORCH_DTYPES = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
 
@@ -160,6 +160,10 @@ class AcousticFDTD:
             self._wall_mask = int(sum(1 << c for c in range(64) if k.wall_is_wall[c]))
         self._wall_tape: tuple[torch.Tensor, int] | None = None
         self._div_tape: tuple[torch.Tensor, int] | None = None
+        # A grid whose nz the widest lane does not divide runs in padded rows (PaddedRows,
+        # allocated at the first run long enough to use them); False keeps the reference layout.
+        self.padded_rows = True
+        self._padded = None
         self._cell = self._face = None
         if k.has_layer:
             self._cell = [torch.as_tensor(x, **opts) for x in k.cell_profiles]
@@ -272,7 +276,11 @@ class AcousticFDTD:
         recording = None
         if self._receivers and record:
             recording = torch.zeros((len(self._receivers), n_steps), **self._opts)
-        desc = self._descriptor(recording, tape=record, div=True)
+        padded = self._padded_rows(n_steps, record)
+        if padded is not None:
+            desc = padded.descriptor(recording)
+        else:
+            desc = self._descriptor(recording, tape=record, div=True)
         step = self.step_index
 
         def launch() -> None:
@@ -280,14 +288,34 @@ class AcousticFDTD:
                 raise RuntimeError("the solver advanced since this run was prepared")
             stream = torch.cuda.current_stream(self.device).cuda_stream
             with torch.cuda.device(self.device):
+                if padded is not None:
+                    padded.load()
                 status = self._lib.acfdtd_hip_run(
                     ctypes.byref(desc), step, n_steps, 0, ctypes.c_void_p(stream))
-            _native.check(status, "acfdtd_hip_run")
+                _native.check(status, "acfdtd_hip_run")
+                if padded is not None:
+                    padded.store()
             self.step_index += n_steps
             if recording is not None:
                 self._runs.append(recording)
 
         return launch
+
+    #: Runs shorter than this stay in the reference layout: the padded rows' copies in and out
+    #: would cost more than the wider lanes save.
+    PADDED_MIN_STEPS = 8
+
+    def _padded_rows(self, n_steps: int, record: bool):
+        """The padded rows this run takes (allocated on first use), or None: only the fused path
+        of a grid whose nz the widest lane does not divide, with no tape to write (tapes keep the
+        reference layout), for at least PADDED_MIN_STEPS steps."""
+        if (not self.padded_rows or self._swap is None or n_steps < self.PADDED_MIN_STEPS
+                or padded_row(self) == self.grid.nz or self._div_tape is not None
+                or (record and self._wall_tape is not None)):
+            return None
+        if self._padded is None:
+            self._padded = PaddedRows(self)
+        return self._padded
 
     def capture(self, n_steps: int, record: bool = True) -> "CapturedSteps":
         """Capture steps ``step_index`` .. ``step_index + n_steps - 1`` of :meth:`run` into one
@@ -549,6 +577,90 @@ class AcousticFDTD:
                 setattr(d, name, self._checked(self._swap[a + 1], g.velocity_shape(a), name))
         if self._psi_swap is not None:
             d.psi_swap = self._checked(self._psi_swap, (k.n_relax, *g.shape), "psi swap")
+        return d
+
+
+def padded_row(solver) -> int:
+    """Row length of ``solver``'s fields in padded rows: nz rounded up to the widest lane of its
+    precision (4 cells fp32, 2 fp64); nz itself when that width divides it."""
+    widest = 4 if solver.np_dtype.itemsize == 4 else 2
+    return -(-solver.grid.nz // widest) * widest
+
+
+class PaddedRows:
+    """A solver's state, swap set and cell tables in rows of ``padded_row`` elements (every
+    field, vz included): the layout ``acfdtd_hip_run`` and the batch API take as the
+    descriptor's ``z_pitch``, so that a room whose nz is not a multiple of the widest lane (4
+    cells fp32, 2 fp64) advances on full-width lanes. The elements past the room's own in each
+    row are padding, which the kernels never let reach a cell. :meth:`load` and :meth:`store`
+    copy the state in and out (stream-ordered device copies)."""
+
+    def __init__(self, solver) -> None:
+        self.solver = solver
+        g = solver.grid
+        self.zp = padded_row(solver)
+        opts = solver._opts
+        zp = self.zp
+
+        def fields():
+            return [torch.zeros((g.nx, g.ny, zp), **opts),
+                    torch.zeros((g.nx + 1, g.ny, zp), **opts),
+                    torch.zeros((g.nx, g.ny + 1, zp), **opts),
+                    torch.zeros((g.nx, g.ny, zp), **opts)]
+
+        n_relax = solver.coefficients.n_relax
+        self.state = fields()
+        self.psi = torch.zeros((n_relax, g.nx, g.ny, zp), **opts) if n_relax else None
+        self.swap = fields() if solver._swap is not None else None
+        self.psi_swap = (torch.zeros((n_relax, g.nx, g.ny, zp), **opts)
+                         if solver._psi_swap is not None else None)
+        self._tables: dict[tuple, torch.Tensor] = {}
+
+    def _cells(self, cells) -> torch.Tensor:
+        """(i, j * nz + k) pairs re-addressed to rows of zp: (i, j * zp + k), on the device
+        (kept, so that a table a captured graph's room table points at stays alive)."""
+        key = tuple(cells)
+        if key not in self._tables:
+            nz = self.solver.grid.nz
+            table = np.array([(i, (jk // nz) * self.zp + jk % nz) for i, jk in cells],
+                             dtype=np.int32).ravel()
+            self._tables[key] = torch.as_tensor(table, device=self.solver.device)
+        return self._tables[key]
+
+    def _views(self):
+        """(padded view, room tensor) pairs of the state that crosses a run's boundaries."""
+        s, nz = self.solver, self.solver.grid.nz
+        pairs = [(self.state[0][..., :nz], s.p)]
+        pairs += [(self.state[a + 1][..., : nz + (a == 2)], s.velocity[a]) for a in range(3)]
+        if self.psi is not None:
+            pairs.append((self.psi[..., :nz], s.relaxation_states))
+        return pairs
+
+    def load(self) -> None:
+        """Copy the room's state into the padded rows."""
+        for padded, room in self._views():
+            padded.copy_(room)
+
+    def store(self) -> None:
+        """Copy the padded rows' state back into the room's tensors."""
+        for padded, room in self._views():
+            room.copy_(padded)
+
+    def descriptor(self, recording) -> _native.Desc:
+        s = self.solver
+        d = s._descriptor(recording, tape=False)
+        d.z_pitch = self.zp
+        d.p, d.vx, d.vy, d.vz = (t.data_ptr() for t in self.state)
+        if self.psi is not None:
+            d.psi = self.psi.data_ptr()
+        if self.swap is not None:
+            d.p_swap, d.vx_swap, d.vy_swap, d.vz_swap = (t.data_ptr() for t in self.swap)
+        if self.psi_swap is not None:
+            d.psi_swap = self.psi_swap.data_ptr()
+        if s._sources:
+            d.source_cell = self._cells([(i, jk) for i, jk, _ in s._sources]).data_ptr()
+        if recording is not None and s._receivers:
+            d.receiver_cell = self._cells(s._receivers).data_ptr()
         return d
 
 

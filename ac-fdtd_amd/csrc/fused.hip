@@ -13,7 +13,8 @@ namespace acfdtd {
 
 
 
-template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED>
+template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED,
+          bool PADDED = false>
 __global__ __launch_bounds__(512) void fused_step_vec(const SweepArgs<real> a,
                                                       const FusedIO<real> io, const int64_t step,
                                                       const int64_t column) {
@@ -22,7 +23,7 @@ __global__ __launch_bounds__(512) void fused_step_vec(const SweepArgs<real> a,
     if (!flat_lane(a.tiles, a.ny, a.zp, jb, j, k0)) return;
     const int i0 = xc * a.tiles.chunk;
     const int i1 = min(i0 + a.tiles.chunk, a.nx);
-    fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED>(
+    fused_march_vec<real, V, NR, WALLS, LAYER, COMPENSATED, PADDED>(
         a, i0, i1, j, k0, step, column, io.p_in, io.vx_in, io.vy_in, io.vz_in, io.p_out,
         io.vx_out, io.vy_out, io.vz_out, a.psi, a.wall_table, a.cell[0], a.face[0], a.cell[1],
         a.face[1], a.cell[2], a.face[2], a.recording);
@@ -33,42 +34,48 @@ template <typename real>
 using FusedKernel = void (*)(const SweepArgs<real>, const FusedIO<real>, const int64_t,
                              const int64_t);
 
-template <typename real, int V, int NR, bool W, bool L, bool C>
+template <typename real, int V, int NR, bool W, bool L, bool C, bool P>
 FusedKernel<real> fused_kernel() {
-    return fused_step_vec<real, V, NR, W, L, C>;
+    return fused_step_vec<real, V, NR, W, L, C, P>;
 }
 
 // Compensated kernels exist for fp32 only, and 4-wide lanes only for fp32 (16 bytes).
-template <typename real, int V, int NR>
+template <typename real, int V, int NR, bool P>
 FusedKernel<real> pick_fused_nr(bool walls, bool layer, bool comp) {
     if constexpr (sizeof(real) == 4) {
         if (comp) {
-            if (walls) return layer ? fused_kernel<real, V, NR, true, true, true>()
-                                    : fused_kernel<real, V, NR, true, false, true>();
-            return layer ? fused_kernel<real, V, NR, false, true, true>()
-                         : fused_kernel<real, V, NR, false, false, true>();
+            if (walls) return layer ? fused_kernel<real, V, NR, true, true, true, P>()
+                                    : fused_kernel<real, V, NR, true, false, true, P>();
+            return layer ? fused_kernel<real, V, NR, false, true, true, P>()
+                         : fused_kernel<real, V, NR, false, false, true, P>();
         }
     }
     (void)comp;
-    if (walls) return layer ? fused_kernel<real, V, NR, true, true, false>()
-                            : fused_kernel<real, V, NR, true, false, false>();
-    return layer ? fused_kernel<real, V, NR, false, true, false>()
-                 : fused_kernel<real, V, NR, false, false, false>();
+    if (walls) return layer ? fused_kernel<real, V, NR, true, true, false, P>()
+                            : fused_kernel<real, V, NR, true, false, false, P>();
+    return layer ? fused_kernel<real, V, NR, false, true, false, P>()
+                 : fused_kernel<real, V, NR, false, false, false, P>();
 }
 
-template <typename real, int V>
+template <typename real, int V, bool P = false>
 FusedKernel<real> pick_fused_v(int nr, bool walls, bool layer, bool comp) {
     switch (nr) {
-        case 0: return pick_fused_nr<real, V, 0>(walls, layer, comp);
-        case 1: return pick_fused_nr<real, V, 1>(walls, layer, comp);
-        case 2: return pick_fused_nr<real, V, 2>(walls, layer, comp);
-        case 3: return pick_fused_nr<real, V, 3>(walls, layer, comp);
-        default: return pick_fused_nr<real, V, 4>(walls, layer, comp);
+        case 0: return pick_fused_nr<real, V, 0, P>(walls, layer, comp);
+        case 1: return pick_fused_nr<real, V, 1, P>(walls, layer, comp);
+        case 2: return pick_fused_nr<real, V, 2, P>(walls, layer, comp);
+        case 3: return pick_fused_nr<real, V, 3, P>(walls, layer, comp);
+        default: return pick_fused_nr<real, V, 4, P>(walls, layer, comp);
     }
 }
 
+// Padded rows (z_pitch) take the kernels of the widest lanes (4 cells fp32, 2 fp64) only; null
+// when the pitch allows narrower lanes alone (the caller then runs the in-place sweeps).
 template <typename real>
-FusedKernel<real> pick_fused(int vec, int nr, bool walls, bool layer, bool comp) {
+FusedKernel<real> pick_fused(int vec, int nr, bool walls, bool layer, bool comp,
+                             bool padded = false) {
+    constexpr int widest = sizeof(real) == 4 ? 4 : 2;
+    if (padded) return vec == widest ? pick_fused_v<real, widest, true>(nr, walls, layer, comp)
+                                     : nullptr;
     if constexpr (sizeof(real) == 4) {
         if (vec == 4) return pick_fused_v<real, 4>(nr, walls, layer, comp);
     }
@@ -83,8 +90,9 @@ int64_t launch_fused_pairs(const acfdtd_hip_desc &d, int64_t step_offset, int64_
     fill_args(d, f);
     f.tiles = fused_tiling(d);
     const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
-    const FusedKernel<real> fk =
-        pick_fused<real>(f.tiles.vec, d.n_relax, d.wall_mask != 0, d.has_layer != 0, comp);
+    const FusedKernel<real> fk = pick_fused<real>(f.tiles.vec, d.n_relax, d.wall_mask != 0,
+                                                  d.has_layer != 0, comp, d.z_pitch > 0);
+    if (fk == nullptr) return 0;
     const FusedIO<real> forward = fused_io<real>(d, false), backward = fused_io<real>(d, true);
     const dim3 grid(f.tiles.launched), block(64, f.tiles.tile_j);
     int64_t n = 0;

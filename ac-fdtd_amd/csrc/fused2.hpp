@@ -86,7 +86,7 @@ __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fus
 // more scalar registers here (about 800 more lane reads of spilled scalars per kernel), which
 // measured 6 % slower at 256^3.
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED,
-          bool DIVT = false>
+          bool DIVT = false, bool PADDED = false>
 __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused2_step(const SweepArgs<real> a,
                                                              const Fused2IO<real> io,
                                                              const Tiling2 t2, const int64_t step,
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     }
     for (int q = t; q < 2 * a.n_receivers; q += (int)blockDim.x) rx_lds[q] = a.receiver_cell[q];
     lds_barrier();
-    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED, false, DIVT>(
+    fused2_march<real, V, NR, WALLS, LAYER, COMPENSATED, STAGED, PADDED, DIVT>(
         a, t2, i0, i1, j, k0, valid, owner, t, step, column, io.p_in, io.vx_in, io.vy_in,
         io.vz_in, io.psi_in, io.p_out, io.vx_out, io.vy_out, io.vz_out, io.psi_out, lds, src_lds,
         len_lds, now_lds, rx_lds, stage);
@@ -145,21 +145,21 @@ template <typename real>
 using Fused2Kernel = void (*)(const SweepArgs<real>, const Fused2IO<real>, const Tiling2,
                               const int64_t, const int64_t);
 
-template <typename real, int V, int NR, bool STAGED, bool DIVT = false>
+template <typename real, int V, int NR, bool STAGED, bool DIVT = false, bool PADDED = false>
 Fused2Kernel<real> pick_fused2_nr(bool walls, bool layer, bool comp) {
     if constexpr (sizeof(real) == 4) {
         if (comp) {
-            if (walls) return layer ? fused2_step<real, V, NR, true, true, true, STAGED, DIVT>
-                                    : fused2_step<real, V, NR, true, false, true, STAGED, DIVT>;
-            return layer ? fused2_step<real, V, NR, false, true, true, STAGED, DIVT>
-                         : fused2_step<real, V, NR, false, false, true, STAGED, DIVT>;
+            if (walls) return layer ? fused2_step<real, V, NR, true, true, true, STAGED, DIVT, PADDED>
+                                    : fused2_step<real, V, NR, true, false, true, STAGED, DIVT, PADDED>;
+            return layer ? fused2_step<real, V, NR, false, true, true, STAGED, DIVT, PADDED>
+                         : fused2_step<real, V, NR, false, false, true, STAGED, DIVT, PADDED>;
         }
     }
     (void)comp;
-    if (walls) return layer ? fused2_step<real, V, NR, true, true, false, STAGED, DIVT>
-                            : fused2_step<real, V, NR, true, false, false, STAGED, DIVT>;
-    return layer ? fused2_step<real, V, NR, false, true, false, STAGED, DIVT>
-                 : fused2_step<real, V, NR, false, false, false, STAGED, DIVT>;
+    if (walls) return layer ? fused2_step<real, V, NR, true, true, false, STAGED, DIVT, PADDED>
+                            : fused2_step<real, V, NR, true, false, false, STAGED, DIVT, PADDED>;
+    return layer ? fused2_step<real, V, NR, false, true, false, STAGED, DIVT, PADDED>
+                 : fused2_step<real, V, NR, false, false, false, STAGED, DIVT, PADDED>;
 }
 
 // The kernel for a lane width V (cells per lane), plain or staged.
@@ -171,6 +171,18 @@ Fused2Kernel<real> pick_fused2_kind(int nr, bool walls, bool layer, bool comp) {
         case 2: return pick_fused2_nr<real, V, 2, STAGED>(walls, layer, comp);
         case 3: return pick_fused2_nr<real, V, 3, STAGED>(walls, layer, comp);
         default: return pick_fused2_nr<real, V, 4, STAGED>(walls, layer, comp);
+    }
+}
+
+// The plain sweep on padded rows (z_pitch), for the widest lanes (V = 4 fp32, 2 fp64).
+template <typename real, int V>
+Fused2Kernel<real> pick_fused2_padded(int nr, bool walls, bool layer, bool comp) {
+    switch (nr) {
+        case 0: return pick_fused2_nr<real, V, 0, false, false, true>(walls, layer, comp);
+        case 1: return pick_fused2_nr<real, V, 1, false, false, true>(walls, layer, comp);
+        case 2: return pick_fused2_nr<real, V, 2, false, false, true>(walls, layer, comp);
+        case 3: return pick_fused2_nr<real, V, 3, false, false, true>(walls, layer, comp);
+        default: return pick_fused2_nr<real, V, 4, false, false, true>(walls, layer, comp);
     }
 }
 
@@ -197,12 +209,15 @@ extern template Fused2Kernel<double> pick_fused2_kind<double, 2, true>(int, bool
 extern template Fused2Kernel<double> pick_fused2_kind<double, 1, false>(int, bool, bool, bool);
 extern template Fused2Kernel<float> pick_fused2_divt<float, 4>(int, bool, bool, bool);
 extern template Fused2Kernel<double> pick_fused2_divt<double, 2>(int, bool, bool, bool);
+extern template Fused2Kernel<float> pick_fused2_padded<float, 4>(int, bool, bool, bool);
+extern template Fused2Kernel<double> pick_fused2_padded<double, 2>(int, bool, bool, bool);
 
 // Whether the staged variant may run: asked for, 16-byte lanes (4 cells fp32, 2 fp64), and
 // every field it stages 16-byte aligned in both sets (otherwise the plain two-step sweep runs,
 // with the same results).
 inline bool staged_applies(const acfdtd_hip_desc &d, const int vec) {
-    if (!(d.flags & ACFDTD_HIP_STAGED) || vec * d.real_bytes != 16 || d.div_tape) return false;
+    if (!(d.flags & ACFDTD_HIP_STAGED) || vec * d.real_bytes != 16 || d.div_tape || d.z_pitch)
+        return false;
     const void *fields[] = {d.p, d.vx, d.vy, d.p_swap, d.vx_swap, d.vy_swap};
     for (const void *f : fields)
         if (reinterpret_cast<uintptr_t>(f) % 16 != 0) return false;
@@ -219,6 +234,11 @@ Fused2Kernel<real> pick_fused2(const acfdtd_hip_desc &d) {
     const bool comp = (d.flags & ACFDTD_HIP_COMPENSATED) != 0;
     const bool walls = d.wall_mask != 0, layer = d.has_layer != 0;
     const bool staged = staged_applies(d, vec);
+    if (d.z_pitch > 0) {
+        // fused2_rows_apply: the widest lanes only.
+        if constexpr (sizeof(real) == 4) return pick_fused2_padded<real, 4>(d.n_relax, walls, layer, comp);
+        else return pick_fused2_padded<real, 2>(d.n_relax, walls, layer, comp);
+    }
     if (d.div_tape) {
         // fused2_divt_applies: the widest lanes only.
         if constexpr (sizeof(real) == 4) return pick_fused2_divt<real, 4>(d.n_relax, walls, layer, comp);
@@ -259,9 +279,10 @@ void fused2_io(const acfdtd_hip_desc &d, Fused2IO<real> &forward, Fused2IO<real>
     backward.psi_in = forward.psi_out;
 }
 
-// With a divergence tape the two-step sweep has kernels for the widest lanes only.
+// With a divergence tape, and with padded rows, the two-step sweep has kernels for the widest
+// lanes only.
 inline bool fused2_divt_applies(const acfdtd_hip_desc &d) {
-    return d.div_tape == nullptr || fused_vec(d) * d.real_bytes == 16;
+    return (d.div_tape == nullptr && d.z_pitch == 0) || fused_vec(d) * d.real_bytes == 16;
 }
 
 template <typename real>

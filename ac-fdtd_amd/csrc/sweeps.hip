@@ -277,7 +277,7 @@ extern "C" const char *acfdtd_hip_last_error(void) { return g_last_error.c_str()
 
 extern "C" int acfdtd_hip_run(const acfdtd_hip_desc *desc, int64_t step_offset, int64_t n_steps,
                               int64_t recording_column, void *stream) {
-    const int status = validate(desc);
+    const int status = validate(desc, true);
     if (status != ACFDTD_HIP_OK) return status;
     if (n_steps < 0 || step_offset < 0 || recording_column < 0)
         return fail(ACFDTD_HIP_EINVAL, "n_steps, step_offset and recording_column must be >= 0");

@@ -287,7 +287,8 @@ inline int validate(const acfdtd_hip_desc *d, const bool padded = false) {
     if (d->reserved != 0) return fail(ACFDTD_HIP_EINVAL, "reserved descriptor field must be 0");
     if (d->z_pitch != 0) {
         if (!padded)
-            return fail(ACFDTD_HIP_EINVAL, "padded rows (z_pitch) are taken by the batch API only");
+            return fail(ACFDTD_HIP_EINVAL,
+                        "padded rows (z_pitch) are taken by acfdtd_hip_run and the batch API only");
         if (d->z_pitch <= d->nz)
             return fail(ACFDTD_HIP_EINVAL, "z_pitch must be 0 or greater than nz");
         if (d->wall_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no wall tape");

@@ -130,8 +130,8 @@ typedef struct acfdtd_hip_desc {
      * sweep. Results (boundary tape included) are bit-identical either way. */
     void *psi_swap;
 
-    /* Padded rows, accepted by the batch API only (the single-room entry points return
-     * ACFDTD_HIP_EINVAL unless it is 0). 0: the layout above. Otherwise z_pitch > nz is the row
+    /* Padded rows, accepted by acfdtd_hip_run and the batch API only (the other entry points
+     * return ACFDTD_HIP_EINVAL unless it is 0). 0: the layout above. Otherwise z_pitch > nz is the row
      * length, in elements, of every field and swap buffer, vz included: p and each psi field
      * nx*ny*z_pitch, vx (nx+1)*ny*z_pitch, vy nx*(ny+1)*z_pitch, vz nx*ny*z_pitch, cell (i, j, k)
      * at (i*ny + j)*z_pitch + k, vz face k of row (i, j) at (i*ny + j)*z_pitch + k; source and
@@ -139,7 +139,8 @@ typedef struct acfdtd_hip_desc {
      * k >= nz + 1 for vz) belong to no cell: the sweeps read and write them but never let them
      * reach a cell, so their contents are arbitrary. It lets a room whose nz is not a multiple
      * of the lane width (4 cells fp32, 2 fp64) take full-width lanes: z_pitch a multiple of
-     * that width. */
+     * that width (with another pitch acfdtd_hip_run takes the two in-place sweeps). No boundary
+     * or divergence tape, no staged sweep with padded rows. */
     int32_t z_pitch;
     int32_t reserved; /* 0 */
 

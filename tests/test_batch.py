@@ -65,13 +65,22 @@ def test_batch_equals_rooms_alone(dtype, layer, shapes):
         np.testing.assert_array_equal(b.recorded_pressure, a.recorded_pressure)
 
 
-def test_batch_refuses_mixed_rooms():
+def test_batch_refuses_mixed_precisions_and_groups_mixed_kernels():
+    """Rooms of different precisions are refused; rooms whose kernels differ (here sponge
+    presence) advance in one launch group each, still each as if alone."""
     from ac_fdtd_amd.batch import BatchedRooms
 
     a = _room((8, 8, 8), 0, np.float64, False)
     b = _room((8, 8, 8), 1, np.float32, False)
     with pytest.raises(ValueError, match="dtype"):
         BatchedRooms([a, b])
-    c = _room((8, 8, 8), 2, np.float64, True)
-    with pytest.raises(RuntimeError, match="sponge presence"):
-        BatchedRooms([a, c])
+    c = _room((9, 8, 7), 2, np.float64, True)
+    alone = [_room((8, 8, 8), 0, np.float64, False), _room((9, 8, 7), 2, np.float64, True)]
+    batch = BatchedRooms([a, c])
+    assert batch.two_step == [True, True]
+    batch.run(9)
+    batch.close()
+    for solver, reference in zip((a, c), alone):
+        reference.run(9)
+        np.testing.assert_array_equal(solver.p.cpu().numpy(), reference.p.cpu().numpy())
+        np.testing.assert_array_equal(solver.recorded_pressure, reference.recorded_pressure)

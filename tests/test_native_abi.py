@@ -80,7 +80,6 @@ def _desc(**overrides):
         ({"has_layer": 1}, EINVAL, "sponge profile"),
         ({"n_sources": 1}, EINVAL, "source arrays"),
         ({"n_receivers": 2}, EINVAL, "receiver arrays"),
-        ({"z_pitch": 8}, EINVAL, "batch API only"),
         ({"reserved": 1}, EINVAL, "reserved"),
     ],
 )
@@ -124,11 +123,19 @@ def test_short_wall_tapes_are_refused_before_any_launch(lib):
 
 
 def test_padded_rows_are_checked_by_the_batch_api(lib):
-    """z_pitch (padded rows) is a batch-only layout: it must exceed nz and takes no wall tape;
-    a valid one is accepted, and a padded room takes lanes as wide as the pitch allows."""
+    """z_pitch (padded rows) is a layout of acfdtd_hip_run and the batch API alone: the other
+    entry points refuse it; it must exceed nz and takes no wall tape; a valid one is accepted,
+    and a padded room takes lanes as wide as the pitch allows."""
     from ac_fdtd_amd.batch import _bind
 
     _bind(lib)
+    padded = _desc(nz=5, z_pitch=8)
+    assert lib.acfdtd_hip_energy(ctypes.byref(padded), None, None, None) == EINVAL
+    assert "acfdtd_hip_run and the batch API only" in lib.acfdtd_hip_last_error().decode()
+    assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(padded), 0, 0, None) == EINVAL
+    assert lib.acfdtd_hip_fused2_step(ctypes.byref(padded), 0, 0, 0, None) == EINVAL
+    assert lib.acfdtd_hip_run(ctypes.byref(_desc(nz=5, z_pitch=5)), 0, 1, 0, None) == EINVAL
+    assert "greater than nz" in lib.acfdtd_hip_last_error().decode()
     swaps = {"p_swap": 4096, "vx_swap": 4096, "vy_swap": 4096, "vz_swap": 4096}
     rooms = (_native.Desc * 2)(_desc(nz=4, **swaps), _desc(nz=5, z_pitch=8, **swaps))
     assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 2) > 0
