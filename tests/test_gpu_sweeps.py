@@ -126,6 +126,31 @@ def test_running_in_pieces_is_bitwise_running_at_once():
     np.testing.assert_array_equal(pieces.recorded_pressure, whole.recorded_pressure)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("nz", [13, 14, 15, 17])
+def test_padded_rows_are_the_reference_layout_bit_for_bit(nz, dtype):
+    """A grid whose nz the widest lane does not divide runs in padded rows (full-width lanes,
+    state copied in and out): fields, relaxation states and recordings equal the same run in
+    the reference layout and the oracle's, for run lengths with every remainder."""
+    grid = Grid(dx=0.03, nx=16, ny=12, nz=nz)
+    config = dict(PARITY["all"])
+    padded = prepare(hip(grid, dtype=dtype, **config), grid)
+    plain = prepare(hip(grid, dtype=dtype, **config), grid)
+    oracle = prepare(OracleFDTD(grid, dtype=dtype, **config), grid)
+    plain.padded_rows = False
+    for n in (9, 10, 11, 12):
+        padded.run(n)
+        plain.run(n)
+        oracle.run(n)
+    widest = 4 if np.dtype(dtype).itemsize == 4 else 2
+    assert (padded._padded is not None) == (nz % widest != 0) and plain._padded is None
+    assert_same_state(padded, oracle)
+    for a, b in zip(padded.save_state(), plain.save_state()):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    np.testing.assert_array_equal(padded.recorded_pressure, oracle.recorded_pressure)
+    np.testing.assert_array_equal(padded.recorded_pressure, plain.recorded_pressure)
+
+
 def test_a_prepared_run_is_run():
     """prepare_run + its launch (what the bench times) is run() bit for bit; a launch after the
     solver moved on, and the subclasses that advance their own scheme, are refused."""
