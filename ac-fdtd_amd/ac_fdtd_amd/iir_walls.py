@@ -29,11 +29,14 @@ __all__ = ["IIRWallFDTD"]
 
 
 class IIRWallFDTD(AcousticFDTD):
-    """The reference solver's API with frequency-dependent walls (``walls``)."""
+    """The reference solver's API with frequency-dependent walls (``walls``). ``fused``
+    (default) advances the steps in fused pairs, one sweep per step between two sets of fields;
+    False, the two in-place sweeps per step. The same results either way."""
 
     def __init__(self, grid, medium=AIR, courant: float = 1.0, dtype=np.float64,
                  walls: FrequencyDependentWalls | None = None, absorbing_layer=None,
-                 air_absorption=None, device="cuda", tile_j: int = 0, chunk_x: int = 0):
+                 air_absorption=None, device="cuda", tile_j: int = 0, chunk_x: int = 0,
+                 fused: bool = True):
         if absorbing_layer is not None:
             raise ValueError("frequency-dependent walls run without a sponge, which would damp "
                              "the walls' memory held on the wall faces")
@@ -43,7 +46,7 @@ class IIRWallFDTD(AcousticFDTD):
             raise ValueError(f"at most {_native.MAX_BRANCHES} branches per wall")
         super().__init__(grid, medium, courant, dtype,
                          WallAdmittances(*self.iir_walls.instantaneous(dt)), None,
-                         air_absorption, device, fused=False, tile_j=tile_j, chunk_x=chunk_x,
+                         air_absorption, device, fused=fused, tile_j=tile_j, chunk_x=chunk_x,
                          time_block=1)
         self._branches = self.iir_walls.branch_count()
         self._face_mask = self.iir_walls.face_mask() if self._branches else 0

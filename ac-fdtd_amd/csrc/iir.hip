@@ -1,10 +1,13 @@
 // Frequency-dependent (IIR) walls on gfx950: the wall update that follows each step.
 //
-// The step itself is the first-order scheme's two in-place sweeps (sweeps.hip) with the walls'
-// instantaneous admittance in the wall table and the boundary tape written; this kernel then
-// runs each face cell's branch filters on the step's mean pressure and writes the branches'
-// memory as the wall face's velocity for the next step (include/acfdtd_hip.h,
-// acfdtd_hip_iir_desc). One lane per face cell in the tape's face-plane order, so the tape
+// The step itself is the first-order scheme's fused step (fused.hip: one sweep from one set of
+// fields to the other, in pairs A -> B -> A, when the descriptor has the swap set; an odd last
+// step, or a descriptor without it, takes the two in-place sweeps of sweeps.hip) with the
+// walls' instantaneous admittance in the wall table and the boundary tape written; this kernel
+// then runs each face cell's branch filters on the step's mean pressure and writes the branches'
+// memory as the wall face's velocity for the next step, into the set that holds the step's
+// result (include/acfdtd_hip.h, acfdtd_hip_iir_desc). The fused step copies every wall face
+// from one set to the other, so faces without a filter keep their values either way. One lane per face cell in the tape's face-plane order, so the tape
 // reads, the state reads and writes (2 per branch, contiguous per cell) and the wall-face
 // stores of x and y faces are coalesced; 2 (ny nz + nx nz + nx ny) cells per step, a few
 // hundred kilobytes against the sweeps' hundreds of megabytes. The CPU oracle restates this
@@ -120,15 +123,38 @@ int iir_run(const acfdtd_hip_desc &desc, const acfdtd_hip_iir_desc &iir, int64_t
     a.z_hi = planes.z_hi;
     a.stride = planes.stride;
     const unsigned blocks = (unsigned)((planes.stride + 255) / 256);
+    const bool filters = a.face_mask != 0 && a.n_branches > 0;
     acfdtd_hip_desc d = desc;
     d.wall_tape_rows = 1;
-    for (int64_t n = 0; n < n_steps; ++n) {
+    int64_t n = 0;
+    if (desc.p_swap && desc.vx_swap && desc.vy_swap && desc.vz_swap) {
+        real *const first[3] = {static_cast<real *>(desc.vx), static_cast<real *>(desc.vy),
+                                static_cast<real *>(desc.vz)};
+        real *const second[3] = {static_cast<real *>(desc.vx_swap),
+                                 static_cast<real *>(desc.vy_swap),
+                                 static_cast<real *>(desc.vz_swap)};
+        for (; n + 1 < n_steps; n += 2) {
+            for (int h = 0; h < 2; ++h) {
+                d.wall_tape_step0 = step_offset + n + h;
+                const int rc = acfdtd_hip_fused_step(&d, step_offset + n + h, column + n + h, h, s);
+                if (rc != ACFDTD_HIP_OK) return rc;
+                real *const *out = h == 0 ? second : first;  // the step's result
+                a.vx = out[0];
+                a.vy = out[1];
+                a.vz = out[2];
+                if (filters) hipLaunchKernelGGL(iir_wall_update<real>, dim3(blocks), dim3(256), 0, s, a);
+            }
+        }
+        a.vx = first[0];
+        a.vy = first[1];
+        a.vz = first[2];
+    }
+    for (; n < n_steps; ++n) {
         d.wall_tape_step0 = step_offset + n;
         int rc = acfdtd_hip_velocity_sweep(&d, s);
         if (rc == ACFDTD_HIP_OK) rc = acfdtd_hip_pressure_sweep(&d, step_offset + n, column + n, s);
         if (rc != ACFDTD_HIP_OK) return rc;
-        if (a.face_mask != 0 && a.n_branches > 0)
-            hipLaunchKernelGGL(iir_wall_update<real>, dim3(blocks), dim3(256), 0, s, a);
+        if (filters) hipLaunchKernelGGL(iir_wall_update<real>, dim3(blocks), dim3(256), 0, s, a);
     }
     return check_hip(hipGetLastError(), "wall update launch");
 }

@@ -441,9 +441,13 @@ typedef struct acfdtd_hip_iir_desc {
     void *state;
 } acfdtd_hip_iir_desc;
 
-/* Advance n_steps: per step the velocity sweep, the pressure sweep writing the tape, and the
- * wall update. desc needs a wall tape of at least one row (used as scratch: the call writes
- * row 0 every step), no sponge, and the wall table of the instantaneous admittances. */
+/* Advance n_steps: per step the first-order step writing the tape, then the wall update. With
+ * the swap set in desc the steps go in fused pairs (one sweep from one set to the other, the
+ * wall update writing into the set that holds the result, the state back in the first set
+ * after each pair); an odd last step, or desc without the swap set, is the velocity sweep and
+ * the pressure sweep in place. Same results either way. desc needs a wall tape of at least one
+ * row (used as scratch: the call writes row 0 every step), no sponge, and the wall table of
+ * the instantaneous admittances. */
 int acfdtd_hip_iir_run(const acfdtd_hip_desc *desc, const acfdtd_hip_iir_desc *iir,
                        int64_t step_offset, int64_t n_steps, int64_t recording_column,
                        void *stream);

@@ -173,14 +173,15 @@ def _prepare(solver, grid, seed=0):
 @pytest.mark.parametrize("air", [False, True], ids=["no-air", "air"])
 @pytest.mark.parametrize("nz", [16, 15])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["fp32", "fp64"])
-def test_hip_iir_walls_are_bitwise_the_oracle(dtype, nz, air):
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
+def test_hip_iir_walls_are_bitwise_the_oracle(dtype, nz, air, fused):
     from ac_fdtd_amd.iir_walls import IIRWallFDTD
 
     grid = Grid(dx=0.03, nx=20, ny=17, nz=nz)
     walls = FrequencyDependentWalls(TABLES[0], 0.3, TABLES[1], 0.0, TABLES[2], TABLES[0])
     extra = {"air_absorption": AirAbsorption(18.0, 40.0)} if air else {}
     oracle = _prepare(OracleImpedanceFDTD(grid, dtype=dtype, walls=walls, **extra), grid)
-    solver = _prepare(IIRWallFDTD(grid, dtype=dtype, walls=walls, **extra), grid)
+    solver = _prepare(IIRWallFDTD(grid, dtype=dtype, walls=walls, fused=fused, **extra), grid)
     for n in (23, 40):
         oracle.run(n)
         solver.run(n)
