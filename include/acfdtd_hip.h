@@ -464,8 +464,11 @@ int acfdtd_hip_iir_run(const acfdtd_hip_desc *desc, const acfdtd_hip_iir_desc *i
  *   other cells: p_x = (p - p_y) - p_z; p_a = A_c,a p_a - B_c,a (pres Da) for each axis;
  *                p = (p_x + p_y) + p_z (p_y and p_z stored);
  *   then sources and receivers as acfdtd_hip_run. desc must have rigid walls (wall_mask 0),
- * no relaxation states, no sponge, no flags and no tape; swap buffers and time_block are
- * ignored (one velocity and one pressure sweep in place per step). No reference counterpart. */
+ * no relaxation states, no sponge, no flags and no tape. With the swap set in desc the steps go
+ * in fused pairs (one sweep per step from one set of fields to the other, A -> B -> A, the split
+ * parts updated in place); an odd last step, or desc without the swap set, is one velocity and
+ * one pressure sweep in place. Same results either way; time_block is ignored. No reference
+ * counterpart. */
 typedef struct acfdtd_hip_pml_desc {
     uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
     uint32_t struct_size; /* sizeof(acfdtd_hip_pml_desc) */

@@ -101,17 +101,21 @@ def test_the_pml_reflects_far_less_than_the_sponge():
 @pytest.mark.usefixtures("require_gpu")
 @pytest.mark.parametrize("faces", [None, ((0, 1), (2, 0))], ids=["all-faces", "two-faces"])
 @pytest.mark.parametrize("shape,tile_j,chunk_x", [((20, 17, 15), 0, 0), ((13, 40, 70), 2, 3),
-                                                  ((9, 6, 130), 1, 1)])
+                                                  ((9, 6, 130), 1, 1), ((14, 11, 16), 0, 0),
+                                                  ((12, 9, 24), 3, 2)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["fp32", "fp64"])
-def test_hip_pml_is_bitwise_the_oracle(dtype, shape, tile_j, chunk_x, faces):
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "two-sweep"])
+def test_hip_pml_is_bitwise_the_oracle(dtype, shape, tile_j, chunk_x, faces, fused):
+    """Bitwise the oracle on lanes of 4, 2 and 1 z cells (fused steps) and on the two in-place
+    sweeps, with odd step counts (a fused pair, then an in-place step)."""
     from ac_fdtd_amd.pml import PMLFDTD
 
     grid = Grid(dx=0.03, nx=shape[0], ny=shape[1], nz=shape[2])
     layer = AbsorbingLayer(thickness=3) if faces is None else AbsorbingLayer(thickness=3,
                                                                               faces=faces)
     oracle = _prepare(OraclePMLFDTD(grid, dtype=dtype, layer=layer), grid, 5)
-    solver = _prepare(PMLFDTD(grid, dtype=dtype, layer=layer, tile_j=tile_j, chunk_x=chunk_x),
-                      grid, 5)
+    solver = _prepare(PMLFDTD(grid, dtype=dtype, layer=layer, tile_j=tile_j, chunk_x=chunk_x,
+                              fused=fused), grid, 5)
     for n in (19, 12):
         oracle.run(n)
         solver.run(n)
