@@ -304,10 +304,13 @@ def gradient_cost(size, steps, seed=4):
     impulse at 1 kHz on a 34.3 mm grid. Returns seconds for a forward pass and for forward +
     backward, with walls only (boundary tape, no checkpoints) and with walls and the air
     strengths: from a divergence tape of every step (the default within 64 GiB), and from
-    checkpoints every 100 steps with each segment's divergences recomputed (as beyond it)."""
+    checkpoints every 100 steps with each segment's divergences recomputed (as beyond it); and,
+    in a room lined with the 16-cell sponge on all faces, with walls, sponge and air strengths
+    (checkpoints every 100 steps, each segment's states recomputed into a ring)."""
     import torch
 
-    from ac_fdtd_amd import AIR, AirAbsorption, Grid, WallAdmittances, admittance_from_absorption
+    from ac_fdtd_amd import (AIR, AbsorbingLayer, AirAbsorption, Grid, WallAdmittances,
+                             admittance_from_absorption)
     from ac_fdtd_amd.autograd import RoomResponse
     from ac_fdtd_amd.sources import band_limited_impulse
 
@@ -319,7 +322,7 @@ def gradient_cost(size, steps, seed=4):
     points = [tuple((0.15 + 0.7 * rng.uniform(0.0, 1.0, 3)) * lengths) for _ in range(5)]
     start = admittance_from_absorption(0.2)
 
-    def model(air, history_bytes=64 << 30):
+    def model(air, history_bytes=64 << 30, sponge=False):
         from ac_fdtd_amd.grid import max_stable_time_step
 
         dt = max_stable_time_step(dx, AIR.sound_speed)
@@ -328,6 +331,7 @@ def gradient_cost(size, steps, seed=4):
             receivers=points[1:], dtype=np.float32,
             walls=WallAdmittances.from_absorption(0.2),
             air_absorption=AirAbsorption(20.0, 50.0) if air else None, checkpoint_every=100,
+            absorbing_layer=AbsorbingLayer(thickness=16) if sponge else None,
             history_bytes=history_bytes, autotune=True,
         )
 
@@ -341,20 +345,23 @@ def gradient_cost(size, steps, seed=4):
     out = {"grid": [size] * 3, "steps": steps}
     # walls_and_air is RoomResponse's default for this size (its divergence tape fits the
     # 64 GiB history budget); walls_and_air_checkpointed forces the path beyond the budget.
-    cases = (("walls_tape", False, 64 << 30), ("walls_and_air", True, 64 << 30),
-             ("walls_and_air_checkpointed", True, 0))
-    for key, air, history in cases:
-        room = model(air, history)
+    cases = (("walls_tape", False, 64 << 30, False), ("walls_and_air", True, 64 << 30, False),
+             ("walls_and_air_checkpointed", True, 0, False),
+             ("walls_sponge_and_air", True, 64 << 30, True))
+    for key, air, history, sponge in cases:
+        room = model(air, history, sponge)
         target = room(torch.tensor(hidden, dtype=torch.float64)).detach()
         xi = torch.full((6,), start, dtype=torch.float64, requires_grad=True)
         strengths = room.default_air_strength().requires_grad_(True) if air else None
+        sigma = room.default_sponge_strength().requires_grad_(True) if sponge else None
 
         def forward():
             with torch.no_grad():
-                room(xi.detach(), air_strength=None if strengths is None else strengths.detach())
+                room(xi.detach(), air_strength=None if strengths is None else strengths.detach(),
+                     sponge_strength=None if sigma is None else sigma.detach())
 
         def forward_backward():
-            loss = ((room(xi, air_strength=strengths) - target) ** 2).sum()
+            loss = ((room(xi, air_strength=strengths, sponge_strength=sigma) - target) ** 2).sum()
             loss.backward()
 
         forward()
@@ -369,7 +376,9 @@ def gradient_cost(size, steps, seed=4):
             "checkpoints": room.report.checkpoints,
             "divergence_bytes": room.report.div_tape_bytes,
             "adjoint_two_steps_per_sweep": room.report.adjoint_two_step,
-            "path": ("boundary tape" if not air else "divergence tape of every step"
+            "path": ("boundary tape" if not air else
+                     "checkpoints every 100 steps, states recomputed per segment into a ring"
+                     if sponge else "divergence tape of every step"
                      if room.report.checkpoints == 0 else
                      "checkpoints every 100 steps, divergences recomputed per segment"),
         }
