@@ -101,6 +101,12 @@ class Desc(_c.Structure):
         ("div_tape", _ptr),
         ("div_tape_step0", _c.c_int64),
         ("div_tape_rows", _c.c_int64),
+        ("layer_tape", _ptr),
+        ("layer_tape_step0", _c.c_int64),
+        ("layer_tape_rows", _c.c_int64),
+        ("layer_tape_stride", _c.c_int64),
+        ("layer_lo", _c.c_int32 * 3),
+        ("layer_hi", _c.c_int32 * 3),
     ]
 
 
@@ -244,6 +250,11 @@ def library() -> ctypes.CDLL:
     lib.acfdtd_hip_fused2_step.argtypes = [_c.POINTER(Desc), _c.c_int64, _c.c_int64, _c.c_int, _ptr]
     lib.acfdtd_hip_wall_tape_stride.restype = _c.c_int64
     lib.acfdtd_hip_wall_tape_stride.argtypes = [_c.c_int32, _c.c_int32, _c.c_int32]
+    lib.acfdtd_hip_layer_tape_stride.restype = _c.c_int64
+    lib.acfdtd_hip_layer_tape_stride.argtypes = [_c.POINTER(Desc)]
+    lib.acfdtd_hip_layer_tape_unpack.restype = _c.c_int
+    lib.acfdtd_hip_layer_tape_unpack.argtypes = [_c.POINTER(Desc), _c.c_int64, _ptr, _ptr, _ptr,
+                                                 _ptr, _ptr]
     lib.acfdtd_hip_adjoint_partials_count.restype = _c.c_size_t
     lib.acfdtd_hip_adjoint_partials_count.argtypes = [_c.POINTER(Desc)]
     lib.acfdtd_hip_adjoint_face_sweep.restype = _c.c_int

@@ -22,10 +22,15 @@ forward state (the step is linear in it); the parameter gradients do:
   checkpoint of the full state every ``checkpoint_every`` steps, and the backward pass
   recomputes each segment's divergences (two steps per sweep) into a ring of
   ``checkpoint_every`` rows before running the adjoint through it;
-* the sponge strength needs the forward fields at every step: checkpoints every
-  ``checkpoint_every`` steps, and each segment's fields recomputed into a ring of states before
-  the adjoint (one step per sweep) runs through it; requested together with the sponge, the
-  air gradients take the same path.
+* the sponge strength needs the damped forward fields of the layer's cells at every step. When
+  ``n_steps`` rows of them (a layer tape, about 16 B per layer cell and step in fp32), plus the
+  divergence tape if the air gradients are asked for too, fit in ``history_bytes`` and the
+  adjoint runs two steps per sweep, the forward kernels write them and the two-step adjoint
+  reads them: one forward and one adjoint pass again (the last one to four adjoint steps, which
+  run one per sweep, read states unpacked from the tape). Otherwise the forward pass keeps
+  checkpoints every ``checkpoint_every`` steps, and each segment's fields are recomputed into
+  a ring of states before the adjoint (one step per sweep) runs through it; requested together
+  with the sponge, the air gradients then take the same path.
 
 There is no reference counterpart: the reference's PyTorch backend uses ``out=`` operations and
 host-float coefficients and cannot be differentiated (``src/ac_fdtd/torch_backend.py:149-195``,
@@ -63,6 +68,7 @@ class GradientReport:
     checkpoint_bytes: int = 0
     tape_bytes: int = 0
     div_tape_bytes: int = 0  # divergence rows held for the air gradients (tape or ring)
+    layer_tape_bytes: int = 0  # layer-cell fields held for the sponge gradient
 
 
 class RoomResponse:
@@ -234,7 +240,29 @@ class _RoomResponseFunction(torch.autograd.Function):
         checkpoints = []
         div_tape = None
         tape_bytes = room.n_steps * room.grid.n_cells * solver.np_dtype.itemsize
-        if want_air and not want_sponge and tape_bytes <= room.history_bytes:
+        layer_bytes = None
+        if (want_sponge and room.fused and room.time_block == 2
+                and solver.adjoint_two_step_applies()):
+            try:
+                layer_bytes = room.n_steps * solver.layer_tape_stride() * solver.np_dtype.itemsize
+            except ValueError:
+                layer_bytes = None  # a layer not at the ends of its axes: the ring path
+        layered = layer_bytes is not None and (
+            layer_bytes + (tape_bytes if want_air else 0) <= room.history_bytes)
+        if layered:
+            # The layer cells' fields (and the divergences) of every step, for the two-step
+            # adjoint to read: no recomputation.
+            layer_tape = torch.empty((room.n_steps, solver.layer_tape_stride()),
+                                     dtype=solver.dtype, device=solver.device)
+            solver.attach_layer_tape(layer_tape, 0)
+            report.layer_tape_bytes = layer_bytes
+            if want_air:
+                div_tape = torch.empty((room.n_steps, room.grid.n_cells), dtype=solver.dtype,
+                                       device=solver.device)
+                solver.attach_div_tape(div_tape, 0)
+                report.div_tape_bytes = tape_bytes
+            solver.run(room.n_steps)
+        elif want_air and not want_sponge and tape_bytes <= room.history_bytes:
             # Every step's divergences, for the adjoint to read: no recomputation.
             div_tape = torch.empty((room.n_steps, room.grid.n_cells), dtype=solver.dtype,
                                    device=solver.device)
@@ -253,6 +281,7 @@ class _RoomResponseFunction(torch.autograd.Function):
         recording = solver.recording_tensor()
         ctx.room, ctx.solver, ctx.checkpoints, ctx.report = room, solver, checkpoints, report
         ctx.want = (want_walls, want_signal, want_sponge, want_air)
+        ctx.layered = layered
         ctx.inputs = (admittance, signal, sponge_strength, air_strength)
         return recording
 
@@ -260,7 +289,8 @@ class _RoomResponseFunction(torch.autograd.Function):
     def backward(ctx, grad_recording):
         room, solver = ctx.room, ctx.solver
         want_walls, want_signal, want_sponge, want_air = ctx.want
-        grads = _run_adjoint(room, solver, grad_recording, ctx.checkpoints, ctx.want, ctx.report)
+        grads = _run_adjoint(room, solver, grad_recording, ctx.checkpoints, ctx.want, ctx.report,
+                             layered=ctx.layered)
         room.report = ctx.report
         admittance, signal, sponge_strength, air_strength = ctx.inputs
 
@@ -283,8 +313,10 @@ class _RoomResponseFunction(torch.autograd.Function):
         )
 
 
-def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, report):
-    """Transpose every step of the forward run, newest first; return the parameter gradients."""
+def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, report,
+                 layered: bool = False):
+    """Transpose every step of the forward run, newest first; return the parameter gradients.
+    ``layered``: the forward pass wrote a layer tape of every step (the sponge gradient)."""
     want_walls, want_signal, want_sponge, want_air = want
     lib = solver._lib
     g, k = solver.grid, solver.coefficients
@@ -297,15 +329,16 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
     recording = solver.recording_tensor()
     # Air gradients from divergences (a tape of every step, or a ring per recomputed segment)
     # unless the sponge gradient's forward states are recomputed anyway.
-    divergences = want_air and not want_sponge
+    divergences = want_air and (layered or not want_sponge)
     fwd = solver._descriptor(recording)
-    two_step = (room.fused and room.time_block == 2 and not want_sponge
+    two_step = (room.fused and room.time_block == 2 and (layered or not want_sponge)
                 and solver.adjoint_two_step_applies())
     if room.autotune:
         from .autotune import adjoint_tiling
 
         if two_step:
-            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, two_step=True, air=divergences)
+            fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, two_step=True, air=divergences,
+                                                     sponge=want_sponge and layered)
         else:
             fwd.tile_j, fwd.chunk_x = adjoint_tiling(solver, fwdgrad=want_sponge or want_air)
     n_partials = lib.acfdtd_hip_adjoint_partials_count(ctypes.byref(fwd))
@@ -387,8 +420,34 @@ def _run_adjoint(room, solver: AcousticFDTD, grad_recording, checkpoints, want, 
         _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), step_hi,
                                                  step_lo, handle), "acfdtd_hip_adjoint_run")
 
+    def unpacked(steps):
+        """Forward states after the given steps, unpacked from the layer tape into slots."""
+        states = [torch.zeros((len(steps), *g.shape), **opts)] + [
+            torch.zeros((len(steps), *g.velocity_shape(a)), **opts) for a in range(3)]
+        for q, n in enumerate(steps):
+            if n is None:
+                continue
+            _native.check(lib.acfdtd_hip_layer_tape_unpack(
+                ctypes.byref(fwd), n, *(ctypes.c_void_p(t[q].data_ptr()) for t in states), handle),
+                "acfdtd_hip_layer_tape_unpack")
+        return states
+
     with torch.cuda.device(solver.device):
-        if div_ring is not None:
+        if layered:
+            # Adjoint steps n_steps - 1 .. r in groups of four through the two-step sweep, which
+            # reads the layer tape; the last r (1 to 4) one per sweep through states unpacked from
+            # it (slot q: the state after step q - 1; slot 0, the initial state, is never read).
+            quads = (n_steps - 1) // 4
+            r = n_steps - 4 * quads
+            last_local(unpacked([n_steps - 1]), 0)
+            use_ring(None, 0, 0)
+            if quads:
+                adjoint_steps(n_steps - 1, r)
+            ring = unpacked([None] + list(range(r)))
+            use_ring(ring, 0, r + 1)
+            adjoint_steps(r - 1, 0)
+            del ring
+        elif div_ring is not None:
             K = room.checkpoint_every
             ring = div_ring
             use_ring(None, 0, 0)

@@ -250,11 +250,12 @@ def forward_plan(solver, tape: bool = False) -> tuple[int, int, int, bool]:
 
 
 def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
-                   two_step: bool = False, air: bool = False) -> tuple[int, int]:
+                   two_step: bool = False, air: bool = False,
+                   sponge: bool = False) -> tuple[int, int]:
     """Fastest (tile_j, chunk_x) of the fused adjoint step: with the wall gradient when the
     solver has a wall table, and with the sponge and air gradients too when ``fwdgrad``; of the
-    two-step adjoint sweep when ``two_step`` (wall gradient, and with ``air`` the air gradients
-    from a divergence tape)."""
+    two-step adjoint sweep when ``two_step`` (wall gradient, with ``air`` the air gradients
+    from a divergence tape, with ``sponge`` the sponge gradient from a layer tape)."""
     if not solver.fused:
         return solver.tile_j, solver.chunk_x
     if candidates is None:
@@ -300,6 +301,23 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
         fwd.div_tape, fwd.div_tape_step0, fwd.div_tape_rows = (divergences.data_ptr(), 0,
                                                                divergences.shape[0])
         adj.grads |= _native.GRAD_AIR
+    layer_tape = None
+    if two_step and sponge and k.has_layer:
+        # Adjoint steps n .. 1 read layer tape rows n .. 0.
+        from .boundaries import layer_weights
+
+        layer_tape = torch.zeros((steps + 2, solver.layer_tape_stride()), **opts)
+        fwd.layer_tape, fwd.layer_tape_step0 = layer_tape.data_ptr(), 0
+        fwd.layer_tape_rows, fwd.layer_tape_stride = layer_tape.shape
+        lo, hi = solver.layer_extent()
+        for a in range(3):
+            fwd.layer_lo[a], fwd.layer_hi[a] = lo[a], hi[a]
+        adj.grads |= _native.GRAD_SPONGE
+        cells, faces = layer_weights(g, solver.absorbing_layer)
+        weights = [torch.as_tensor(w, **opts) for w in (*cells, *faces)]
+        for a in range(3):
+            adj.weight_cell[a] = weights[a].data_ptr()
+            adj.weight_face[a] = weights[3 + a].data_ptr()
     if fwdgrad and not two_step and (k.has_layer or k.n_relax):
         # Adjoint steps n .. 1 read forward states 1 .. n + 1.
         ring = [torch.zeros((steps + 2, *g.shape), **opts)] + [
@@ -328,6 +346,7 @@ def adjoint_tiling(solver, fwdgrad: bool = False, candidates=None,
         _native.check(lib.acfdtd_hip_adjoint_run(ctypes.byref(fwd), ctypes.byref(adj), n, 1,
                                                  handle), "acfdtd_hip_adjoint_run")
 
-    kind = ("adjoint2-air" if divergences is not None else "adjoint2") if two_step else (
+    kind = ("adjoint2" + ("-sponge" if layer_tape is not None else "")
+            + ("-air" if divergences is not None else "")) if two_step else (
         "adjoint-fwdgrad" if ring is not None else "adjoint")
     return _pick(solver, kind, launch, candidates)

@@ -41,6 +41,8 @@ SOURCES = (
     "fused2_64.hip",
     "adjoint2.hip",
     "adjoint2_64.hip",
+    "adjoint2s.hip",
+    "adjoint2s_64.hip",
     "fused2s.hip",
     "fused2s_64.hip",
     "fused2d.hip",

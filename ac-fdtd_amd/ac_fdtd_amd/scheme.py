@@ -160,6 +160,8 @@ class AcousticFDTD:
             self._wall_mask = int(sum(1 << c for c in range(64) if k.wall_is_wall[c]))
         self._wall_tape: tuple[torch.Tensor, int] | None = None
         self._div_tape: tuple[torch.Tensor, int] | None = None
+        self._layer_tape: tuple[torch.Tensor, int] | None = None
+        self._layer_extent = None
         # A grid whose nz the widest lane does not divide runs in padded rows (PaddedRows,
         # allocated at the first run long enough to use them); False keeps the reference layout.
         self.padded_rows = True
@@ -311,6 +313,7 @@ class AcousticFDTD:
         reference layout), for at least PADDED_MIN_STEPS steps."""
         if (not self.padded_rows or self._swap is None or n_steps < self.PADDED_MIN_STEPS
                 or padded_row(self) == self.grid.nz or self._div_tape is not None
+                or self._layer_tape is not None
                 or (record and self._wall_tape is not None)):
             return None
         if self._padded is None:
@@ -498,6 +501,58 @@ class AcousticFDTD:
                 raise ValueError(f"divergence tape must be {self.dtype} on {self.device}")
         self._div_tape = None if tape is None else (tape, int(step0))
 
+    def layer_extent(self) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
+        """Cells of the absorbing layer at the low and high end of each axis (where the grading
+        weights are nonzero), the frame a layer tape holds."""
+        if self.absorbing_layer is None:
+            raise ValueError("the solver has no absorbing layer")
+        if self._layer_extent is None:
+            from .boundaries import layer_weights
+
+            cells, faces = layer_weights(self.grid, self.absorbing_layer)
+            lo, hi = [], []
+            for wc, wf in zip(cells, faces):
+                # A cell is in the layer when its own weight or that of a face it owns (its lower
+                # face; the last cell also the far face) is nonzero.
+                need = np.asarray(wc) > 0
+                n = need.size
+                face_on = np.asarray(wf) > 0
+                need[: n] |= face_on[:n]
+                need[n - 1] |= bool(face_on[n])
+                lead = n if need.all() else int(np.argmin(need))
+                trail = n if need.all() else int(np.argmin(need[::-1]))
+                if not need.all() and need[lead : n - trail].any():
+                    raise ValueError("the absorbing layer is not at the ends of an axis")
+                lo.append(lead)
+                hi.append(trail)
+            self._layer_extent = (tuple(lo), tuple(hi))
+        return self._layer_extent
+
+    def layer_tape_stride(self) -> int:
+        """Elements per row of a layer tape (:meth:`attach_layer_tape`)."""
+        d = _native.Desc()
+        d.nx, d.ny, d.nz = self.grid.shape
+        lo, hi = self.layer_extent()
+        for a in range(3):
+            d.layer_lo[a], d.layer_hi[a] = lo[a], hi[a]
+        return int(self._lib.acfdtd_hip_layer_tape_stride(ctypes.byref(d)))
+
+    def attach_layer_tape(self, tape: torch.Tensor | None, step0: int = 0) -> None:
+        """Write the fields of the absorbing layer's cells after each step ``n`` (pressure,
+        lower face velocities, and the far faces) to row ``n - step0`` of ``tape``, a contiguous
+        16-byte aligned ``(rows, layer_tape_stride())`` tensor of the working precision; None
+        detaches. The sponge-strength gradient reads it (:mod:`ac_fdtd_amd.autograd`)."""
+        if tape is not None:
+            if self.absorbing_layer is None:
+                raise ValueError("a layer tape needs an absorbing layer")
+            if tape.dim() != 2 or tape.shape[1] != self.layer_tape_stride() or not tape.is_contiguous():
+                raise ValueError("layer tape must be a contiguous (rows, layer_tape_stride) tensor")
+            if tape.dtype != self.dtype or tape.device != self.device:
+                raise ValueError(f"layer tape must be {self.dtype} on {self.device}")
+            if tape.data_ptr() % 16:
+                raise ValueError("layer tape must be 16-byte aligned")
+        self._layer_tape = None if tape is None else (tape, int(step0))
+
     def wall_tape_stride(self) -> int:
         return int(self._lib.acfdtd_hip_wall_tape_stride(*self.grid.shape))
 
@@ -571,6 +626,15 @@ class AcousticFDTD:
             d.div_tape = buffer.data_ptr()
             d.div_tape_step0 = step0
             d.div_tape_rows = buffer.shape[0]
+        if (tape if div is None else div) and self._layer_tape is not None:
+            buffer, step0 = self._layer_tape
+            d.layer_tape = buffer.data_ptr()
+            d.layer_tape_step0 = step0
+            d.layer_tape_rows = buffer.shape[0]
+            d.layer_tape_stride = buffer.shape[1]
+            lo, hi = self.layer_extent()
+            for a in range(3):
+                d.layer_lo[a], d.layer_hi[a] = lo[a], hi[a]
         if self._swap is not None:
             d.p_swap = self._checked(self._swap[0], g.shape, "p swap")
             for a, name in enumerate(("vx_swap", "vy_swap", "vz_swap")):

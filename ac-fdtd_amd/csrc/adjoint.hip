@@ -449,10 +449,26 @@ extern "C" int acfdtd_hip_adjoint_run(const acfdtd_hip_desc *desc,
     }
     const bool sponge = (adjoint->grads & ACFDTD_HIP_GRAD_SPONGE) && desc->has_layer;
     const bool air = (adjoint->grads & ACFDTD_HIP_GRAD_AIR) && desc->n_relax > 0;
-    // The air gradients read the divergence tape when the forward descriptor has one, else
-    // (like the sponge gradient) the forward states.
-    const bool states = sponge || (air && desc->div_tape == nullptr);
     if (adjoint->fwd_count < 0) return fail(ACFDTD_HIP_EINVAL, "fwd_count must be >= 0");
+    // The sponge gradient reads the layer tape when the forward descriptor has one and no
+    // forward states are given (two adjoint steps per sweep only), else the forward states; the
+    // air gradients read the divergence tape when there is one, else the forward states.
+    const bool sponge_tape = sponge && desc->layer_tape != nullptr && adjoint->fwd_count == 0;
+    const bool states = (sponge && !sponge_tape) || (air && desc->div_tape == nullptr);
+    if (sponge_tape) {
+        Tiling2 t2;
+        if (!adjoint2_applies(*desc, *adjoint) ||
+            !make_tiling2(*desc, desc->real_bytes == 8 ? 2 : 4, t2))
+            return fail(ACFDTD_HIP_EINVAL,
+                        "the layer tape is read by the two-step adjoint only, which does not apply "
+                        "to these descriptors (give the forward states instead)");
+        if (step_lo < 1 || (step_hi - step_lo + 1) % 4 != 0)
+            return fail(ACFDTD_HIP_EINVAL,
+                        "with the layer tape, adjoint steps run in groups of four from step 1 on "
+                        "(give the forward states for the others)");
+        if (!layer_tape_covers(*desc, step_lo - 1, step_hi))
+            return fail(ACFDTD_HIP_EINVAL, "layer tape does not cover these steps");
+    }
     if (states) {
         if (adjoint->fwd_count == 0 || !adjoint->fwd_vx || !adjoint->fwd_vy || !adjoint->fwd_vz ||
             (sponge && !adjoint->fwd_p))

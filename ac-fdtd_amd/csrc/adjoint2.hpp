@@ -19,8 +19,9 @@
 // source-signal gradient are bit-identical to single adjoint steps; the fp64 gradient partials
 // are summed in another order. The wall and signal gradients are carried, and with AIR the
 // air-strength gradients, whose forward input (every cell's divergence of every step) comes
-// from the forward pass's divergence tape; the sponge gradient reads the forward states of
-// every step and stays on the single step.
+// from the forward pass's divergence tape; with SPONGE the sponge-strength gradient, whose
+// forward input (the damped fields of the layer's cells at every step) comes from the forward
+// pass's layer tape.
 
 #pragma once
 
@@ -59,7 +60,24 @@ struct AdjLane {
     bool wall_in[V];
     real fu_in[V], fp_in[V];
     real cdiv;
+    // SPONGE: the grading weights of the lane's rows and cells, and its place in the frame
+    // planes of the layer tape (-1 outside the frame).
+    real wyc, wyf_lo, wyf_hi, wzc[V], wzf[V + 1];
+    int64_t lt_pos;
 };
+
+template <typename real, int V>
+__device__ __forceinline__ void adj_lane_sponge(const SweepArgs<real> &a, const AdjointArgs<real> &g,
+                                                const int j, const int k0, AdjLane<real, V> &c) {
+    c.wyc = g.w_cell[1][j];
+    c.wyf_lo = g.w_face[1][j];
+    c.wyf_hi = g.w_face[1][j + 1];
+#pragma unroll
+    for (int e = 0; e < V; ++e) c.wzc[e] = g.w_cell[2][k0 + e];
+#pragma unroll
+    for (int e = 0; e <= V; ++e) c.wzf[e] = g.w_face[2][k0 + e];
+    c.lt_pos = a.layer.in_frame(j, k0);
+}
 
 template <typename real, int V, int NR, bool LAYER>
 __device__ __forceinline__ void adj_lane(const SweepArgs<real> &a, const int j, const int k0,
@@ -155,7 +173,10 @@ struct AdjPlane {
 // relaxation). `own` marks the cells whose gradients this lane adds and writes; tv holds the
 // boundary tape of `step` for the lane's wall cells, dv (AIR) the forward divergences of `step`,
 // whose air-gradient terms go to air_acc (relax_scale, then one per relaxation state).
-template <typename real, int V, int NR, bool LAYER, bool XWALL, bool AIR = false>
+// SPONGE: the sponge-strength gradient of `own` cells into sp_acc, from the layer tape rows of
+// step + 1 (lt_v: the velocities the faces' terms multiply) and of `step` (lt_p: the pressures
+// of the local part); own_xhi: this lane also adds the term of its cells' upper x face.
+template <typename real, int V, int NR, bool LAYER, bool XWALL, bool AIR = false, bool SPONGE = false>
 __device__ __forceinline__ void adjoint_cells(
     const SweepArgs<real> &a, const AdjLane<real, V> &c, const int i, const int64_t step,
     const int h, const bool own, const bool want_walls, real *grad_signal, const real (&tv)[V],
@@ -164,7 +185,9 @@ __device__ __forceinline__ void adjoint_cells(
     const real (&lvx_v)[V], const real (&lvy_lo_in)[V], const real (&lvy_hi_in)[V],
     const real (&lvz_v)[V], const real lvz_top, const real (&lp_in)[V],
     const real (&states_in)[NR > 0 ? NR : 1][V], AdjPlane<real, V, NR> &o, double (&acc)[6],
-    const real (&dv)[V], double (&air_acc)[1 + (NR > 0 ? NR : 1)]) {
+    const real (&dv)[V], double (&air_acc)[1 + (NR > 0 ? NR : 1)],
+    const AdjointArgs<real> &g, const real *lt_v, const real *lt_p, const bool own_xhi,
+    double &sp_acc) {
     const int nx = a.nx;
     const real vel = a.vel;
     const bool up_interior = !XWALL || i + 1 < nx;
@@ -172,7 +195,8 @@ __device__ __forceinline__ void adjoint_cells(
     const real fxc = LAYER ? uniform_load(a.cell[0] + i) : real(1);
     const real fxf_hi = LAYER ? uniform_load(a.face[0] + i + 1) : real(1);
 
-    // Faces (adjoint_face_sweep).
+    // Faces (adjoint_face_sweep); SPONGE keeps the undamped values for the sponge terms.
+    real s_tx[V], s_ty_lo[V], s_ty_hi[V], s_tz[V + 1];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const real tx = lvx_v[e] + (ld_c[e] - (up_interior ? ld_up[e] : real(0)));
@@ -184,10 +208,64 @@ __device__ __forceinline__ void adjoint_cells(
         const real below = e > 0 ? ld_c[e - 1] : (c.has_zlo ? ldz_lo : real(0));
         const real tz = lvz_v[e] + (below - ld_c[e]);
         o.lvz[e] = damp3<real, LAYER>(tz, fxc, c.fyc, c.fzf[e]);
+        s_tx[e] = tx;
+        s_ty_lo[e] = ty_lo;
+        s_ty_hi[e] = ty_hi;
+        s_tz[e] = tz;
     }
     {
         const real tz = lvz_top + (ld_c[V - 1] - (c.has_zhi ? ldz_hi : real(0)));
         o.lvz[V] = damp3<real, LAYER>(tz, fxc, c.fyc, c.fzf[V]);
+        s_tz[V] = tz;
+    }
+
+    // Sponge terms of the faces, as adjoint_march_vec adds them: the undamped adjoint velocity
+    // times the forward velocity after the step (tape row lt_v) times -dt times the face's
+    // summed weights.
+    real wxc = 0;
+    int64_t lt_q = -1;
+    const double mdt = -(double)g.dt;
+    if constexpr (SPONGE) {
+        const LayerFrame &f = a.layer;
+        wxc = uniform_load(g.w_cell[0] + i);
+        lt_q = f.offset(i, c.j, c.k0, c.lt_pos);
+        if (own) {
+            double s = 0;
+            if (lt_q >= 0) {
+                real fy[V], fz[V];
+                vload<real, V>(lt_v + 2 * f.cells + lt_q, fy);
+                vload<real, V>(lt_v + 3 * f.cells + lt_q, fz);
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    s += (double)(wxc + c.wyf_lo + c.wzc[e]) * s_ty_lo[e] * fy[e];
+                    s += (double)(wxc + c.wyc + c.wzf[e]) * s_tz[e] * fz[e];
+                }
+            }
+            if (!c.has_yhi) {
+                real fy[V];
+                vload<real, V>(lt_v + f.far_y() + (int64_t)i * a.nz + c.k0, fy);
+#pragma unroll
+                for (int e = 0; e < V; ++e) s += (double)(wxc + c.wyf_hi + c.wzc[e]) * s_ty_hi[e] * fy[e];
+            }
+            if (!c.has_zhi)
+                s += (double)(wxc + c.wyc + c.wzf[V]) * s_tz[V] *
+                     lt_v[f.far_z() + (int64_t)i * a.ny + c.j];
+            sp_acc += mdt * s;
+        }
+        if (own_xhi) {
+            // The upper x face is the lower face of plane i + 1 (the far face of the last plane).
+            const int64_t q1 = i + 1 == nx ? f.far_x() + (int64_t)c.j * a.nz + c.k0
+                                           : f.cells + f.offset(i + 1, c.j, c.k0, c.lt_pos);
+            if (i + 1 == nx || q1 >= f.cells) {
+                const real wxf = uniform_load(g.w_face[0] + i + 1);
+                real fx[V];
+                vload<real, V>(lt_v + q1, fx);
+                double s = 0;
+#pragma unroll
+                for (int e = 0; e < V; ++e) s += (double)(wxf + c.wyc + c.wzc[e]) * s_tx[e] * fx[e];
+                sp_acc += mdt * s;
+            }
+        }
     }
 
     // Stencil (adjoint_cell_sweep, stencil = 1).
@@ -203,8 +281,20 @@ __device__ __forceinline__ void adjoint_cells(
         lpc[e] = lp_in[e] + vel * (((up_x - lo_x) + (up_y - lo_y)) + (up_z - lo_z));
     }
 
-    // Local part of `step` (adjoint_cell_sweep, local = 1): receivers in, signal gradient out.
+    // Local part of `step` (adjoint_cell_sweep, local = 1): receivers in, signal gradient out;
+    // SPONGE: the sources' injection of `step`, which the tape's pressures include and the
+    // sponge term excludes.
+    real injected[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) injected[e] = 0;
     if (c.special && i >= c.ev_lo && i <= c.ev_hi) {
+        if (SPONGE && own) {
+            for (int s = 0; s < a.n_sources; ++s) {
+                const int e = c.src_cell[2 * s + 1] - c.jk0;
+                if (c.src_cell[2 * s] == i && e >= 0 && e < V && step < c.src_len[s])
+                    lane_add<real, V>(injected, e, a.src * a.source_signal[s * a.source_stride + step]);
+            }
+        }
         for (int r = 0; r < a.n_receivers; ++r) {
             const int e = c.rx_cell[2 * r + 1] - c.jk0;
             if (c.rx_cell[2 * r] == i && e >= 0 && e < V) lane_add<real, V>(lpc, e, c.rx_now[2 * r + h]);
@@ -218,9 +308,18 @@ __device__ __forceinline__ void adjoint_cells(
         }
     }
     const int cx = XWALL ? wall_class(i, nx) : 0;
+    // Sponge term of each cell: lp times the forward pressure of `step` (tape row lt_p) less
+    // its injection, times -dt times the cell's summed weights.
+    real bp[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) bp[e] = 0;
+    if (SPONGE && own && lt_q >= 0) vload<real, V>(lt_p + lt_q, bp);
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const real lpe = lpc[e];
+        if (SPONGE && own && lt_q >= 0)
+            sp_acc += mdt * (double)(wxc + (c.wyc + c.wzc[e])) * (double)lpe *
+                      (double)(bp[e] - injected[e]);
         const real lhat = LAYER ? ((lpe * fxc) * c.fyc) * c.fzc[e] : lpe;
         real ltil = lhat, lp_new = 0;
         bool is_wall = c.wall_in[e];
@@ -294,13 +393,15 @@ __device__ __forceinline__ void load_tape(const SweepArgs<real> &a, const AdjLan
     }
 }
 
-template <typename real, int V, int NR, bool LAYER, bool AIR = false>
+template <typename real, int V, int NR, bool LAYER, bool AIR = false, bool SPONGE = false>
 __device__ __forceinline__ void adjoint2_march(
     const SweepArgs<real> &a, const Tiling2 &t2, const int i0, const int i1, const int j,
     const int k0, const bool valid, const bool owner, const int t, const int64_t m,
     const bool want_walls, real *grad_signal, const Adjoint2IO<real> &io, real *__restrict__ lds,
     const int32_t *src_cell, const int64_t *src_len, const int32_t *rx_cell, const real *rx_now,
-    const real *wtab, double (&acc)[6], double (&air_acc)[1 + (NR > 0 ? NR : 1)]) {
+    const real *wtab, double (&acc)[6], double (&air_acc)[1 + (NR > 0 ? NR : 1)],
+    const AdjointArgs<real> &g, double &sp_acc) {
+    static_assert(!SPONGE || LAYER, "the sponge gradient needs the layer");
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
     const int nx = a.nx, nz = a.nz;
@@ -315,8 +416,33 @@ __device__ __forceinline__ void adjoint2_march(
     const real *__restrict__ lpsi_in = io.lpsi_in;
     AdjLane<real, V> c;
     if (valid) adj_lane<real, V, NR, LAYER>(a, j, k0, src_cell, src_len, rx_cell, rx_now, wtab, c);
+    if (SPONGE && valid) adj_lane_sponge<real, V>(a, g, j, k0, c);
     // The lane constants' loads complete on every path into the march (see fused2_march).
     wait_memory();
+    // SPONGE: layer tape rows of forward steps m, m - 1, m - 2 (step A's faces read row m and
+    // its local part row m - 1; step B's faces row m - 1 and its local part row m - 2).
+    const real *lt_row[3] = {nullptr, nullptr, nullptr};
+    if (SPONGE) {
+#pragma unroll
+        for (int h = 0; h < 3; ++h)
+            lt_row[h] = a.layer_tape + (m - h - a.layer_tape_step0) * a.layer_tape_stride;
+    }
+    // The sponge term of a lower x face from its undamped adjoint velocity w (first faces).
+    auto sponge_xface = [&](const int s, const real *row, const real (&w)[V]) __attribute__((always_inline)) {
+        if constexpr (SPONGE) {
+            const LayerFrame &f = a.layer;
+            const int64_t q = f.offset(s, j, k0, c.lt_pos);
+            if (q >= 0) {
+                const real wxf = uniform_load(g.w_face[0] + s);
+                real fx[V];
+                vload<real, V>(row + f.cells + q, fx);
+                double sum = 0;
+#pragma unroll
+                for (int e = 0; e < V; ++e) sum += (double)(wxf + c.wyc + c.wzc[e]) * w[e] * fx[e];
+                sp_acc += -(double)g.dt * sum;
+            }
+        }
+    };
 
     auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
         return lds + ((plane % 3) * T + lane) * V;
@@ -330,11 +456,15 @@ __device__ __forceinline__ void adjoint2_march(
         vload<real, V>(lvx_in + L.cell(s), v);
         vload<real, V>(ld_in + L.cell(s > 0 ? s - 1 : s), prev);
         const real fxf = LAYER ? uniform_load(a.face[0] + s) : real(1);
+        real ws[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = v[e] + ((s > 0 ? prev[e] : real(0)) - ld_a[e]);
             s1_lvx_lo[e] = damp3<real, LAYER>(w, fxf, c.fyc, c.fzc[e]);
+            ws[e] = w;
         }
+        // Face 0 belongs to no halo plane: its first step's sponge term is added here.
+        if (SPONGE && owner && s == 0 && i0 == 0) sponge_xface(0, lt_row[0], ws);
     };
     // Step A (adjoint step m, local part of m - 1) of plane s from global memory into o.
     auto stepA = [&](const int s, AdjPlane<real, V, NR> &o, auto xwall_tag) __attribute__((always_inline)) {
@@ -373,10 +503,12 @@ __device__ __forceinline__ void adjoint2_march(
 #pragma unroll
             for (int e = 0; e < V; ++e) dv[e] = 0;
         }
-        adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
+        // Upper x face s + 1: this chunk's when its owner cell s + 1 is, or it is the far face.
+        const bool own_xhi = SPONGE && owner && s + 1 >= i0 && s < i1 && (s + 1 < i1 || s + 1 == nx);
+        adjoint_cells<real, V, NR, LAYER, XWALL, AIR, SPONGE>(
             a, c, s, m - 1, 0, own, want_walls, grad_signal, tv, ld_a, ld_up, ld_ylo, ld_yhi,
             ldz_lo, ldz_hi, s1_lvx_lo, lvx_v, lvy_lo_in, lvy_hi_in, lvz_v, lvz_top, lpc, states, o,
-            acc, dv, air_acc);
+            acc, dv, air_acc, g, lt_row[0], lt_row[1], own_xhi, sp_acc);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             ld_a[e] = ld_up[e];
@@ -407,10 +539,11 @@ __device__ __forceinline__ void adjoint2_march(
 #pragma unroll
         for (int e = 0; e < V; ++e) lvz_v[e] = A.lvz[e];
         AdjPlane<real, V, NR> o;
-        adjoint_cells<real, V, NR, LAYER, XWALL, AIR>(
+        adjoint_cells<real, V, NR, LAYER, XWALL, AIR, SPONGE>(
             a, c, i, m - 2, 1, true, want_walls, grad_signal, tv, A.ld, ld_next, ld_ylo, ld_yhi,
             ldz_lo, ldz_hi, s2_lvx_lo, A.lvx_hi, A.lvy_lo, A.lvy_hi, lvz_v, A.lvz[V], A.lp, A.psi,
-            o, acc, A.dv_b, air_acc);
+            o, acc, A.dv_b, air_acc, g, lt_row[1], lt_row[2],
+            SPONGE && (i + 1 < i1 || i + 1 == nx), sp_acc);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(io.lvx_out + cc + L.plane, o.lvx_hi);
         vstore<real, V>(io.lvy_out + cy, o.lvy_lo);
@@ -452,12 +585,15 @@ __device__ __forceinline__ void adjoint2_march(
         // Step B's adjoint vx on face i0, the lower face of the first cell (the prologue of
         // adjoint_march_vec with step A's results as inputs).
         const real fxf = LAYER ? uniform_load(a.face[0] + i0) : real(1);
+        real ws[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const real w = v_half_lo[e] + ((i0 > 0 ? ld_prev[e] : real(0)) - A.ld[e]);
             s2_lvx_lo[e] = damp3<real, LAYER>(w, fxf, c.fyc, c.fzc[e]);
+            ws[e] = w;
         }
         vstore<real, V>(io.lvx_out + L.cell(i0), s2_lvx_lo);
+        sponge_xface(i0, lt_row[1], ws);
     }
 
     // One plane: step A of plane i + 1 into `next`, step B of plane i from `here`. The march
@@ -523,7 +659,17 @@ __device__ inline void flush_air_rows(const double (&acc)[NA], const int count, 
     }
 }
 
-template <typename real, int V, int NR, bool LAYER, bool AIR = false>
+// Each wave adds its sum of the sponge partial into its own row of `partials`.
+__device__ inline void flush_sponge_rows(const double acc, const int64_t row, double *partials) {
+    if (!__any(acc != 0)) return;
+    const double sum = wave_sum(acc);
+    if ((threadIdx.x & 63u) == 0) {
+        double *p = partials + row * kGradSlots + kGradSponge;
+        *p = *p + sum;
+    }
+}
+
+template <typename real, int V, int NR, bool LAYER, bool AIR = false, bool SPONGE = false>
 __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     const SweepArgs<real> a, const AdjointArgs<real> g, const Adjoint2IO<real> io,
     const Tiling2 t2, const int64_t m) {
@@ -574,12 +720,14 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     const bool want_walls = (g.grads & ACFDTD_HIP_GRAD_WALLS) && a.wall_tape;
     double acc[6] = {};
     double air_acc[1 + (NR > 0 ? NR : 1)] = {};
-    adjoint2_march<real, V, NR, LAYER, AIR>(a, t2, i0, i1, j, k0, valid, owner, t, m, want_walls,
-                                            g.grad_signal, io, lds, src_lds, len_lds, rx_lds,
-                                            rxg_lds, walls ? wt_lds : nullptr, acc, air_acc);
+    double sp_acc = 0;
+    adjoint2_march<real, V, NR, LAYER, AIR, SPONGE>(
+        a, t2, i0, i1, j, k0, valid, owner, t, m, want_walls, g.grad_signal, io, lds, src_lds,
+        len_lds, rx_lds, rxg_lds, walls ? wt_lds : nullptr, acc, air_acc, g, sp_acc);
     const int64_t row = (int64_t)b * (t2.threads / 64) + t / 64;
     if (want_walls) flush_wall_rows(acc, row, g.partials);
     if (AIR) flush_air_rows(air_acc, 1 + NR, row, g.partials);
+    if (SPONGE) flush_sponge_rows(sp_acc, row, g.partials);
 }
 
 template <typename real>
@@ -597,6 +745,22 @@ Adjoint2Kernel<real> pick_adjoint2_nr(bool layer, bool air) {
     return layer ? adjoint2_step<real, V, NR, true> : adjoint2_step<real, V, NR, false>;
 }
 
+// SPONGE kernels (the sponge gradient from the layer tape; the layer is present), with or
+// without the air gradients. Instantiated in adjoint2s.hip and adjoint2s_64.hip.
+template <typename real>
+Adjoint2Kernel<real> pick_adjoint2_sponge(int nr, bool air) {
+    constexpr int V = sizeof(real) == 4 ? 4 : 2;
+    switch (nr) {
+        case 0: return adjoint2_step<real, V, 0, true, false, true>;
+        case 1: return air ? adjoint2_step<real, V, 1, true, true, true> : adjoint2_step<real, V, 1, true, false, true>;
+        case 2: return air ? adjoint2_step<real, V, 2, true, true, true> : adjoint2_step<real, V, 2, true, false, true>;
+        case 3: return air ? adjoint2_step<real, V, 3, true, true, true> : adjoint2_step<real, V, 3, true, false, true>;
+        default: return air ? adjoint2_step<real, V, 4, true, true, true> : adjoint2_step<real, V, 4, true, false, true>;
+    }
+}
+extern template Adjoint2Kernel<float> pick_adjoint2_sponge<float>(int, bool);
+extern template Adjoint2Kernel<double> pick_adjoint2_sponge<double>(int, bool);
+
 // AIR kernels (air gradients from the divergence tape) when the air gradients are asked for.
 template <typename real>
 Adjoint2Kernel<real> pick_adjoint2(int nr, bool layer, bool air) {
@@ -610,16 +774,17 @@ Adjoint2Kernel<real> pick_adjoint2(int nr, bool layer, bool air) {
 }
 
 // Whether acfdtd_hip_adjoint_run may advance two adjoint steps per sweep: asked for, both
-// sets complete, no sponge gradient (it reads the forward states of every step), air gradients
-// only from a divergence tape, the source and receiver tables within the LDS copies.
+// sets complete, no forward states given (the steps that read them run one per sweep), the
+// sponge gradient only from a layer tape and the air gradients only from a divergence tape,
+// the source and receiver tables within the LDS copies.
 inline bool adjoint2_applies(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint_desc &g) {
     const bool sponge = (g.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer;
     const bool air = (g.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0;
     const bool states = g.fwd_count > 0 && (sponge || (air && d.div_tape == nullptr));
     return g.time_block == 2 && has_adjoint_swap(g) && g.lp_swap != nullptr &&
            (d.n_relax == 0 || g.lpsi_swap != nullptr) && !states &&
-           !(air && d.div_tape == nullptr) && d.n_sources <= kFused2Sources &&
-           d.n_receivers <= kFused2Receivers;
+           !(sponge && d.layer_tape == nullptr) && !(air && d.div_tape == nullptr) &&
+           d.n_sources <= kFused2Sources && d.n_receivers <= kFused2Receivers;
 }
 
 template <typename real>
@@ -632,7 +797,9 @@ int64_t launch_adjoint2_quads(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint
     AdjointArgs<real> g;
     fill_adjoint(gd, g);
     const bool air = (gd.grads & ACFDTD_HIP_GRAD_AIR) && d.n_relax > 0 && d.div_tape != nullptr;
-    const Adjoint2Kernel<real> k = pick_adjoint2<real>(d.n_relax, d.has_layer != 0, air);
+    const bool sponge = (gd.grads & ACFDTD_HIP_GRAD_SPONGE) && d.has_layer && d.layer_tape != nullptr;
+    const Adjoint2Kernel<real> k = sponge ? pick_adjoint2_sponge<real>(d.n_relax, air)
+                                          : pick_adjoint2<real>(d.n_relax, d.has_layer != 0, air);
     real *first[6] = {static_cast<real *>(gd.ldiv), static_cast<real *>(gd.lvx),
                       static_cast<real *>(gd.lvy), static_cast<real *>(gd.lvz),
                       static_cast<real *>(gd.lp), static_cast<real *>(gd.lpsi)};

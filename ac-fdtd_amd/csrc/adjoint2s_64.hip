@@ -1,0 +1,10 @@
+// fp64 instantiation of the two-step adjoint sweep carrying the sponge-strength gradient
+// (templates in adjoint2.hpp).
+
+#include "adjoint2.hpp"
+
+namespace acfdtd {
+
+template Adjoint2Kernel<double> pick_adjoint2_sponge<double>(int, bool);
+
+}  // namespace acfdtd

@@ -140,8 +140,8 @@ int batch_validate(const acfdtd_hip_desc *rooms, int32_t n_rooms, int64_t *n_til
             return fail(ACFDTD_HIP_EINVAL, "rooms of a batch must share precision, relaxation "
                                            "count, sponge presence, flags and tile_j");
         if (!has_swap(b)) return fail(ACFDTD_HIP_EINVAL, "batched rooms need swap buffers");
-        if (b.wall_tape || b.div_tape)
-            return fail(ACFDTD_HIP_EINVAL, "batched rooms do not take wall or divergence tapes");
+        if (b.wall_tape || b.div_tape || b.layer_tape)
+            return fail(ACFDTD_HIP_EINVAL, "batched rooms take no wall tapes, divergence tapes or layer tapes");
         total += batch_room_tiling(b, vec).tiles;
     }
     if (total > (int64_t)INT32_MAX - 8) return fail(ACFDTD_HIP_EINVAL, "too many tiles");

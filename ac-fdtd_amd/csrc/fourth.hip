@@ -572,7 +572,7 @@ extern "C" int acfdtd_hip_fourth_run(const acfdtd_hip_desc *desc, double c1, dou
         return fail(ACFDTD_HIP_EINVAL, "the fourth-order sweeps take rigid walls only (images)");
     if (desc->flags != 0)
         return fail(ACFDTD_HIP_EINVAL, "the fourth-order sweeps take no flags");
-    if (desc->wall_tape || desc->div_tape)
+    if (desc->wall_tape || desc->div_tape || desc->layer_tape)
         return fail(ACFDTD_HIP_EINVAL, "the fourth-order sweeps write no tape");
     if (n_steps < 0 || step_offset < 0 || recording_column < 0)
         return fail(ACFDTD_HIP_EINVAL, "n_steps, step_offset and recording_column must be >= 0");

@@ -141,6 +141,7 @@ extern "C" int acfdtd_hip_fused_step(const acfdtd_hip_desc *desc, int64_t step,
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
     if (!div_tape_covers(*desc, step, step))
         return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover this step");
+    if (desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "the layer tape is written by acfdtd_hip_run only");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return fused_once<double>(*desc, step, recording_column, reverse != 0, s);
@@ -172,6 +173,10 @@ extern "C" int acfdtd_hip_run_states(const acfdtd_hip_desc *desc, int64_t step_o
     d.div_tape = nullptr;
     d.div_tape_step0 = 0;
     d.div_tape_rows = 0;
+    d.layer_tape = nullptr;
+    d.layer_tape_step0 = 0;
+    d.layer_tape_rows = 0;
+    d.layer_tape_stride = 0;
     const int status = validate(&d);
     if (status != ACFDTD_HIP_OK) return status;
     g_last_error.clear();

@@ -186,11 +186,12 @@ Fused2Kernel<real> pick_fused2_padded(int nr, bool walls, bool layer, bool comp)
     }
 }
 
-// The plain sweep that also writes the divergence tape, for the widest lanes (V = 4 fp32,
-// 2 fp64) and at least one relaxation state (a tape needs them).
+// The plain sweep that also writes the gradient tapes (divergence, layer), for the widest
+// lanes (V = 4 fp32, 2 fp64); without relaxation states only the layer tape applies.
 template <typename real, int V>
 Fused2Kernel<real> pick_fused2_divt(int nr, bool walls, bool layer, bool comp) {
     switch (nr) {
+        case 0: return pick_fused2_nr<real, V, 0, false, true>(walls, layer, comp);
         case 1: return pick_fused2_nr<real, V, 1, false, true>(walls, layer, comp);
         case 2: return pick_fused2_nr<real, V, 2, false, true>(walls, layer, comp);
         case 3: return pick_fused2_nr<real, V, 3, false, true>(walls, layer, comp);
@@ -216,7 +217,8 @@ extern template Fused2Kernel<double> pick_fused2_padded<double, 2>(int, bool, bo
 // every field it stages 16-byte aligned in both sets (otherwise the plain two-step sweep runs,
 // with the same results).
 inline bool staged_applies(const acfdtd_hip_desc &d, const int vec) {
-    if (!(d.flags & ACFDTD_HIP_STAGED) || vec * d.real_bytes != 16 || d.div_tape || d.z_pitch)
+    if (!(d.flags & ACFDTD_HIP_STAGED) || vec * d.real_bytes != 16 || d.div_tape || d.layer_tape ||
+        d.z_pitch)
         return false;
     const void *fields[] = {d.p, d.vx, d.vy, d.p_swap, d.vx_swap, d.vy_swap};
     for (const void *f : fields)
@@ -239,7 +241,7 @@ Fused2Kernel<real> pick_fused2(const acfdtd_hip_desc &d) {
         if constexpr (sizeof(real) == 4) return pick_fused2_padded<real, 4>(d.n_relax, walls, layer, comp);
         else return pick_fused2_padded<real, 2>(d.n_relax, walls, layer, comp);
     }
-    if (d.div_tape) {
+    if (d.div_tape || d.layer_tape) {
         // fused2_divt_applies: the widest lanes only.
         if constexpr (sizeof(real) == 4) return pick_fused2_divt<real, 4>(d.n_relax, walls, layer, comp);
         else return pick_fused2_divt<real, 2>(d.n_relax, walls, layer, comp);
@@ -279,10 +281,11 @@ void fused2_io(const acfdtd_hip_desc &d, Fused2IO<real> &forward, Fused2IO<real>
     backward.psi_in = forward.psi_out;
 }
 
-// With a divergence tape, and with padded rows, the two-step sweep has kernels for the widest
-// lanes only.
+// With a divergence or layer tape, and with padded rows, the two-step sweep has kernels for the
+// widest lanes only.
 inline bool fused2_divt_applies(const acfdtd_hip_desc &d) {
-    return (d.div_tape == nullptr && d.z_pitch == 0) || fused_vec(d) * d.real_bytes == 16;
+    return (d.div_tape == nullptr && d.layer_tape == nullptr && d.z_pitch == 0) ||
+           fused_vec(d) * d.real_bytes == 16;
 }
 
 template <typename real>

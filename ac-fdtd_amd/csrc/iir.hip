@@ -170,6 +170,7 @@ extern "C" int acfdtd_hip_iir_run(const acfdtd_hip_desc *desc, const acfdtd_hip_
                                   void *stream) {
     int status = validate(desc);
     if (status != ACFDTD_HIP_OK) return status;
+    if (desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "IIR walls write no layer tape");
     if (iir == nullptr) return fail(ACFDTD_HIP_EINVAL, "IIR wall descriptor is NULL");
     if (iir->abi_version != ACFDTD_HIP_ABI_VERSION || iir->struct_size != sizeof(acfdtd_hip_iir_desc))
         return fail(ACFDTD_HIP_EABI, "IIR wall descriptor version or size mismatch");

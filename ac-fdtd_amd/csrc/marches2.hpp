@@ -735,7 +735,9 @@ constexpr int stage_bytes() {
 }
 
 // PADDED: rows may be padded (z_pitch), see fused_march_vec; not with STAGED or a tape.
-// DIVT: owner lanes write both steps' divergences to the divergence tape (a.div_tape).
+// DIVT: owner lanes write both steps' divergences to the divergence tape (a.div_tape, when
+// there is one) and the fields of their layer-frame cells to the layer tape (a.layer_tape,
+// when there is one).
 template <typename real, int V, int NR, bool WALLS, bool LAYER, bool COMPENSATED, bool STAGED,
           bool PADDED = false, bool DIVT = false>
 __device__ __forceinline__ void fused2_march(
@@ -782,6 +784,30 @@ __device__ __forceinline__ void fused2_march(
 
     auto slot = [&](const int plane, const int lane) __attribute__((always_inline)) {
         return lds + ((plane % 3) * T + lane) * V;
+    };
+    // Layer tape: the fields of plane s after step n (o, and vx_lo on the lower x face) for the
+    // lane's cells when they lie in the layer's frame, and the far faces they own.
+    const int64_t lt_pos = DIVT && valid && a.layer_tape ? a.layer.in_frame(j, k0) : -1;
+    auto layer_write = [&](const int s, const int64_t n, const PlaneState<real, V, NR> &o,
+                           const real (&vx_lo)[V]) __attribute__((always_inline)) {
+        if constexpr (DIVT) {
+            if (a.layer_tape == nullptr) return;
+            real *row = a.layer_tape + (n - a.layer_tape_step0) * a.layer_tape_stride;
+            const LayerFrame &f = a.layer;
+            const int64_t q = f.offset(s, j, k0, lt_pos);
+            if (q >= 0) {
+                real lower[V];
+#pragma unroll
+                for (int e = 0; e < V; ++e) lower[e] = o.vz[e];
+                vstore<real, V>(row + q, o.p);
+                vstore<real, V>(row + f.cells + q, vx_lo);
+                vstore<real, V>(row + 2 * f.cells + q, o.vy_lo);
+                vstore<real, V>(row + 3 * f.cells + q, lower);
+            }
+            if (s == nx - 1) vstore<real, V>(row + f.far_x() + (int64_t)j * a.nz + k0, o.vx_hi);
+            if (!c.has_yhi) vstore<real, V>(row + f.far_y() + (int64_t)s * a.nz + k0, o.vy_hi);
+            if (!c.has_zhi) row[f.far_z() + (int64_t)s * a.ny + j] = o.vz[V];
+        }
     };
     auto share = [&](const int plane, const real (&p)[V]) __attribute__((always_inline)) {
         real *d = slot(plane, t);
@@ -917,8 +943,9 @@ __device__ __forceinline__ void fused2_march(
         else run(Interior{});
         if (a.wall_tape && owner && s >= i0 && s < i1)
             tape_cells<real, V, XWALL>(a, c, s, step, t2.tape_vec != 0, tape_value);
-        if (DIVT && owner && s >= i0 && s < i1)
+        if (DIVT && a.div_tape && owner && s >= i0 && s < i1)
             vstore<real, V>(a.div_tape + (step - a.div_tape_step0) * n_cells + L.cell(s), div_value);
+        if (DIVT && owner && s >= i0 && s < i1) layer_write(s, step, o, s1_vx_lo);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             pn_c[e] = in.p_up[e];
@@ -976,7 +1003,9 @@ __device__ __forceinline__ void fused2_march(
         if (plain_xy(i)) run(Wall{});
         else run(Interior{});
         if (a.wall_tape) tape_cells<real, V, XWALL>(a, c, i, step + 1, t2.tape_vec != 0, tape_value);
-        if (DIVT) vstore<real, V>(a.div_tape + (step + 1 - a.div_tape_step0) * n_cells + cc, div_value);
+        if (DIVT && a.div_tape)
+            vstore<real, V>(a.div_tape + (step + 1 - a.div_tape_step0) * n_cells + cc, div_value);
+        if (DIVT) layer_write(i, step + 1, o, s2_vx_lo);
         const int64_t cy = L.face_y(i), cz = L.face_z(i);
         if (i + 1 < i1 || i + 1 == nx) vstore<real, V>(vx_out + cc + L.plane, o.vx_hi);
         vstore<real, V>(vy_out + cy, o.vy_lo);

@@ -215,11 +215,77 @@ template int inplace_steps<float>(const acfdtd_hip_desc &, int64_t, int64_t, int
 template int inplace_steps<double>(const acfdtd_hip_desc &, int64_t, int64_t, int64_t,
                                    hipStream_t);
 
+// Layer tape row `step` from the state fields (pack) or into them (unpack: zeros off the
+// frame), one thread per cell. Owner cells move their lower faces; the last cell of each axis
+// also its far face.
+template <typename real, bool PACK>
+__global__ __launch_bounds__(256) void layer_tape_move(const SweepArgs<real> a, real *row,
+                                                       real *p, real *vx, real *vy, real *vz) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    const LayerFrame &f = a.layer;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(c % nz);
+        const int j = (int)((c / nz) % ny);
+        const int i = (int)(c / ((int64_t)ny * nz));
+        const int64_t q = f.offset(i, j, k, f.in_frame(j, k));
+        const int64_t cy = ((int64_t)i * (ny + 1) + j) * nz + k;
+        const int64_t cz = ((int64_t)i * ny + j) * (nz + 1) + k;
+        if (PACK) {
+            if (q >= 0) {
+                row[q] = p[c];
+                row[f.cells + q] = vx[c];
+                row[2 * f.cells + q] = vy[cy];
+                row[3 * f.cells + q] = vz[cz];
+            }
+            if (i == nx - 1) row[f.far_x() + (int64_t)j * nz + k] = vx[c + (int64_t)ny * nz];
+            if (j == ny - 1) row[f.far_y() + (int64_t)i * nz + k] = vy[cy + nz];
+            if (k == nz - 1) row[f.far_z() + (int64_t)i * ny + j] = vz[cz + 1];
+        } else {
+            p[c] = q >= 0 ? row[q] : real(0);
+            vx[c] = q >= 0 ? row[f.cells + q] : real(0);
+            vy[cy] = q >= 0 ? row[2 * f.cells + q] : real(0);
+            vz[cz] = q >= 0 ? row[3 * f.cells + q] : real(0);
+            if (i == nx - 1) vx[c + (int64_t)ny * nz] = row[f.far_x() + (int64_t)j * nz + k];
+            if (j == ny - 1) vy[cy + nz] = row[f.far_y() + (int64_t)i * nz + k];
+            if (k == nz - 1) vz[cz + 1] = row[f.far_z() + (int64_t)i * ny + j];
+        }
+    }
+}
+
+template <typename real, bool PACK>
+int layer_tape_launch(const acfdtd_hip_desc &d, int64_t step, real *p, real *vx, real *vy,
+                      real *vz, hipStream_t stream) {
+    SweepArgs<real> a;
+    fill_args(d, a);
+    real *row = a.layer_tape + (step - a.layer_tape_step0) * a.layer_tape_stride;
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+    hipLaunchKernelGGL((layer_tape_move<real, PACK>), dim3((unsigned)blocks), dim3(256), 0,
+                       stream, a, row, p, vx, vy, vz);
+    return check_hip(hipGetLastError(), "layer tape launch");
+}
+
 template <typename real>
 int run_steps(const acfdtd_hip_desc &d, int64_t step_offset, int64_t n_steps, int64_t column,
               hipStream_t stream) {
     int64_t n = 0;
     if (n_steps > 3) n = launch_fused2_quads<real>(d, step_offset, n_steps, column, stream);
+    if (d.layer_tape) {
+        // The two-step sweeps write their rows; the other steps run one by one in place, each
+        // followed by its row copied out of the state.
+        for (; n < n_steps; ++n) {
+            int rc = inplace_steps<real>(d, step_offset + n, 1, column + n, stream);
+            if (rc == ACFDTD_HIP_OK)
+                rc = layer_tape_launch<real, true>(d, step_offset + n, static_cast<real *>(d.p),
+                                                   static_cast<real *>(d.vx),
+                                                   static_cast<real *>(d.vy),
+                                                   static_cast<real *>(d.vz), stream);
+            if (rc != ACFDTD_HIP_OK) return rc;
+        }
+        return check_hip(hipGetLastError(), "kernel launch");
+    }
     if (has_swap(d) && n_steps - n > 1)
         n += launch_fused_pairs<real>(d, step_offset + n, n_steps - n, column + n, stream);
     if (n < n_steps) return inplace_steps<real>(d, step_offset + n, n_steps - n, column + n, stream);
@@ -288,6 +354,8 @@ extern "C" int acfdtd_hip_run(const acfdtd_hip_desc *desc, int64_t step_offset, 
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these steps");
     if (!div_tape_covers(*desc, step_offset, step_offset + n_steps - 1))
         return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover these steps");
+    if (!layer_tape_covers(*desc, step_offset, step_offset + n_steps - 1))
+        return fail(ACFDTD_HIP_EINVAL, "layer tape does not cover these steps");
     if (n_steps == 0) return ACFDTD_HIP_OK;
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -299,6 +367,7 @@ extern "C" int acfdtd_hip_run(const acfdtd_hip_desc *desc, int64_t step_offset, 
 extern "C" int acfdtd_hip_velocity_sweep(const acfdtd_hip_desc *desc, void *stream) {
     const int status = validate(desc);
     if (status != ACFDTD_HIP_OK) return status;
+    if (desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "the layer tape is written by acfdtd_hip_run only");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return single_sweep<double>(*desc, false, 0, 0, s);
@@ -318,10 +387,36 @@ extern "C" int acfdtd_hip_pressure_sweep(const acfdtd_hip_desc *desc, int64_t st
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover this step");
     if (!div_tape_covers(*desc, step, step))
         return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover this step");
+    if (desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "the layer tape is written by acfdtd_hip_run only");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8) return single_sweep<double>(*desc, true, step, recording_column, s);
     return single_sweep<float>(*desc, true, step, recording_column, s);
+}
+
+extern "C" int64_t acfdtd_hip_layer_tape_stride(const acfdtd_hip_desc *desc) {
+    if (desc == nullptr || desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return -1;
+    return LayerFrame(desc->nx, desc->ny, desc->nz, desc->layer_lo, desc->layer_hi).stride;
+}
+
+extern "C" int acfdtd_hip_layer_tape_unpack(const acfdtd_hip_desc *desc, int64_t step, void *p,
+                                            void *vx, void *vy, void *vz, void *stream) {
+    const int status = validate(desc);
+    if (status != ACFDTD_HIP_OK) return status;
+    if (!desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "the descriptor has no layer tape");
+    if (!p || !vx || !vy || !vz) return fail(ACFDTD_HIP_EINVAL, "field pointer is NULL");
+    if (step < 0 || !layer_tape_covers(*desc, step, step))
+        return fail(ACFDTD_HIP_EINVAL, "layer tape does not cover this step");
+    g_last_error.clear();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->real_bytes == 8)
+        return layer_tape_launch<double, false>(*desc, step, static_cast<double *>(p),
+                                                static_cast<double *>(vx),
+                                                static_cast<double *>(vy),
+                                                static_cast<double *>(vz), s);
+    return layer_tape_launch<float, false>(*desc, step, static_cast<float *>(p),
+                                           static_cast<float *>(vx), static_cast<float *>(vy),
+                                           static_cast<float *>(vz), s);
 }
 
 extern "C" int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz) {
@@ -456,6 +551,7 @@ extern "C" int acfdtd_hip_fused2_step(const acfdtd_hip_desc *desc, int64_t step,
         return fail(ACFDTD_HIP_EINVAL, "wall tape does not cover these two steps");
     if (!div_tape_covers(*desc, step, step + 1))
         return fail(ACFDTD_HIP_EINVAL, "divergence tape does not cover these two steps");
+    if (desc->layer_tape) return fail(ACFDTD_HIP_EINVAL, "the layer tape is written by acfdtd_hip_run only");
     g_last_error.clear();
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (desc->real_bytes == 8)

@@ -135,6 +135,63 @@ __device__ inline T uniform_load(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// The cells a layer tape holds (the absorbing layer's frame) and where each lies in a row:
+// whole planes i < xl or i >= nx - xh; in the other planes whole rows j < yl or j >= ny - yh,
+// and in the other rows the cells k < zl or k >= nz - zh (zl, zh multiples of 4, so the cells
+// of a lane of 1, 2 or 4 are all in or all out when nz is a multiple of the lane width).
+struct LayerFrame {
+    int nx, ny, nz, xl, xh, yl, yh, zl, zh;
+    int64_t whole, frame, cells, stride;
+    __host__ __device__ LayerFrame() : nx(0), ny(0), nz(0), xl(0), xh(0), yl(0), yh(0), zl(0),
+                                       zh(0), whole(0), frame(0), cells(0), stride(0) {}
+    __host__ __device__ LayerFrame(int nx_, int ny_, int nz_, const int32_t *lo, const int32_t *hi)
+        : nx(nx_), ny(ny_), nz(nz_) {
+        auto clampn = [](int v, int n) { return v < 0 ? 0 : (v > n ? n : v); };
+        xl = clampn(lo[0], nx);
+        xh = clampn(hi[0], nx);
+        if (xl + xh >= nx) xl = nx, xh = 0;
+        yl = clampn(lo[1], ny);
+        yh = clampn(hi[1], ny);
+        zl = (clampn(lo[2], nz) + 3) / 4 * 4;
+        zh = (clampn(hi[2], nz) + 3) / 4 * 4;
+        if (zl + zh >= nz) zl = nz, zh = 0, yl = ny, yh = 0;
+        if (yl + yh >= ny) yl = ny, yh = 0;
+        whole = (int64_t)ny * nz;
+        frame = (int64_t)(yl + yh) * nz + (int64_t)(ny - yl - yh) * (zl + zh);
+        cells = plane_base(nx);
+        const int64_t far = (int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny;
+        stride = (4 * cells + far + 3) / 4 * 4;
+    }
+    __host__ __device__ bool plane_whole(int i) const { return i < xl || i >= nx - xh; }
+    // The first frame cell of plane i.
+    __host__ __device__ int64_t plane_base(int i) const {
+        if (i <= xl) return (int64_t)i * whole;
+        const int mid = nx - xl - xh;
+        const int in_mid = i - xl < mid ? i - xl : mid;
+        const int past = i - (nx - xh) > 0 ? i - (nx - xh) : 0;
+        return (int64_t)xl * whole + (int64_t)in_mid * frame + (int64_t)past * whole;
+    }
+    // Position of cell (j, k) within a plane that is not whole, or -1 outside the frame.
+    __host__ __device__ int64_t in_frame(int j, int k) const {
+        const int mid = ny - yl - yh, zw = zl + zh;
+        if (j < yl) return (int64_t)j * nz + k;
+        if (j >= yl + mid) return (int64_t)yl * nz + (int64_t)mid * zw + (int64_t)(j - yl - mid) * nz + k;
+        const int64_t row = (int64_t)yl * nz + (int64_t)(j - yl) * zw;
+        if (k < zl) return row + k;
+        if (k >= nz - zh) return row + zl + (k - (nz - zh));
+        return -1;
+    }
+    // Offset of cell (i, j, k) in a row's p part (-1 outside the frame), given in_frame(j, k).
+    __host__ __device__ int64_t offset(int i, int j, int k, int64_t frame_pos) const {
+        if (plane_whole(i)) return plane_base(i) + (int64_t)j * nz + k;
+        return frame_pos < 0 ? -1 : plane_base(i) + frame_pos;
+    }
+    // Parts of a row: p, vx, vy, vz of the frame cells, then the far faces.
+    __host__ __device__ int64_t far_x() const { return 4 * cells; }
+    __host__ __device__ int64_t far_y() const { return 4 * cells + (int64_t)ny * nz; }
+    __host__ __device__ int64_t far_z() const { return 4 * cells + (int64_t)ny * nz + (int64_t)nx * nz; }
+};
+
 template <typename real>
 struct SweepArgs {
     Tiling tiles;
@@ -167,6 +224,11 @@ struct SweepArgs {
     // Divergence tape: row n - div_tape_step0 of nx*ny*nz values holds step n's divergences.
     real *div_tape;
     int64_t div_tape_step0;
+    // Layer tape: row n - layer_tape_step0 holds the frame's fields after step n.
+    real *layer_tape;
+    int64_t layer_tape_step0;
+    int64_t layer_tape_stride;
+    LayerFrame layer;
 };
 
 // The layout of a room's fields from its arguments (padded rows or not).
@@ -263,12 +325,22 @@ void fill_args(const acfdtd_hip_desc &d, SweepArgs<real> &a) {
                          d.wall_tape_stride % 4 == 0;
     a.div_tape = static_cast<real *>(d.div_tape);
     a.div_tape_step0 = d.div_tape_step0;
+    a.layer_tape = static_cast<real *>(d.layer_tape);
+    a.layer_tape_step0 = d.layer_tape_step0;
+    a.layer_tape_stride = d.layer_tape_stride;
+    a.layer = LayerFrame(d.nx, d.ny, d.nz, d.layer_lo, d.layer_hi);
 }
 
 // Whether the descriptor's divergence tape holds rows for steps lo .. hi (true without a tape).
 inline bool div_tape_covers(const acfdtd_hip_desc &d, int64_t lo, int64_t hi) {
     return d.div_tape == nullptr || hi < lo ||
            (lo >= d.div_tape_step0 && hi - d.div_tape_step0 < d.div_tape_rows);
+}
+
+// The same for the layer tape.
+inline bool layer_tape_covers(const acfdtd_hip_desc &d, int64_t lo, int64_t hi) {
+    return d.layer_tape == nullptr || hi < lo ||
+           (lo >= d.layer_tape_step0 && hi - d.layer_tape_step0 < d.layer_tape_rows);
 }
 
 // `padded`: the caller's kernels take padded rows (z_pitch > 0); the other entry points refuse
@@ -293,6 +365,17 @@ inline int validate(const acfdtd_hip_desc *d, const bool padded = false) {
             return fail(ACFDTD_HIP_EINVAL, "z_pitch must be 0 or greater than nz");
         if (d->wall_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no wall tape");
         if (d->div_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no divergence tape");
+        if (d->layer_tape) return fail(ACFDTD_HIP_EINVAL, "padded rows take no layer tape");
+    }
+    if (d->layer_tape) {
+        if (!d->has_layer) return fail(ACFDTD_HIP_EINVAL, "a layer tape needs an absorbing layer");
+        const LayerFrame f(d->nx, d->ny, d->nz, d->layer_lo, d->layer_hi);
+        if (d->layer_tape_stride < f.stride || d->layer_tape_stride % 4 != 0 ||
+            d->layer_tape_rows < 0 || d->layer_tape_step0 < 0 ||
+            reinterpret_cast<uintptr_t>(d->layer_tape) % 16 != 0)
+            return fail(ACFDTD_HIP_EINVAL,
+                        "layer tape: 16-byte aligned, stride a multiple of 4 of at least "
+                        "acfdtd_hip_layer_tape_stride, rows from a step >= 0");
     }
     if (d->div_tape && (d->n_relax < 1 || d->div_tape_rows < 0 || d->div_tape_step0 < 0))
         return fail(ACFDTD_HIP_EINVAL,

@@ -305,8 +305,10 @@ def gradient_cost(size, steps, seed=4):
     backward, with walls only (boundary tape, no checkpoints) and with walls and the air
     strengths: from a divergence tape of every step (the default within 64 GiB), and from
     checkpoints every 100 steps with each segment's divergences recomputed (as beyond it); and,
-    in a room lined with the 16-cell sponge on all faces, with walls, sponge and air strengths
-    (checkpoints every 100 steps, each segment's states recomputed into a ring)."""
+    in a room lined with the 16-cell sponge on all faces, with walls, sponge and air strengths:
+    from a layer tape and a divergence tape of every step (the default within 64 GiB), and from
+    checkpoints every 100 steps with each segment's states recomputed into a ring (as beyond
+    it)."""
     import torch
 
     from ac_fdtd_amd import (AIR, AbsorbingLayer, AirAbsorption, Grid, WallAdmittances,
@@ -347,7 +349,8 @@ def gradient_cost(size, steps, seed=4):
     # 64 GiB history budget); walls_and_air_checkpointed forces the path beyond the budget.
     cases = (("walls_tape", False, 64 << 30, False), ("walls_and_air", True, 64 << 30, False),
              ("walls_and_air_checkpointed", True, 0, False),
-             ("walls_sponge_and_air", True, 64 << 30, True))
+             ("walls_sponge_and_air", True, 64 << 30, True),
+             ("walls_sponge_and_air_checkpointed", True, 0, True))
     for key, air, history, sponge in cases:
         room = model(air, history, sponge)
         target = room(torch.tensor(hidden, dtype=torch.float64)).detach()
@@ -375,8 +378,11 @@ def gradient_cost(size, steps, seed=4):
             "recomputed_steps": room.report.recomputed_steps,
             "checkpoints": room.report.checkpoints,
             "divergence_bytes": room.report.div_tape_bytes,
+            "layer_tape_bytes": room.report.layer_tape_bytes,
             "adjoint_two_steps_per_sweep": room.report.adjoint_two_step,
             "path": ("boundary tape" if not air else
+                     "layer tape and divergence tape of every step"
+                     if sponge and room.report.checkpoints == 0 else
                      "checkpoints every 100 steps, states recomputed per segment into a ring"
                      if sponge else "divergence tape of every step"
                      if room.report.checkpoints == 0 else

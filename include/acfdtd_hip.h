@@ -155,6 +155,30 @@ typedef struct acfdtd_hip_desc {
     void *div_tape;
     int64_t div_tape_step0;
     int64_t div_tape_rows;
+
+    /* Optional layer tape, for the sponge-strength gradient (device, working precision, 16-byte
+     * aligned). For absolute step n, row n - layer_tape_step0 (of layer_tape_rows rows,
+     * layer_tape_stride elements each, at least acfdtd_hip_layer_tape_stride(desc) and a
+     * multiple of 4) receives the fields after step n of the cells of the absorbing layer's
+     * frame: p (after the step's source injection), and the velocities of each cell's lower x,
+     * y and z faces. The frame is given by layer_lo / layer_hi, the number of layer cells at
+     * the low and high end of each axis (0 where that face has no layer): planes i < lo[0] or
+     * i >= nx - hi[0] are whole; in the other planes rows j < lo[1] or j >= ny - hi[1] are
+     * whole, and the other rows hold the cells k < zl or k >= nz - zh, zl and zh being lo[2]
+     * and hi[2] rounded up to multiples of 4 (a side whose counts cover the axis makes
+     * everything whole). Row layout: p of the frame cells (C of them, plane by plane, row by row,
+     * k ascending), then vx, vy, vz of the same cells' lower faces (C each), then the far faces:
+     * vx of face nx (ny*nz, j*nz + k), vy of face ny (nx*nz, i*nz + k), vz of face nz (nx*ny,
+     * i*ny + j). Written by acfdtd_hip_run (the other stepping entry points refuse a
+     * descriptor with one); read by the two-step adjoint for the sponge gradient
+     * (acfdtd_hip_adjoint_run) and unpacked by acfdtd_hip_layer_tape_unpack. Needs has_layer;
+     * not with padded rows. No reference counterpart: the reference has no gradients. */
+    void *layer_tape;
+    int64_t layer_tape_step0;
+    int64_t layer_tape_rows;
+    int64_t layer_tape_stride;
+    int32_t layer_lo[3];
+    int32_t layer_hi[3];
 } acfdtd_hip_desc;
 
 /* sizeof(acfdtd_hip_desc), for the caller's mirror check. Replaces acfdtd_struct_size
@@ -226,6 +250,19 @@ void acfdtd_hip_batch_destroy(acfdtd_hip_batch *batch);
  * of 4. */
 int64_t acfdtd_hip_wall_tape_stride(int32_t nx, int32_t ny, int32_t nz);
 
+/* Elements per row of desc's layer tape (see acfdtd_hip_desc::layer_tape): 4 C + ny nz + nx nz
+ * + nx ny rounded up to a multiple of 4, C the frame's cell count from nx, ny, nz, layer_lo
+ * and layer_hi; -1 for a malformed descriptor. No reference counterpart. */
+int64_t acfdtd_hip_layer_tape_stride(const acfdtd_hip_desc *desc);
+
+/* The fields of layer tape row `step` - layer_tape_step0 as full state fields (device, the
+ * shapes of p, vx, vy, vz): the frame cells' p and lower faces and the far faces from the
+ * tape, every other element 0. The sponge gradient reads them only where the grading weights
+ * are nonzero, so such a state stands in for the forward state of that step
+ * (acfdtd_hip_adjoint_desc::fwd_p ...). No reference counterpart. */
+int acfdtd_hip_layer_tape_unpack(const acfdtd_hip_desc *desc, int64_t step, void *p, void *vx,
+                                 void *vy, void *vz, void *stream);
+
 /* Energy terms of the stored state, in fp64 on the device: out[0] = sum p^2,
  * out[1] = sum over all faces of v^2, out[2] = sum over interior faces of v_a (p_hi - p_lo).
  * `out` is a device pointer to 3 doubles; `workspace` a device buffer of at least
@@ -264,8 +301,8 @@ int acfdtd_hip_stream_copy(const void *src, void *dst, size_t bytes, void *strea
  * per wavefront: [0..5] dL/da summed over the cells of each face plane (FACES order),
  * [6] dL/dsigma_max, [7] dL/drelax_scale, [8..11] dL/dgain_r. */
 #define ACFDTD_HIP_GRAD_WALLS 1u  /* needs the forward boundary tape (desc->wall_tape) */
-#define ACFDTD_HIP_GRAD_SPONGE 2u /* needs fwd_* and the grading weights */
-#define ACFDTD_HIP_GRAD_AIR 4u    /* needs fwd_vx / fwd_vy / fwd_vz */
+#define ACFDTD_HIP_GRAD_SPONGE 2u /* needs the grading weights, and fwd_* or desc->layer_tape */
+#define ACFDTD_HIP_GRAD_AIR 4u    /* needs fwd_vx / fwd_vy / fwd_vz or desc->div_tape */
 
 typedef struct acfdtd_hip_adjoint_desc {
     uint32_t abi_version; /* ACFDTD_HIP_ABI_VERSION */
@@ -305,9 +342,12 @@ typedef struct acfdtd_hip_adjoint_desc {
     int64_t fwd_count;
     /* acfdtd_hip_adjoint_run with time_block = 2: second lp and lpsi (same shapes). With them
      * and the swap set above, the leading multiple of four adjoint steps whose local parts all
-     * exist (steps >= 1) run two per sweep when only wall and signal gradients are asked for,
-     * nz is a multiple of 4 (fp32) or 2 (fp64) and there are at most 64 sources and 256
-     * receivers; the other steps one per sweep. The results are the same bits (the fp64
+     * exist (steps >= 1) run two per sweep when no forward states are given (fwd_count 0: wall
+     * and signal gradients, air gradients from desc->div_tape, the sponge gradient from
+     * desc->layer_tape), nz is a multiple of 4 (fp32) or 2 (fp64) and there are at most 64
+     * sources and 256 receivers; the other steps one per sweep. With a layer tape and no
+     * forward states every step of the call must run two per sweep: step_lo >= 1 and a
+     * multiple of four steps, else ACFDTD_HIP_EINVAL. The results are the same bits (the fp64
      * partials summed in another order); the state ends in the first set. */
     void *lp_swap, *lpsi_swap;
 } acfdtd_hip_adjoint_desc;

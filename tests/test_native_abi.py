@@ -173,6 +173,78 @@ def test_divergence_tapes_are_checked_before_any_launch(lib):
     assert "divergence tapes" in lib.acfdtd_hip_last_error().decode()
 
 
+def _frame_cells(shape, lo, hi):
+    """Cells of a layer tape's frame, counted by brute force over a mask: whole planes and rows
+    at the layer's x and y ends, z cells within lo[2] and hi[2] rounded up to whole groups of
+    4, everything when an axis's ends meet."""
+    nx, ny, nz = shape
+    zl, zh = -(-lo[2] // 4) * 4, -(-hi[2] // 4) * 4
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    in_x = (i < lo[0]) | (i >= nx - hi[0]) if lo[0] + hi[0] < nx else np.ones_like(i, bool)
+    if zl + zh >= nz:
+        in_yz = np.ones_like(i, bool)
+    else:
+        in_y = (j < lo[1]) | (j >= ny - hi[1]) if lo[1] + hi[1] < ny else np.ones_like(i, bool)
+        in_yz = in_y | (k < zl) | (k >= nz - zh)
+    return int((in_x | in_yz).sum())
+
+
+@pytest.mark.parametrize(("shape", "lo", "hi"), [
+    ((20, 14, 12), (3, 3, 3), (3, 3, 3)),
+    ((16, 16, 16), (4, 0, 0), (0, 0, 4)),
+    ((9, 7, 6), (2, 1, 0), (1, 0, 1)),
+    ((10, 9, 8), (0, 0, 3), (0, 0, 3)),     # z ends meet after rounding: whole planes
+    ((6, 5, 12), (4, 3, 0), (3, 2, 0)),     # x and y ends meet
+])
+def test_layer_tape_stride_counts_the_frame(lib, shape, lo, hi):
+    d = _desc(nx=shape[0], ny=shape[1], nz=shape[2])
+    for a in range(3):
+        d.layer_lo[a], d.layer_hi[a] = lo[a], hi[a]
+    nx, ny, nz = shape
+    far = ny * nz + nx * nz + nx * ny
+    want = -(-(4 * _frame_cells(shape, lo, hi) + far) // 4) * 4
+    assert lib.acfdtd_hip_layer_tape_stride(ctypes.byref(d)) == want
+    assert lib.acfdtd_hip_layer_tape_stride(None) == -1
+
+
+def test_layer_tapes_are_checked_before_any_launch(lib):
+    """A layer tape needs the absorbing layer, alignment, a long enough stride and a row for
+    every step acfdtd_hip_run advances; the entry points that would not write it refuse it."""
+    from ac_fdtd_amd.batch import _bind
+
+    _bind(lib)
+    layer = {"has_layer": 1, "cell": (ctypes.c_void_p * 3)(4096, 4096, 4096),
+             "face": (ctypes.c_void_p * 3)(4096, 4096, 4096)}
+    stride = lib.acfdtd_hip_layer_tape_stride(ctypes.byref(_desc()))
+    tape = {"layer_tape": 4096, "layer_tape_step0": 10, "layer_tape_rows": 4,
+            "layer_tape_stride": stride}
+    assert lib.acfdtd_hip_run(ctypes.byref(_desc(**tape)), 10, 1, 0, None) == EINVAL
+    assert "absorbing layer" in lib.acfdtd_hip_last_error().decode()
+    for bad in ({"layer_tape": 4100}, {"layer_tape_stride": stride - 4},
+                {"layer_tape_stride": stride + 2}):
+        desc = _desc(**layer, **{**tape, **bad})
+        assert lib.acfdtd_hip_run(ctypes.byref(desc), 10, 1, 0, None) == EINVAL
+        assert "layer tape" in lib.acfdtd_hip_last_error().decode()
+    desc = _desc(**layer, **tape)
+    for offset, n in ((9, 1), (10, 5), (13, 2)):
+        assert lib.acfdtd_hip_run(ctypes.byref(desc), offset, n, 0, None) == EINVAL
+        assert "layer tape does not cover" in lib.acfdtd_hip_last_error().decode()
+    swaps = {"p_swap": 4096, "vx_swap": 4096, "vy_swap": 4096, "vz_swap": 4096, "time_block": 2}
+    desc = _desc(**layer, **tape, **swaps)
+    assert lib.acfdtd_hip_velocity_sweep(ctypes.byref(desc), None) == EINVAL
+    assert lib.acfdtd_hip_pressure_sweep(ctypes.byref(desc), 10, 0, None) == EINVAL
+    assert lib.acfdtd_hip_fused_step(ctypes.byref(desc), 10, 0, 0, None) == EINVAL
+    assert lib.acfdtd_hip_fused2_step(ctypes.byref(desc), 10, 0, 0, None) == EINVAL
+    assert "acfdtd_hip_run only" in lib.acfdtd_hip_last_error().decode()
+    rooms = (_native.Desc * 1)(desc)
+    assert lib.acfdtd_hip_batch_workspace_bytes(rooms, 1) == 0
+    assert "layer tapes" in lib.acfdtd_hip_last_error().decode()
+    fields = [ctypes.c_void_p(4096)] * 4
+    assert lib.acfdtd_hip_layer_tape_unpack(ctypes.byref(desc), 14, *fields, None) == EINVAL
+    assert lib.acfdtd_hip_layer_tape_unpack(ctypes.byref(_desc(**layer)), 0, *fields, None) == EINVAL
+    assert "no layer tape" in lib.acfdtd_hip_last_error().decode()
+
+
 def test_energy_without_allocation_refuses_grids_beyond_its_scratch(lib):
     """ceil(ny / 8) * ceil(nz / 64) workgroup columns must fit the fixed scratch; refused before
     any HIP call."""
