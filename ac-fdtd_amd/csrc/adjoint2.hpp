@@ -698,8 +698,8 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     const bool owner = valid && r >= 0 && r < t2.rows && sl >= 0 && sl < t2.seg;
     const int j = valid ? jj : 0;
     const int k0 = valid ? gg * V : 0;
-    const int i0 = xc * t2.chunk;
-    const int i1 = min(i0 + t2.chunk, a.nx);
+    int i0, i1;
+    chunk_planes(t2, blk, xc, a.nx, i0, i1);
     // adjoint2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
     for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
         src_lds[2 * q] = a.source_cell[2 * q];

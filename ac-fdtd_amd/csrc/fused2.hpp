@@ -50,8 +50,8 @@ __device__ __forceinline__ void fused2_block(const SweepArgs<real> &a, const Fus
     const bool owner = valid && r >= 0 && r < t2.rows && sl >= 0 && sl < t2.seg;
     const int j = valid ? jj : 0;
     const int k0 = valid ? gg * V : 0;
-    const int i0 = xc * t2.chunk;
-    const int i1 = min(i0 + t2.chunk, a.nx);
+    int i0, i1;
+    chunk_planes(t2, blk, xc, a.nx, i0, i1);
     // fused2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
     for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
         src_lds[2 * q] = a.source_cell[2 * q];
@@ -119,8 +119,8 @@ __global__ __launch_bounds__(kFused2Threads, ACFDTD_FUSED2_MIN_WAVES) void fused
     const bool owner = valid && r >= 0 && r < t2.rows && sl >= 0 && sl < t2.seg;
     const int j = valid ? jj : 0;
     const int k0 = valid ? gg * V : 0;
-    const int i0 = xc * t2.chunk;
-    const int i1 = min(i0 + t2.chunk, a.nx);
+    int i0, i1;
+    chunk_planes(t2, blk, xc, a.nx, i0, i1);
     // fused2_applies admits at most kFused2Sources sources and kFused2Receivers receivers.
     for (int q = t; q < a.n_sources; q += (int)blockDim.x) {
         src_lds[2 * q] = a.source_cell[2 * q];
@@ -295,6 +295,7 @@ int64_t launch_fused2_quads(const acfdtd_hip_desc &d, int64_t step_offset, int64
     const Fused2Kernel<real> k = pick_fused2<real>(d);
     Tiling2 t2;
     if (!make_tiling2(d, fused_vec(d), t2, reinterpret_cast<const void *>(k))) return 0;
+    balance_tiling2(t2, d.nx);
     SweepArgs<real> a;
     fill_args(d, a);
     Fused2IO<real> forward, backward;
@@ -319,6 +320,7 @@ int launch_fused2_once(const acfdtd_hip_desc &d, int64_t step, int64_t column, b
         fused2_applies(d) && fused2_divt_applies(d) ? pick_fused2<real>(d) : nullptr;
     if (k == nullptr || !make_tiling2(d, fused_vec(d), t2, reinterpret_cast<const void *>(k)))
         return fail(ACFDTD_HIP_EINVAL, "the two-step sweep does not apply to this descriptor");
+    balance_tiling2(t2, d.nx);
     SweepArgs<real> a;
     fill_args(d, a);
     Fused2IO<real> forward, backward;

@@ -277,10 +277,27 @@ struct Tiling2 {
     int zhalo;    // 1 when a segment is shorter than the row, else 0
     int pitch;    // S + 2 zhalo: threads per block row
     int threads;  // (R + 2) * pitch, rounded up to whole wavefronts (<= kFused2Threads)
-    int chunk;    // x planes per march
+    int chunk;    // x planes per march (fixed chunks)
     int nblk_g, nblk_j, nblk, nchunk, tiles, launched;
     int tape_vec;  // boundary tape rows start V-aligned: x and y face rows as vector stores
+    // Balanced chunks (balance_tiling2) when >= 0: row blocks below `extra` split x into
+    // nchunk + 1 chunks, the others into nchunk, each of nearly equal length; -1: fixed chunks
+    // of `chunk` planes (the last one shorter).
+    int extra;
 };
+
+// The x planes [i0, i1) that chunk xc of row block blk marches.
+__host__ __device__ inline void chunk_planes(const Tiling2 &t, const int blk, const int xc,
+                                             const int nx, int &i0, int &i1) {
+    if (t.extra < 0) {
+        i0 = xc * t.chunk;
+        i1 = i0 + t.chunk < nx ? i0 + t.chunk : nx;
+        return;
+    }
+    const int64_t n = t.nchunk + (blk < t.extra ? 1 : 0);
+    i0 = (int)((int64_t)xc * nx / n);
+    i1 = (int)((int64_t)(xc + 1) * nx / n);
+}
 
 // Whether acfdtd_hip_run may use the two-step sweep for this descriptor.
 inline bool fused2_applies(const acfdtd_hip_desc &d) {
@@ -418,7 +435,28 @@ inline bool make_tiling2(const acfdtd_hip_desc &d, int vec, Tiling2 &t,
     t.tape_vec = d.wall_tape != nullptr &&
                  reinterpret_cast<uintptr_t>(d.wall_tape) % ((size_t)vec * d.real_bytes) == 0 &&
                  d.wall_tape_stride % vec == 0;
+    t.extra = -1;
     return true;
+}
+
+// Workgroups of a two-step sweep run one per CU (8 wavefronts of up to 256 VGPRs) or a few
+// (smaller blocks), in rounds. When the tiles overrun a whole number of rounds, the last round
+// leaves CUs idle while its workgroups march; instead, split x into as many chunks as fill whole
+// rounds, every row block into nchunk or nchunk + 1 chunks of nearly equal length (the first
+// `extra` row blocks one more). The tile ids keep their order (row block fastest), so the
+// workgroups of one XCD still share their halo rows in its L2. Unchanged when the tiles fit one
+// round or the chunks would fall below two planes.
+inline void balance_tiling2(Tiling2 &t, const int nx) {
+    const int waves = t.threads / 64;
+    const int64_t units = (int64_t)device_compute_units() * (waves >= 8 ? 1 : 8 / waves);
+    if (t.tiles <= units) return;
+    const int64_t total = (t.tiles + units - 1) / units * units;
+    const int64_t q = total / t.nblk, r = total % t.nblk;
+    if (q < 1 || nx / (q + (r > 0 ? 1 : 0)) < 2 || total > (int64_t)INT32_MAX - 8) return;
+    t.nchunk = (int)q;
+    t.extra = (int)r;
+    t.tiles = (int)total;
+    t.launched = ceil_div(total, 8) * 8;
 }
 
 // Per-lane constants of the step: position, neighbour existence, sponge factors, wall classes.
