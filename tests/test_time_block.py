@@ -184,3 +184,31 @@ def test_two_step_boundary_tape_is_bitwise_single_steps(dtype, case, pad):
     stride = 2 * (ny * nz + nx * nz + nx * ny)  # the row's written length; the stride pads it
     assert one.wall_tape_stride() >= stride and one.wall_tape_stride() % 4 == 0
     assert not torch.any(tapes[1][:, :stride] == -7.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("config", ["walls", "all"])
+def test_balanced_chunks_are_bitwise_single_steps(dtype, config):
+    """A single room whose two-step tiles overrun a whole round of workgroups runs in balanced x
+    chunks (row blocks in nchunk or nchunk + 1 chunks of 2 to 3 planes here, chunk edges at
+    uneven planes): the same bits as single steps, boundary tape included. 62 row blocks of 2
+    rows (one wavefront per workgroup) in 34 chunks of 6 planes are 2108 tiles, more than the
+    8 one-wavefront workgroups per CU that run at once."""
+    shape, tile_j, chunk_x = (200, 124, 32), 1, 6
+    units = torch.cuda.get_device_properties(0).multi_processor_count * 8
+    tiles = -(-shape[1] // 2) * -(-shape[0] // chunk_x)
+    assert tiles > units, "the case must overrun one round to exercise the balanced chunks"
+    one = _solver(shape, dtype, CONFIGS[config], 1, tile_j, chunk_x)
+    two = _solver(shape, dtype, CONFIGS[config], 2, tile_j, chunk_x)
+    tapes = None
+    if config == "all":
+        tapes = [torch.full((30, s.wall_tape_stride()), -7.0, dtype=s.dtype, device=s.device)
+                 for s in (one, two)]
+        one.attach_wall_tape(tapes[0], 0)
+        two.attach_wall_tape(tapes[1], 0)
+    for n_steps in (8, 5, 4, 7, 6):
+        one.run(n_steps)
+        two.run(n_steps)
+    _assert_same(one, two)
+    if tapes is not None:
+        assert torch.equal(tapes[0], tapes[1])
