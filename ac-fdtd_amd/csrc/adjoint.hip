@@ -382,8 +382,12 @@ extern "C" size_t acfdtd_hip_adjoint_partials_count(const acfdtd_hip_desc *desc)
     const Tiling t = tiling_for(*desc);
     int64_t waves = std::max<int64_t>((int64_t)t.tiles * t.tile_j, adjoint_fused_waves(*desc));
     Tiling2 t2;
-    if (make_tiling2(*desc, desc->real_bytes == 8 ? 2 : 4, t2))
+    if (make_tiling2(*desc, desc->real_bytes == 8 ? 2 : 4, t2)) {
+        // The two-step adjoint runs balanced chunks (launch_adjoint2_quads): as many tiles or
+        // more.
+        balance_tiling2(t2, desc->nx);
         waves = std::max<int64_t>(waves, (int64_t)t2.tiles * (t2.threads / 64));
+    }
     return (size_t)waves * kGradSlots;
 }
 

@@ -792,6 +792,7 @@ int64_t launch_adjoint2_quads(const acfdtd_hip_desc &d, const acfdtd_hip_adjoint
                               int64_t step_hi, int64_t n_steps, hipStream_t stream) {
     Tiling2 t2;
     if (!adjoint2_applies(d, gd) || !make_tiling2(d, sizeof(real) == 4 ? 4 : 2, t2)) return 0;
+    balance_tiling2(t2, d.nx);
     SweepArgs<real> a;
     fill_args(d, a);
     AdjointArgs<real> g;

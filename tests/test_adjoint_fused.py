@@ -80,7 +80,10 @@ def test_fused_adjoint_is_bitwise_the_two_sweep_adjoint(dtype, shape, mode):
 # (shape, tile_j, chunk_x): workgroup edges inside the grid along x and y; the last case
 # splits rows of 32 fp32 lane groups into segments (10 rows of 32 do not fit 512 lanes).
 TWO_STEP_CASES = [((20, 14, 12), 1, 3), ((13, 22, 16), 2, 5), ((9, 30, 64), 3, 4),
-                  ((11, 37, 128), 8, 4)]
+                  ((11, 37, 128), 8, 4),
+                  # 62 row blocks of 2 rows in 34 chunks overrun a round of one-wavefront
+                  # workgroups on 256 CUs: balanced chunks of 2 to 3 planes
+                  ((200, 124, 32), 1, 6)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
