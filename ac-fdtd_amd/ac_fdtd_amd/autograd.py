@@ -81,13 +81,15 @@ class RoomResponse:
         n_steps: steps per evaluation.
         sources: ``[(point, signal), ...]``; the signals are the defaults for ``signal``.
         receivers: receiver points.
-        checkpoint_every: state checkpoint interval K, used when the sponge gradient is
-            requested (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field sets), and
-            for the air gradients beyond ``history_bytes`` (``ceil(n_steps / K)`` full states
-            plus ``K`` divergence rows).
-        history_bytes: device memory the forward pass may keep for the air gradients: a
-            divergence tape of ``n_steps * n_cells`` values when it fits (default 64 GiB, of
-            the MI355X's 288 GB), else checkpoints and recomputation.
+        checkpoint_every: state checkpoint interval K, used beyond ``history_bytes``: for the
+            sponge gradient (memory: ``ceil(n_steps / K)`` full states plus ``K + 1`` field
+            sets) and for the air gradients (``ceil(n_steps / K)`` full states plus ``K``
+            divergence rows).
+        history_bytes: device memory the forward pass may keep for the volumetric gradients
+            (default 64 GiB, of the MI355X's 288 GB): a divergence tape of ``n_steps * n_cells``
+            values for the air strengths, and with the sponge strength a layer tape of
+            ``n_steps`` rows (``solver.layer_tape_stride()`` values each) as well, when they fit;
+            else checkpoints and recomputation.
         autotune: pick the tilings of the forward and of the adjoint step by timing them
             (:mod:`ac_fdtd_amd.autotune`); gradients then differ from an untuned run only in
             the summation order of their fp64 partials.
