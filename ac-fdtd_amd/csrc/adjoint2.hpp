@@ -44,6 +44,17 @@ struct Adjoint2IO {
     real *ld_out, *lvx_out, *lvy_out, *lvz_out, *lp_out, *lpsi_out;
 };
 
+// LDS layout of the block's grading weights (SPONGE): z cell weights, z face weights, y cell
+// weights, y face weights. They are read where the sponge terms need them rather than held in
+// registers through the march (the two-step adjoint has none to spare, and an LDS read does
+// not wait for the vector memory operations in flight).
+template <typename real, int V>
+struct WeightLds {
+    static constexpr int kZ = kFused2Threads * V + 8;
+    static constexpr int kY = 2 * 8 + 4;  // rows of a block (at most 16) and its halo rows
+    static constexpr int zc = 0, zf = kZ, yc = 2 * kZ, yf = 2 * kZ + kY, size = 2 * kZ + 2 * kY;
+};
+
 // Per-lane constants, as adjoint_march_vec computes them before its march.
 template <typename real, int V>
 struct AdjLane {
@@ -60,22 +71,25 @@ struct AdjLane {
     bool wall_in[V];
     real fu_in[V], fp_in[V];
     real cdiv;
-    // SPONGE: the grading weights of the lane's rows and cells, and its place in the frame
-    // planes of the layer tape (-1 outside the frame).
-    real wyc, wyf_lo, wyf_hi, wzc[V], wzf[V + 1];
+    // SPONGE: the grading weights of the lane's rows and cells (in the block's LDS copy, see
+    // WeightLds), and its place in the frame planes of the layer tape (-1 outside the frame).
+    const real *wl;  // LDS weights of the block
+    int wy, wz;      // the lane's row and first cell in them
     int64_t lt_pos;
+    __device__ real w_yc() const { return wl[WeightLds<real, V>::yc + wy]; }
+    __device__ real w_yf(const int h) const { return wl[WeightLds<real, V>::yf + wy + h]; }
+    __device__ real w_zc(const int e) const { return wl[WeightLds<real, V>::zc + wz + e]; }
+    __device__ real w_zf(const int e) const { return wl[WeightLds<real, V>::zf + wz + e]; }
 };
 
+
 template <typename real, int V>
-__device__ __forceinline__ void adj_lane_sponge(const SweepArgs<real> &a, const AdjointArgs<real> &g,
-                                                const int j, const int k0, AdjLane<real, V> &c) {
-    c.wyc = g.w_cell[1][j];
-    c.wyf_lo = g.w_face[1][j];
-    c.wyf_hi = g.w_face[1][j + 1];
-#pragma unroll
-    for (int e = 0; e < V; ++e) c.wzc[e] = g.w_cell[2][k0 + e];
-#pragma unroll
-    for (int e = 0; e <= V; ++e) c.wzf[e] = g.w_face[2][k0 + e];
+__device__ __forceinline__ void adj_lane_sponge(const SweepArgs<real> &a, const real *wl,
+                                                const int wy, const int wz, const int j,
+                                                const int k0, AdjLane<real, V> &c) {
+    c.wl = wl;
+    c.wy = wy;
+    c.wz = wz;
     c.lt_pos = a.layer.in_frame(j, k0);
 }
 
@@ -237,18 +251,18 @@ __device__ __forceinline__ void adjoint_cells(
                 vload<real, V>(lt_v + 3 * f.cells + lt_q, fz);
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
-                    s += (double)(wxc + c.wyf_lo + c.wzc[e]) * s_ty_lo[e] * fy[e];
-                    s += (double)(wxc + c.wyc + c.wzf[e]) * s_tz[e] * fz[e];
+                    s += (double)(wxc + c.w_yf(0) + c.w_zc(e)) * s_ty_lo[e] * fy[e];
+                    s += (double)(wxc + c.w_yc() + c.w_zf(e)) * s_tz[e] * fz[e];
                 }
             }
             if (!c.has_yhi) {
                 real fy[V];
                 vload<real, V>(lt_v + f.far_y() + (int64_t)i * a.nz + c.k0, fy);
 #pragma unroll
-                for (int e = 0; e < V; ++e) s += (double)(wxc + c.wyf_hi + c.wzc[e]) * s_ty_hi[e] * fy[e];
+                for (int e = 0; e < V; ++e) s += (double)(wxc + c.w_yf(1) + c.w_zc(e)) * s_ty_hi[e] * fy[e];
             }
             if (!c.has_zhi)
-                s += (double)(wxc + c.wyc + c.wzf[V]) * s_tz[V] *
+                s += (double)(wxc + c.w_yc() + c.w_zf(V)) * s_tz[V] *
                      lt_v[f.far_z() + (int64_t)i * a.ny + c.j];
             sp_acc += mdt * s;
         }
@@ -262,7 +276,7 @@ __device__ __forceinline__ void adjoint_cells(
                 vload<real, V>(lt_v + q1, fx);
                 double s = 0;
 #pragma unroll
-                for (int e = 0; e < V; ++e) s += (double)(wxf + c.wyc + c.wzc[e]) * s_tx[e] * fx[e];
+                for (int e = 0; e < V; ++e) s += (double)(wxf + c.w_yc() + c.w_zc(e)) * s_tx[e] * fx[e];
                 sp_acc += mdt * s;
             }
         }
@@ -318,7 +332,7 @@ __device__ __forceinline__ void adjoint_cells(
     for (int e = 0; e < V; ++e) {
         const real lpe = lpc[e];
         if (SPONGE && own && lt_q >= 0)
-            sp_acc += mdt * (double)(wxc + (c.wyc + c.wzc[e])) * (double)lpe *
+            sp_acc += mdt * (double)(wxc + (c.w_yc() + c.w_zc(e))) * (double)lpe *
                       (double)(bp[e] - injected[e]);
         const real lhat = LAYER ? ((lpe * fxc) * c.fyc) * c.fzc[e] : lpe;
         real ltil = lhat, lp_new = 0;
@@ -400,7 +414,7 @@ __device__ __forceinline__ void adjoint2_march(
     const bool want_walls, real *grad_signal, const Adjoint2IO<real> &io, real *__restrict__ lds,
     const int32_t *src_cell, const int64_t *src_len, const int32_t *rx_cell, const real *rx_now,
     const real *wtab, double (&acc)[6], double (&air_acc)[1 + (NR > 0 ? NR : 1)],
-    const AdjointArgs<real> &g, double &sp_acc) {
+    const AdjointArgs<real> &g, double &sp_acc, const real *wl, const int wy, const int wz) {
     static_assert(!SPONGE || LAYER, "the sponge gradient needs the layer");
     using Wall = std::integral_constant<bool, true>;
     using Interior = std::integral_constant<bool, false>;
@@ -416,7 +430,7 @@ __device__ __forceinline__ void adjoint2_march(
     const real *__restrict__ lpsi_in = io.lpsi_in;
     AdjLane<real, V> c;
     if (valid) adj_lane<real, V, NR, LAYER>(a, j, k0, src_cell, src_len, rx_cell, rx_now, wtab, c);
-    if (SPONGE && valid) adj_lane_sponge<real, V>(a, g, j, k0, c);
+    if (SPONGE && valid) adj_lane_sponge<real, V>(a, wl, wy, wz, j, k0, c);
     // The lane constants' loads complete on every path into the march (see fused2_march).
     wait_memory();
     // SPONGE: layer tape rows of forward steps m, m - 1, m - 2 (step A's faces read row m and
@@ -438,7 +452,7 @@ __device__ __forceinline__ void adjoint2_march(
                 vload<real, V>(row + f.cells + q, fx);
                 double sum = 0;
 #pragma unroll
-                for (int e = 0; e < V; ++e) sum += (double)(wxf + c.wyc + c.wzc[e]) * w[e] * fx[e];
+                for (int e = 0; e < V; ++e) sum += (double)(wxf + c.w_yc() + c.w_zc(e)) * w[e] * fx[e];
                 sp_acc += -(double)g.dt * sum;
             }
         }
@@ -681,6 +695,7 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     __shared__ int32_t rx_lds[2 * kFused2Receivers];
     __shared__ real rxg_lds[2 * kFused2Receivers];
     __shared__ real wt_lds[2 * ACFDTD_HIP_WALL_CODES];
+    __shared__ real w_lds[SPONGE ? WeightLds<real, V>::size : 1];
     const unsigned n = (unsigned)t2.launched;
     unsigned b = blockIdx.x;
     b = (b & 7u) * (n >> 3) + (b >> 3);  // contiguous tiles per XCD, as decode_tile
@@ -688,6 +703,23 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     const int blk = (int)(b % (unsigned)t2.nblk);
     const int xc = (int)(b / (unsigned)t2.nblk);
     const int bg = blk % t2.nblk_g, bj = blk / t2.nblk_g;
+    // SPONGE: the grading weights of the block's rows (one halo row each side) and cells.
+    const int kbase = max(0, (bg * t2.seg - t2.zhalo) * V);
+    const int jbase = bj * t2.rows - 1;
+    if (SPONGE) {
+        using W = WeightLds<real, V>;
+        const int nzc = min(W::kZ - 1, (t2.seg + 2 * t2.zhalo) * V);
+        for (int q = (int)threadIdx.x; q <= nzc; q += (int)blockDim.x) {
+            const int k = kbase + q;
+            w_lds[W::zc + q] = k < a.nz ? g.w_cell[2][k] : real(0);
+            w_lds[W::zf + q] = k <= a.nz ? g.w_face[2][k] : real(0);
+        }
+        for (int q = (int)threadIdx.x; q < t2.rows + 3 && q < W::kY; q += (int)blockDim.x) {
+            const int jr = jbase + q;
+            w_lds[W::yc + q] = jr >= 0 && jr < a.ny ? g.w_cell[1][jr] : real(0);
+            w_lds[W::yf + q] = jr >= 0 && jr <= a.ny ? g.w_face[1][jr] : real(0);
+        }
+    }
     const int t = (int)threadIdx.x;
     const int r = t / t2.pitch - 1;
     const int sl = t % t2.pitch - t2.zhalo;
@@ -723,7 +755,8 @@ __global__ __launch_bounds__(kFused2Threads, 2) void adjoint2_step(
     double sp_acc = 0;
     adjoint2_march<real, V, NR, LAYER, AIR, SPONGE>(
         a, t2, i0, i1, j, k0, valid, owner, t, m, want_walls, g.grad_signal, io, lds, src_lds,
-        len_lds, rx_lds, rxg_lds, walls ? wt_lds : nullptr, acc, air_acc, g, sp_acc);
+        len_lds, rx_lds, rxg_lds, walls ? wt_lds : nullptr, acc, air_acc, g, sp_acc, w_lds,
+        jj - jbase, k0 - kbase);
     const int64_t row = (int64_t)b * (t2.threads / 64) + t / 64;
     if (want_walls) flush_wall_rows(acc, row, g.partials);
     if (AIR) flush_air_rows(air_acc, 1 + NR, row, g.partials);
